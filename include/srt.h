@@ -1,0 +1,217 @@
+/*
+ * srt.h - C ABI of the MI355X (gfx950) sparse-register-tiling SpMM library
+ * (libsrt.so, built from sparse-register-tiling_amd/csrc).
+ *
+ * C = A_sparse * B_dense, A packed into M_r-row panels whose columns are
+ * grouped by nonzero pattern ("nano-kernel" pattern), B and C dense row-major
+ * fp32 in device memory.
+ *
+ * Conventions
+ *   - Every call returns an int status: SRT_OK (0) or an SRT_ERR_* code; the
+ *     message of the last failure on the calling thread is srt_last_error().
+ *     Nothing in the library calls exit() (the reference exits on errors:
+ *     cpp_testbed/src/spmm/nano/SpMM_SOP.h:177, template:300-303).
+ *   - Pointers documented "device" are device (HBM) pointers on the current
+ *     HIP device; "host" pointers are ordinary host memory.
+ *   - Work is enqueued on the caller's stream (srt_stream_t is hipStream_t;
+ *     NULL = the default stream).  Calls that must read sizes back (plan
+ *     creation, pack_tile) synchronise that stream before returning.
+ *   - Plans and packed tiles are immutable after creation; concurrent
+ *     executes of one plan on different streams are safe.
+ *   - CSR inputs: row pointer and column index arrays of index_bytes = 4
+ *     (int32) or 8 (int64) bytes each, zero-based, no duplicate entries.
+ */
+#ifndef SRT_H_
+#define SRT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *srt_stream_t; /* == hipStream_t */
+
+enum {
+    SRT_OK = 0,
+    SRT_ERR_INVALID = 1,     /* bad argument / shape / state */
+    SRT_ERR_UNSUPPORTED = 2, /* no executor compiled for a pattern set, etc. */
+    SRT_ERR_HIP = 3,         /* a HIP runtime call failed */
+    SRT_ERR_IO = 4,          /* mapping file not found / unreadable */
+    SRT_ERR_NOMEM = 5        /* device or host allocation failed */
+};
+
+/* Zero-pattern sentinel id (tools/codegen/gen_vector_specialized.py:42 value). */
+#define SRT_ZERO_PATTERN_ID 32768
+
+/* Maximum panel height (pattern bitmask width) the library accepts. */
+#define SRT_MAX_MR 16
+
+const char *srt_last_error(void);
+int srt_version(void);
+
+/* ------------------------------------------------------------------------
+ * Tiling configuration.  Mirrors sop::TileConfig as set by
+ * cpp_testbed/src/spmm/nano/SpMM_SOP.h:107-114 (fields M_c .. runtime_schedule),
+ * plus the GPU realisation fields.  get_config returns the realised values.
+ * ---------------------------------------------------------------------- */
+typedef struct srt_tile_config {
+    int M_c;               /* rows per workgroup (M_r * panels per workgroup)        */
+    int K_c;               /* K tile; realised as K (no K split on the GPU path)      */
+    int N_c;               /* columns per wavefront (64 * vec_width)                  */
+    int tiling_strategy;   /* reference enum: 0 manual, 1 CAKE, 2 CAKE+TLB (recorded) */
+    float beta;            /* CAKE tiling beta (recorded; not a BLAS beta)            */
+    int sparse_a;          /* recorded                                                */
+    int max_tlb_entries;   /* recorded                                                */
+    int tlb_page_size;     /* recorded                                                */
+    int runtime_schedule;  /* reference outer schedule id (KNM=0 ...) (recorded)      */
+    int vec_width;         /* V in {1,2,4}: fp32 columns per lane; 0 = choose        */
+    int waves_per_block;   /* panels per workgroup, 1..8; 0 = choose                 */
+    int xcd_remap;         /* 1: group workgroups of one N tile on one XCD; -1 = choose */
+    int unroll;            /* packed columns in flight per wave: 1,2,4,8; 0 = choose  */
+} srt_tile_config;
+
+/* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54). */
+void srt_tile_config_default(srt_tile_config *cfg);
+
+/* ------------------------------------------------------------------------
+ * Pattern mappings (nonzero code -> list of kernel patterns).
+ * ---------------------------------------------------------------------- */
+typedef struct srt_mapping srt_mapping;
+
+/* Read mapping_<mapping_id>.txt from the first directory that has it.
+ * Replaces sop::read_pattern_mapping(mapping_id, dirs)
+ * (cpp_testbed/demo/count_nkerns.cpp:54-55; file format sbench/SOP/mapping.py:14-17). */
+int srt_mapping_read(const char *mapping_id, const char *const *search_dirs, int num_dirs,
+                     srt_mapping **out);
+
+/* Build a mapping from arrays: code_ptr[2^M_r + 1], code_patterns[code_ptr[2^M_r]];
+ * the kernel patterns of code c are code_patterns[code_ptr[c] .. code_ptr[c+1]).
+ * Same content as a mapping file (sbench/SOP/mapping.py:5-18). */
+int srt_mapping_create(int M_r, const int32_t *code_ptr, const int32_t *code_patterns,
+                       srt_mapping **out);
+
+/* M_r and the number of kernel patterns (MicroKernel::num_nkern_patterns, template:42). */
+int srt_mapping_info(const srt_mapping *m, int *M_r, int *num_patterns);
+
+/* id -> kernel pattern bitmask, num_patterns entries (decode_nkern_pattern, template:192).
+ * Ids follow (popcount, bitmask) order. */
+int srt_mapping_pattern_table(const srt_mapping *m, int32_t *patterns_out);
+
+void srt_mapping_destroy(srt_mapping *m);
+
+/* Executor id for a mapping's pattern set (get_executor_id, count_nkerns.cpp:50;
+ * nano/runtime_yaml_config.register:92-98).  Fails with SRT_ERR_UNSUPPORTED if
+ * no CDNA4 executor was generated for the pattern set. */
+int srt_executor_id(const srt_mapping *m, char *buf, size_t buf_len);
+
+/* Number of generated executors and the pattern set of each (for discovery). */
+int srt_executor_count(void);
+int srt_executor_describe(int index, int *M_r, int *num_patterns, int32_t *patterns_out,
+                          char *id_buf, size_t id_buf_len);
+
+/* ------------------------------------------------------------------------
+ * MatMulSpecialized: pack once, execute many times.
+ * Replaces sop::MatMulSpecialized<KernelDesc,DataTransform>
+ * (ctor / get_config / allocate_executor / operator()(C,B) / require_storage,
+ * used at cpp_testbed/src/spmm/nano/SpMM_SOP.h:64,89,117-134,155-164,178).
+ * ---------------------------------------------------------------------- */
+typedef struct srt_plan srt_plan;
+
+/* Pack A (m x k CSR, device arrays Lx[nnz], Lp[m+1], Li[nnz]) on the device.
+ * n is the widest B the plan will see.  executor_id may be NULL (derived from
+ * the mapping).  nThreads is accepted for signature parity and ignored. */
+int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const void *Li,
+                    int index_bytes, const srt_tile_config *cfg, int nThreads,
+                    const char *executor_id, const srt_mapping *mapping, srt_stream_t stream,
+                    srt_plan **out);
+
+int srt_plan_get_config(const srt_plan *p, srt_tile_config *out);
+
+/* Bytes of packed A held on the device (MatMulSpecialized::require_storage). */
+int srt_plan_required_storage(const srt_plan *p, size_t *bytes);
+
+/* Fix the launch shape for B widths up to n (MatMulSpecialized::allocate_executor). */
+int srt_plan_allocate_executor(srt_plan *p, int n);
+
+/* C[0:m, 0:n] = A * B[0:k, 0:n]  (operator()(C,B)).  Device B (ldb >= n) and
+ * C (ldc >= n).  n <= the n given at allocate_executor.  Rows and columns
+ * need no padding: tails are masked. */
+int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, int64_t ldb,
+                     int n, srt_stream_t stream);
+
+typedef struct srt_plan_stats {
+    int M, K, M_r, num_panels, num_patterns;
+    int64_t nnz;            /* stored entries of A                           */
+    int64_t packed_cols;    /* packed columns over all panels                */
+    int64_t packed_values;  /* stored values incl. padding                   */
+    int64_t padding;        /* stored values equal to 0.0f (template:197-201) */
+} srt_plan_stats;
+int srt_plan_stats_get(const srt_plan *p, srt_plan_stats *out);
+
+/* Copy the packed arrays to host memory (for inspection / parity checks):
+ * counts[num_panels*num_patterns], col_ptr[num_panels+1], val_ptr[num_panels+1],
+ * cols[packed_cols], vals[packed_values].  Any pointer may be NULL. */
+int srt_plan_copy_packed(const srt_plan *p, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
+                         int32_t *cols, float *vals);
+
+void srt_plan_destroy(srt_plan *p);
+
+/* ------------------------------------------------------------------------
+ * SOPTile binding primitives (sbench/SOP/_C/pytorch_wrapper_intrin.cpp.template).
+ * ---------------------------------------------------------------------- */
+
+/* sort_and_encode_patterns (template:75-98): device pat_codes[n] (kernel
+ * pattern bitmasks) -> device encoded[n] (ids, sorted ascending, zero pattern
+ * SRT_ZERO_PATTERN_ID last) and device order[n] (stable permutation:
+ * encoded[i] = id(pat_codes[order[i]])).  Fails on a pattern outside the set. */
+int srt_sort_and_encode_patterns(const srt_mapping *m, const int32_t *pat_codes, int n,
+                                 int32_t *encoded, int32_t *order, srt_stream_t stream);
+
+typedef struct srt_tile srt_tile;
+
+/* SOPTile(rows, cols, num_panels) (template:101-114). */
+int srt_tile_create(const srt_mapping *m, int64_t rows, int64_t cols, int64_t num_panels,
+                    srt_tile **out);
+
+/* SOPTile::pack_panel (template:129-217): device pat_codes[n], col_indices[n]
+ * (column of dense_panel), dense_panel[K x M_r] row-major.  Synchronises. */
+int srt_tile_pack_panel(srt_tile *t, int64_t panel_id, int64_t col_offset, const int32_t *pat_codes,
+                        const int32_t *col_indices, int n, const float *dense_panel, int K,
+                        srt_stream_t stream);
+
+/* SOPTile::pack_tile (template:219-261): every panel must be packed. */
+int srt_tile_pack_tile(srt_tile *t, srt_stream_t stream);
+
+/* SOPTile::padding (template:125-127). */
+int srt_tile_padding(const srt_tile *t, int64_t *padding);
+
+/* The panel executor over a packed tile (sop_multiple_panel_executor,
+ * template:265-305): C[0:rows, 0:N_c] (+)= tile * B[:, 0:N_c]; load_c != 0
+ * accumulates into C. */
+int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, float *C,
+                     int64_t ldc, int load_c, const srt_tile_config *cfg, srt_stream_t stream);
+
+int srt_tile_copy_packed(const srt_tile *t, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
+                         int32_t *cols, float *vals, int64_t *packed_cols, int64_t *packed_values);
+
+void srt_tile_destroy(srt_tile *t);
+
+/* ------------------------------------------------------------------------
+ * CSR baseline (sbench/SOP/_C/pytorch_csr_wrapper.cpp:243-312 _csr_kernel,
+ * torch entry csr.executor :314-353): C[m x n] = A_csr * B, device arrays
+ * rowptr[m+1], colind[nnz], vals[nnz].  n_tile in {16,32,64,128,256}
+ * selects columns per wavefront (values below 64 run as 64); tails masked.
+ * Row ranges are clamped to [0, nnz) and column indices outside [0, k) are
+ * skipped, so malformed input cannot make the kernel read out of bounds.
+ * ---------------------------------------------------------------------- */
+int srt_csr_spmm(int m, int k, int n, int64_t nnz, const void *rowptr, const void *colind,
+                 int index_bytes, const float *vals, const float *B, int64_t ldb, float *C,
+                 int64_t ldc, int n_tile, srt_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRT_H_ */

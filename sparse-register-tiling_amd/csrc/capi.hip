@@ -1,0 +1,913 @@
+// C ABI of libsrt: mappings, executor registry, MatMulSpecialized plans,
+// SOPTile packing and the CSR baseline.  Declarations and the reference
+// interface each entry point replaces are in include/srt.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "srt.h"
+#include "srt_internal.h"
+
+namespace srt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+// ------------------------------------------------------------------ registry
+static std::vector<const ExecutorDesc *> &registry() {
+    static std::vector<const ExecutorDesc *> r;
+    return r;
+}
+
+void register_executor(const ExecutorDesc *d) { registry().push_back(d); }
+
+const std::vector<const ExecutorDesc *> &executors() { return registry(); }
+
+const ExecutorDesc *find_executor(int M_r, const std::vector<int32_t> &patterns) {
+    for (const ExecutorDesc *d : registry()) {
+        if (d->M_r != M_r || d->num_patterns != (int)patterns.size()) continue;
+        if (std::equal(patterns.begin(), patterns.end(), d->patterns)) return d;
+    }
+    return nullptr;
+}
+
+const ExecutorDesc *find_executor_by_id(const char *id) {
+    for (const ExecutorDesc *d : registry())
+        if (std::strcmp(d->id, id) == 0) return d;
+    return nullptr;
+}
+
+void Packed::release() {
+    hipFree(counts);
+    hipFree(col_ptr);
+    hipFree(val_ptr);
+    hipFree(cols);
+    hipFree(vals);
+    hipFree(groups);
+    hipFree(grp_ptr);
+    counts = nullptr;
+    col_ptr = val_ptr = grp_ptr = nullptr;
+    cols = groups = nullptr;
+    vals = nullptr;
+}
+
+size_t Packed::bytes() const {
+    return sizeof(int32_t) * (size_t)P * NK + 3 * sizeof(int64_t) * (size_t)(P + 1) +
+           sizeof(int32_t) * (size_t)total_cols + sizeof(float) * (size_t)total_vals +
+           2 * sizeof(int32_t) * (size_t)total_groups;
+}
+
+// ------------------------------------------------------------------ mapping
+struct MappingImpl {
+    int M_r = 0;
+    std::vector<std::vector<int32_t>> table;  // code -> kernel patterns (list order)
+    std::vector<int32_t> patterns;            // id -> pattern
+    std::vector<int32_t> pattern_id;          // pattern -> id or -1
+    std::vector<int32_t> lut_ptr, lut_ids;
+    std::mutex mu;
+    std::deque<DeviceLut> dev;  // per device, lazily uploaded (stable addresses)
+
+    ~MappingImpl() {
+        int cur = -1;
+        hipGetDevice(&cur);
+        for (DeviceLut &d : dev) {
+            if (d.device < 0) continue;
+            hipSetDevice(d.device);
+            hipFree(d.lut_ptr);
+            hipFree(d.lut_ids);
+            hipFree(d.id_pattern);
+            hipFree(d.pattern_id);
+        }
+        if (cur >= 0) hipSetDevice(cur);
+    }
+
+    // Validates the table and derives the id order and the code -> ids LUT.
+    int finalize() {
+        const int ncode = 1 << M_r;
+        if ((int)table.size() != ncode) table.resize(ncode);
+        std::vector<char> seen(ncode, 0);
+        for (int c = 1; c < ncode; ++c) {
+            unsigned uni = 0;
+            if (table[c].empty())
+                return fail(SRT_ERR_INVALID, "mapping has no kernel patterns for code " +
+                                                 std::to_string(c));
+            std::vector<int32_t> kept;
+            for (int32_t p : table[c]) {
+                if (p == 0) continue;  // a zero pattern stores nothing
+                if (p < 0 || p >= ncode)
+                    return fail(SRT_ERR_INVALID, "kernel pattern " + std::to_string(p) +
+                                                     " does not fit M_r=" + std::to_string(M_r));
+                if ((unsigned)p & uni)
+                    return fail(SRT_ERR_INVALID, "kernel patterns of code " + std::to_string(c) +
+                                                     " overlap (a value would be added twice)");
+                uni |= (unsigned)p;
+                kept.push_back(p);
+                seen[p] = 1;
+            }
+            if ((uni & (unsigned)c) != (unsigned)c)
+                return fail(SRT_ERR_INVALID, "kernel patterns of code " + std::to_string(c) +
+                                                 " do not cover it");
+            table[c] = kept;
+        }
+        patterns.clear();
+        for (int p = 1; p < ncode; ++p)
+            if (seen[p]) patterns.push_back(p);
+        std::stable_sort(patterns.begin(), patterns.end(), [](int32_t a, int32_t b) {
+            const int pa = __builtin_popcount((unsigned)a), pb = __builtin_popcount((unsigned)b);
+            return pa != pb ? pa < pb : a < b;
+        });
+        if (patterns.size() > 4096)
+            return fail(SRT_ERR_UNSUPPORTED, "more than 4096 kernel patterns in one mapping");
+        pattern_id.assign(ncode, -1);
+        for (size_t i = 0; i < patterns.size(); ++i) pattern_id[patterns[i]] = (int32_t)i;
+        lut_ptr.assign(ncode + 1, 0);
+        lut_ids.clear();
+        for (int c = 0; c < ncode; ++c) {
+            lut_ptr[c] = (int32_t)lut_ids.size();
+            if (c)
+                for (int32_t p : table[c]) lut_ids.push_back(pattern_id[p]);
+        }
+        lut_ptr[ncode] = (int32_t)lut_ids.size();
+        return SRT_OK;
+    }
+
+    int device_lut(DeviceLut **out) {
+        int device = 0;
+        SRT_HIP_TRY(hipGetDevice(&device));
+        std::lock_guard<std::mutex> g(mu);
+        for (DeviceLut &d : dev)
+            if (d.device == device) {
+                *out = &d;
+                return SRT_OK;
+            }
+        DeviceLut d;
+        d.M_r = M_r;
+        d.num_patterns = (int)patterns.size();
+        const size_t ncode = (size_t)1 << M_r;
+        SRT_HIP_TRY(hipMalloc(&d.lut_ptr, sizeof(int32_t) * (ncode + 1)));
+        SRT_HIP_TRY(hipMalloc(&d.lut_ids, sizeof(int32_t) * std::max<size_t>(1, lut_ids.size())));
+        SRT_HIP_TRY(hipMalloc(&d.id_pattern, sizeof(int32_t) * std::max<size_t>(1, patterns.size())));
+        SRT_HIP_TRY(hipMalloc(&d.pattern_id, sizeof(int32_t) * ncode));
+        SRT_HIP_TRY(hipMemcpy(d.lut_ptr, lut_ptr.data(), sizeof(int32_t) * (ncode + 1),
+                              hipMemcpyHostToDevice));
+        if (!lut_ids.empty())
+            SRT_HIP_TRY(hipMemcpy(d.lut_ids, lut_ids.data(), sizeof(int32_t) * lut_ids.size(),
+                                  hipMemcpyHostToDevice));
+        if (!patterns.empty())
+            SRT_HIP_TRY(hipMemcpy(d.id_pattern, patterns.data(),
+                                  sizeof(int32_t) * patterns.size(), hipMemcpyHostToDevice));
+        SRT_HIP_TRY(hipMemcpy(d.pattern_id, pattern_id.data(), sizeof(int32_t) * ncode,
+                              hipMemcpyHostToDevice));
+        d.device = device;
+        dev.push_back(d);
+        *out = &dev.back();
+        return SRT_OK;
+    }
+};
+
+// Parses "M_r\n<code>: [p, p, ...]\n..." (sbench/SOP/mapping.py:14-17).
+static int parse_mapping(std::istream &in, MappingImpl &m, const std::string &what) {
+    std::string line;
+    if (!std::getline(in, line)) return fail(SRT_ERR_IO, "empty mapping file " + what);
+    m.M_r = std::atoi(line.c_str());
+    if (m.M_r < 1 || m.M_r > SRT_MAX_MR)
+        return fail(SRT_ERR_INVALID, "mapping " + what + ": M_r must be in [1, 16]");
+    m.table.assign((size_t)1 << m.M_r, {});
+    int lineno = 1;
+    while (std::getline(in, line)) {
+        ++lineno;
+        size_t colon = line.find(':');
+        if (colon == std::string::npos) {
+            bool blank = std::all_of(line.begin(), line.end(), [](char ch) { return std::isspace((unsigned char)ch); });
+            if (blank) continue;
+            return fail(SRT_ERR_INVALID, what + ":" + std::to_string(lineno) + ": expected 'code: [..]'");
+        }
+        long code = std::strtol(line.c_str(), nullptr, 10);
+        if (code < 1 || code >= (1L << m.M_r))
+            return fail(SRT_ERR_INVALID, what + ":" + std::to_string(lineno) + ": code out of range");
+        std::vector<int32_t> pats;
+        const char *s = line.c_str() + colon + 1;
+        while (*s) {
+            if (std::isdigit((unsigned char)*s) || *s == '-') {
+                char *e = nullptr;
+                long v = std::strtol(s, &e, 10);
+                pats.push_back((int32_t)v);
+                s = e;
+            } else {
+                ++s;
+            }
+        }
+        m.table[code] = pats;
+    }
+    return m.finalize();
+}
+
+}  // namespace srt
+
+using namespace srt;
+
+struct srt_mapping {
+    std::shared_ptr<MappingImpl> impl;
+};
+
+// ------------------------------------------------------------------- plans
+struct srt_plan {
+    Packed pk;
+    const ExecutorDesc *exec = nullptr;
+    srt_tile_config cfg{};
+    int n_max = 0;
+    int n_alloc = 0;
+    int device = 0;
+};
+
+struct PanelBuf {
+    int32_t *counts = nullptr;  // NK + 1 (last = zero-pattern entries)
+    int32_t *cols = nullptr;
+    float *vals = nullptr;
+    int64_t ncols = 0, nvals = 0, padding = 0;
+    bool packed = false;
+};
+
+struct srt_tile {
+    std::shared_ptr<MappingImpl> map;
+    int64_t rows = 0, cols = 0, num_panels = 0;
+    std::vector<PanelBuf> panels;
+    bool tile_packed = false;
+    Packed pk;
+    const ExecutorDesc *exec = nullptr;
+    int64_t padding_total = 0;
+    int device = 0;
+};
+
+extern "C" {
+
+const char *srt_last_error(void) { return g_last_error.c_str(); }
+
+int srt_version(void) { return 10000; }
+
+void srt_tile_config_default(srt_tile_config *c) {
+    if (!c) return;
+    // SOPConfig defaults (cpp_testbed/src/spmm/nano/SpMM_SOP.h:33-54).
+    c->M_c = 16;
+    c->K_c = 256;
+    c->N_c = 64;
+    c->tiling_strategy = 0;
+    c->beta = 1.0f;
+    c->sparse_a = 1;
+    c->max_tlb_entries = 64;
+    c->tlb_page_size = 4096;
+    c->runtime_schedule = 0;
+    c->vec_width = 0;
+    c->waves_per_block = 0;
+    c->xcd_remap = -1;
+    c->unroll = 0;
+}
+
+// ----------------------------------------------------------------- mapping
+int srt_mapping_read(const char *mapping_id, const char *const *dirs, int num_dirs,
+                     srt_mapping **out) {
+    if (!mapping_id || !out) return fail(SRT_ERR_INVALID, "srt_mapping_read: null argument");
+    std::string name = std::string("mapping_") + mapping_id + ".txt";
+    for (int i = 0; i < num_dirs; ++i) {
+        if (!dirs || !dirs[i]) continue;
+        std::string path = std::string(dirs[i]);
+        if (!path.empty() && path.back() != '/') path += '/';
+        path += name;
+        std::ifstream f(path);
+        if (!f) continue;
+        auto impl = std::make_shared<MappingImpl>();
+        int rc = parse_mapping(f, *impl, path);
+        if (rc) return rc;
+        *out = new srt_mapping{impl};
+        return SRT_OK;
+    }
+    return fail(SRT_ERR_IO, "mapping " + std::string(mapping_id) + " not found in search dirs");
+}
+
+int srt_mapping_create(int M_r, const int32_t *code_ptr, const int32_t *code_patterns,
+                       srt_mapping **out) {
+    if (!code_ptr || !out) return fail(SRT_ERR_INVALID, "srt_mapping_create: null argument");
+    if (M_r < 1 || M_r > SRT_MAX_MR) return fail(SRT_ERR_INVALID, "M_r must be in [1, 16]");
+    auto impl = std::make_shared<MappingImpl>();
+    impl->M_r = M_r;
+    const int ncode = 1 << M_r;
+    impl->table.assign(ncode, {});
+    for (int c = 0; c < ncode; ++c) {
+        if (code_ptr[c + 1] < code_ptr[c]) return fail(SRT_ERR_INVALID, "code_ptr not monotone");
+        for (int e = code_ptr[c]; e < code_ptr[c + 1]; ++e) impl->table[c].push_back(code_patterns[e]);
+    }
+    impl->table[0].clear();
+    int rc = impl->finalize();
+    if (rc) return rc;
+    *out = new srt_mapping{impl};
+    return SRT_OK;
+}
+
+int srt_mapping_info(const srt_mapping *m, int *M_r, int *num_patterns) {
+    if (!m) return fail(SRT_ERR_INVALID, "null mapping");
+    if (M_r) *M_r = m->impl->M_r;
+    if (num_patterns) *num_patterns = (int)m->impl->patterns.size();
+    return SRT_OK;
+}
+
+int srt_mapping_pattern_table(const srt_mapping *m, int32_t *out) {
+    if (!m || !out) return fail(SRT_ERR_INVALID, "null argument");
+    std::copy(m->impl->patterns.begin(), m->impl->patterns.end(), out);
+    return SRT_OK;
+}
+
+void srt_mapping_destroy(srt_mapping *m) { delete m; }
+
+int srt_executor_id(const srt_mapping *m, char *buf, size_t len) {
+    if (!m || !buf) return fail(SRT_ERR_INVALID, "null argument");
+    const ExecutorDesc *d = find_executor(m->impl->M_r, m->impl->patterns);
+    if (!d) return fail(SRT_ERR_UNSUPPORTED, "no CDNA4 executor generated for this pattern set");
+    std::snprintf(buf, len, "%s", d->id);
+    return SRT_OK;
+}
+
+int srt_executor_count(void) { return (int)executors().size(); }
+
+int srt_executor_describe(int index, int *M_r, int *num_patterns, int32_t *patterns_out,
+                          char *id_buf, size_t id_buf_len) {
+    if (index < 0 || index >= (int)executors().size()) return fail(SRT_ERR_INVALID, "bad index");
+    const ExecutorDesc *d = executors()[index];
+    if (M_r) *M_r = d->M_r;
+    if (num_patterns) *num_patterns = d->num_patterns;
+    if (patterns_out) std::copy(d->patterns, d->patterns + d->num_patterns, patterns_out);
+    if (id_buf) std::snprintf(id_buf, id_buf_len, "%s", d->id);
+    return SRT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ launch shape
+namespace srt {
+
+static int log2_small(int v) { return v >= 8 ? 3 : v >= 4 ? 2 : v >= 2 ? 1 : 0; }
+
+// Realise the GPU fields of a tile config for a matrix with P panels and B
+// width n (the choices the reference's heuristic JSON made per sparsity
+// bucket are made here from the launch geometry).
+static void realise_config(srt_tile_config &c, int M_r, int K, int P, int n) {
+    if (c.vec_width != 1 && c.vec_width != 2 && c.vec_width != 4) {
+        // Widest V that still gives >= 2048 wavefronts (2 per SIMD), else 1.
+        c.vec_width = 1;
+        for (int v : {4, 2}) {
+            long long waves = (long long)P * ((n + 64 * v - 1) / (64 * v));
+            if (n % v == 0 && waves >= 2048) {
+                c.vec_width = v;
+                break;
+            }
+        }
+    }
+    if (c.waves_per_block < 1 || c.waves_per_block > 8) c.waves_per_block = 4;
+    if (c.xcd_remap != 0 && c.xcd_remap != 1) c.xcd_remap = 1;
+    if (c.unroll != 1 && c.unroll != 2 && c.unroll != 4 && c.unroll != 8) c.unroll = 4;
+    c.M_c = M_r * c.waves_per_block;
+    c.N_c = 64 * c.vec_width;
+    c.K_c = K;
+}
+
+static bool aligned(const void *p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
+
+static int run_executor(const Packed &pk, const ExecutorDesc *exec, const srt_tile_config &cfg,
+                        const float *B, int64_t ldb, float *C, int64_t ldc, int n, int load_c,
+                        hipStream_t s) {
+    if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
+    if (n == 0 || pk.M == 0) return SRT_OK;
+    if (!B || !C) return fail(SRT_ERR_INVALID, "null B or C");
+    if (ldb < n || ldc < n) return fail(SRT_ERR_INVALID, "ldb and ldc must be >= n");
+    int V = cfg.vec_width;
+    // Vector loads need V | n, V | ldb, V | ldc and 4V-byte aligned B and C.
+    while (V > 1 && (n % V || ldb % V || ldc % V || !aligned(B, 4 * V) || !aligned(C, 4 * V)))
+        V >>= 1;
+    int ui = log2_small(cfg.unroll);
+    const int vi = log2_small(V);
+    ExecLaunchFn fn = nullptr;
+    for (int u = ui; u >= 0 && !fn; --u) fn = exec->launch[vi][u];
+    for (int u = ui + 1; u < 4 && !fn; ++u) fn = exec->launch[vi][u];
+    if (!fn) return fail(SRT_ERR_UNSUPPORTED, "executor variant not generated");
+    // B rows are addressed with a 32-bit byte offset (buffer loads).
+    const int64_t b_extent = (int64_t)pk.K * ldb * (int64_t)sizeof(float);
+    if (b_extent > (int64_t)0xffffffffLL)
+        return fail(SRT_ERR_UNSUPPORTED, "B larger than 4 GiB (K * ldb * 4 bytes)");
+    ExecLaunch a;
+    a.groups = pk.groups;
+    a.grp_ptr = pk.grp_ptr;
+    a.col_ptr = pk.col_ptr;
+    a.val_ptr = pk.val_ptr;
+    a.cols = pk.cols;
+    a.vals = pk.vals;
+    a.B = B;
+    a.b_bytes = (uint32_t)std::max<int64_t>(b_extent, 4);
+    a.C = C;
+    a.ldb = ldb;
+    a.ldc = ldc;
+    a.M = pk.M;
+    a.N = n;
+    a.num_panels = pk.P;
+    a.load_c = load_c;
+    a.vec_width = V;
+    a.waves_per_block = cfg.waves_per_block;
+    a.xcd_remap = cfg.xcd_remap;
+    a.unroll = cfg.unroll;
+    SRT_HIP_TRY(fn(a, s));
+    return SRT_OK;
+}
+
+template <typename T>
+static int dmalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess)
+        return fail(SRT_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return SRT_OK;
+}
+
+struct Scratch {
+    std::vector<void *> ptrs;
+    ~Scratch() {
+        for (void *p : ptrs) hipFree(p);
+    }
+    template <typename T>
+    int get(T **p, size_t count) {
+        int rc = dmalloc(p, count);
+        if (!rc) ptrs.push_back(*p);
+        return rc;
+    }
+};
+
+#define SRT_TRY(expr)            \
+    do {                         \
+        int _rc = (expr);        \
+        if (_rc) return _rc;     \
+    } while (0)
+
+// Per-panel lists of non-empty groups (id, count) from pk.counts.
+static int build_groups(Packed &pk, hipStream_t s) {
+    Scratch sc;
+    int64_t *ng = nullptr;
+    SRT_TRY(sc.get(&ng, pk.P));
+    SRT_TRY(dmalloc(&pk.grp_ptr, (size_t)pk.P + 1));
+    SRT_HIP_TRY(launch_group_count(pk.P, pk.NK, pk.counts, ng, s));
+    SRT_HIP_TRY(launch_exclusive_scan(pk.P, ng, pk.grp_ptr, s));
+    int64_t tot = 0;
+    SRT_HIP_TRY(hipMemcpyAsync(&tot, pk.grp_ptr + pk.P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    pk.total_groups = tot;
+    SRT_TRY(dmalloc(&pk.groups, 2 * (size_t)tot));
+    SRT_HIP_TRY(launch_group_fill(pk.P, pk.NK, pk.counts, pk.grp_ptr, pk.groups, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    return SRT_OK;
+}
+
+// Packs an m x k CSR matrix (device arrays) into pk.  Mirrors the packing
+// done by the MatMulSpecialized constructor (SpMM_SOP.h:117-124).
+static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, const float *Lx, const void *Lp,
+                    const void *Li, int index_bytes, hipStream_t s) {
+    DeviceLut *lut = nullptr;
+    SRT_TRY(map.device_lut(&lut));
+    // The row pointer is walked by the scatter kernel: check it on the host
+    // first (non-negative start, non-decreasing).
+    {
+        std::vector<int64_t> rp(m + 1);
+        if (index_bytes == 8) {
+            SRT_HIP_TRY(hipMemcpyAsync(rp.data(), Lp, 8 * (size_t)(m + 1), hipMemcpyDeviceToHost, s));
+            SRT_HIP_TRY(hipStreamSynchronize(s));
+        } else {
+            std::vector<int32_t> rp32(m + 1);
+            SRT_HIP_TRY(hipMemcpyAsync(rp32.data(), Lp, 4 * (size_t)(m + 1), hipMemcpyDeviceToHost, s));
+            SRT_HIP_TRY(hipStreamSynchronize(s));
+            for (int i = 0; i <= m; ++i) rp[i] = rp32[i];
+        }
+        if (rp[0] < 0) return fail(SRT_ERR_INVALID, "CSR row pointer starts below zero");
+        for (int i = 0; i < m; ++i)
+            if (rp[i + 1] < rp[i]) return fail(SRT_ERR_INVALID, "CSR row pointer decreases at row " + std::to_string(i));
+    }
+    const int M_r = map.M_r;
+    const int NK = (int)map.patterns.size();
+    const int P = (m + M_r - 1) / M_r;
+    pk.M = m;
+    pk.K = k;
+    pk.M_r = M_r;
+    pk.P = P;
+    pk.NK = NK;
+    Scratch sc;
+    uint32_t *codes = nullptr;
+    float *dense = nullptr;
+    int *bad = nullptr;
+    int64_t *ncols = nullptr, *nvals = nullptr, *pad = nullptr;
+    const size_t pk_cells = (size_t)P * (size_t)k;
+    SRT_TRY(sc.get(&codes, pk_cells));
+    SRT_TRY(sc.get(&dense, pk_cells * M_r));
+    SRT_TRY(sc.get(&bad, 1));
+    SRT_TRY(sc.get(&ncols, P));
+    SRT_TRY(sc.get(&nvals, P));
+    SRT_TRY(sc.get(&pad, P));
+    SRT_HIP_TRY(hipMemsetAsync(codes, 0, sizeof(uint32_t) * pk_cells, s));
+    SRT_HIP_TRY(hipMemsetAsync(dense, 0, sizeof(float) * pk_cells * M_r, s));
+    SRT_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), s));
+    SRT_TRY(dmalloc(&pk.counts, (size_t)P * NK));
+    SRT_TRY(dmalloc(&pk.col_ptr, (size_t)P + 1));
+    SRT_TRY(dmalloc(&pk.val_ptr, (size_t)P + 1));
+    SRT_HIP_TRY(launch_scatter_csr(m, k, M_r, Lx, Lp, Li, index_bytes, codes, dense, bad, s));
+    SRT_HIP_TRY(launch_count_codes(P, k, *lut, codes, pk.counts, ncols, nvals, s));
+    SRT_HIP_TRY(launch_exclusive_scan(P, ncols, pk.col_ptr, s));
+    SRT_HIP_TRY(launch_exclusive_scan(P, nvals, pk.val_ptr, s));
+    int h_bad = 0;
+    int64_t tot_c = 0, tot_v = 0;
+    SRT_HIP_TRY(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&tot_c, pk.col_ptr + P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&tot_v, pk.val_ptr + P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    // nnz from the row pointer
+    int64_t lp0 = 0, lpm = 0;
+    if (index_bytes == 8) {
+        SRT_HIP_TRY(hipMemcpyAsync(&lp0, Lp, 8, hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipMemcpyAsync(&lpm, (const int64_t *)Lp + m, 8, hipMemcpyDeviceToHost, s));
+    } else {
+        int32_t a = 0, b = 0;
+        SRT_HIP_TRY(hipMemcpyAsync(&a, Lp, 4, hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipMemcpyAsync(&b, (const int32_t *)Lp + m, 4, hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipStreamSynchronize(s));
+        lp0 = a;
+        lpm = b;
+    }
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    if (h_bad) return fail(SRT_ERR_INVALID, "CSR column index out of range [0, k)");
+    pk.nnz = lpm - lp0;
+    pk.total_cols = tot_c;
+    pk.total_vals = tot_v;
+    SRT_TRY(dmalloc(&pk.cols, (size_t)tot_c));
+    SRT_TRY(dmalloc(&pk.vals, (size_t)tot_v));
+    SRT_HIP_TRY(launch_fill_codes(P, k, 0, *lut, codes, dense, pk.col_ptr, pk.val_ptr, pk.counts,
+                                  pk.cols, pk.vals, pad, s));
+    std::vector<int64_t> h_pad(P);
+    if (P) SRT_HIP_TRY(hipMemcpyAsync(h_pad.data(), pad, sizeof(int64_t) * P, hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    pk.padding = 0;
+    for (int64_t v : h_pad) pk.padding += v;
+    return build_groups(pk, s);
+}
+
+}  // namespace srt
+
+extern "C" {
+
+int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const void *Li,
+                    int index_bytes, const srt_tile_config *cfg, int nThreads,
+                    const char *executor_id, const srt_mapping *mapping, srt_stream_t stream,
+                    srt_plan **out) {
+    (void)nThreads;
+    if (!out || !mapping) return fail(SRT_ERR_INVALID, "srt_plan_create: null argument");
+    *out = nullptr;
+    if (m < 0 || k < 0 || n < 0) return fail(SRT_ERR_INVALID, "negative dimension");
+    if (index_bytes != 4 && index_bytes != 8) return fail(SRT_ERR_INVALID, "index_bytes must be 4 or 8");
+    if (!Lp || ((!Lx || !Li) && m > 0)) return fail(SRT_ERR_INVALID, "null CSR array");
+    MappingImpl &map = *mapping->impl;
+    const ExecutorDesc *exec = executor_id && executor_id[0] ? find_executor_by_id(executor_id)
+                                                             : find_executor(map.M_r, map.patterns);
+    if (!exec)
+        return fail(SRT_ERR_UNSUPPORTED, executor_id && executor_id[0]
+                                             ? "unknown executor id " + std::string(executor_id)
+                                             : "no CDNA4 executor generated for this pattern set");
+    if (exec->M_r != map.M_r || exec->num_patterns != (int)map.patterns.size() ||
+        !std::equal(map.patterns.begin(), map.patterns.end(), exec->patterns))
+        return fail(SRT_ERR_INVALID, "executor and mapping pattern sets differ");
+    auto plan = std::make_unique<srt_plan>();
+    SRT_HIP_TRY(hipGetDevice(&plan->device));
+    int rc = pack_csr(plan->pk, map, m, k, Lx, Lp, Li, index_bytes, (hipStream_t)stream);
+    if (rc) {
+        plan->pk.release();
+        return rc;
+    }
+    plan->exec = exec;
+    if (cfg) plan->cfg = *cfg;
+    else srt_tile_config_default(&plan->cfg);
+    plan->n_max = n;
+    realise_config(plan->cfg, map.M_r, k, plan->pk.P, std::max(n, 1));
+    *out = plan.release();
+    return SRT_OK;
+}
+
+int srt_plan_get_config(const srt_plan *p, srt_tile_config *out) {
+    if (!p || !out) return fail(SRT_ERR_INVALID, "null argument");
+    *out = p->cfg;
+    return SRT_OK;
+}
+
+int srt_plan_required_storage(const srt_plan *p, size_t *bytes) {
+    if (!p || !bytes) return fail(SRT_ERR_INVALID, "null argument");
+    *bytes = p->pk.bytes();
+    return SRT_OK;
+}
+
+int srt_plan_allocate_executor(srt_plan *p, int n) {
+    if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
+    p->n_alloc = n;
+    if (n > p->n_max) p->n_max = n;
+    return SRT_OK;
+}
+
+int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, int64_t ldb, int n,
+                     srt_stream_t stream) {
+    if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    int dev = -1;
+    SRT_HIP_TRY(hipGetDevice(&dev));
+    if (dev != p->device) return fail(SRT_ERR_INVALID, "plan belongs to another device");
+    return run_executor(p->pk, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream);
+}
+
+int srt_plan_stats_get(const srt_plan *p, srt_plan_stats *o) {
+    if (!p || !o) return fail(SRT_ERR_INVALID, "null argument");
+    o->M = p->pk.M;
+    o->K = p->pk.K;
+    o->M_r = p->pk.M_r;
+    o->num_panels = p->pk.P;
+    o->num_patterns = p->pk.NK;
+    o->nnz = p->pk.nnz;
+    o->packed_cols = p->pk.total_cols;
+    o->packed_values = p->pk.total_vals;
+    o->padding = p->pk.padding;
+    return SRT_OK;
+}
+
+static int copy_packed(const Packed &pk, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
+                       int32_t *cols, float *vals) {
+    if (counts && pk.P && pk.NK)
+        SRT_HIP_TRY(hipMemcpy(counts, pk.counts, sizeof(int32_t) * pk.P * pk.NK, hipMemcpyDeviceToHost));
+    if (col_ptr) SRT_HIP_TRY(hipMemcpy(col_ptr, pk.col_ptr, sizeof(int64_t) * (pk.P + 1), hipMemcpyDeviceToHost));
+    if (val_ptr) SRT_HIP_TRY(hipMemcpy(val_ptr, pk.val_ptr, sizeof(int64_t) * (pk.P + 1), hipMemcpyDeviceToHost));
+    if (cols && pk.total_cols)
+        SRT_HIP_TRY(hipMemcpy(cols, pk.cols, sizeof(int32_t) * pk.total_cols, hipMemcpyDeviceToHost));
+    if (vals && pk.total_vals)
+        SRT_HIP_TRY(hipMemcpy(vals, pk.vals, sizeof(float) * pk.total_vals, hipMemcpyDeviceToHost));
+    return SRT_OK;
+}
+
+int srt_plan_copy_packed(const srt_plan *p, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
+                         int32_t *cols, float *vals) {
+    if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    return copy_packed(p->pk, counts, col_ptr, val_ptr, cols, vals);
+}
+
+void srt_plan_destroy(srt_plan *p) {
+    if (!p) return;
+    p->pk.release();
+    delete p;
+}
+
+// ------------------------------------------------------------ SOPTile path
+int srt_sort_and_encode_patterns(const srt_mapping *m, const int32_t *pat_codes, int n,
+                                 int32_t *encoded, int32_t *order, srt_stream_t stream) {
+    if (!m || n < 0) return fail(SRT_ERR_INVALID, "bad argument");
+    if (n == 0) return SRT_OK;
+    if (!pat_codes || !encoded || !order) return fail(SRT_ERR_INVALID, "null array");
+    hipStream_t s = (hipStream_t)stream;
+    DeviceLut *lut = nullptr;
+    SRT_TRY(m->impl->device_lut(&lut));
+    Scratch sc;
+    int32_t *ids = nullptr, *bad = nullptr, *counts = nullptr;
+    int64_t *tot = nullptr;
+    SRT_TRY(sc.get(&ids, n));
+    SRT_TRY(sc.get(&bad, 1));
+    SRT_TRY(sc.get(&counts, lut->num_patterns + 1));
+    SRT_TRY(sc.get(&tot, 2));
+    SRT_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+    SRT_HIP_TRY(launch_encode_entries(n, *lut, pat_codes, nullptr, 0, ids, bad, s));
+    int h_bad = 0;
+    SRT_HIP_TRY(hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    if (h_bad) return fail(SRT_ERR_INVALID, "pattern not in the kernel pattern set");
+    SRT_HIP_TRY(launch_count_entries(n, *lut, ids, counts, tot, tot + 1, s));
+    SRT_HIP_TRY(launch_stable_order(n, lut->num_patterns, ids, counts, encoded, order, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    return SRT_OK;
+}
+
+int srt_tile_create(const srt_mapping *m, int64_t rows, int64_t cols, int64_t num_panels,
+                    srt_tile **out) {
+    if (!m || !out) return fail(SRT_ERR_INVALID, "null argument");
+    if (rows < 0 || cols < 0 || num_panels < 0 || num_panels > 0x7fffffff)
+        return fail(SRT_ERR_INVALID, "bad tile shape");
+    if (num_panels * m->impl->M_r < rows)
+        return fail(SRT_ERR_INVALID, "num_panels * M_r must cover the tile rows");
+    const ExecutorDesc *exec = find_executor(m->impl->M_r, m->impl->patterns);
+    if (!exec) return fail(SRT_ERR_UNSUPPORTED, "no CDNA4 executor generated for this pattern set");
+    auto t = std::make_unique<srt_tile>();
+    t->map = m->impl;
+    t->rows = rows;
+    t->cols = cols;
+    t->num_panels = num_panels;
+    t->panels.resize(num_panels);
+    t->exec = exec;
+    SRT_HIP_TRY(hipGetDevice(&t->device));
+    *out = t.release();
+    return SRT_OK;
+}
+
+int srt_tile_pack_panel(srt_tile *t, int64_t panel_id, int64_t col_offset, const int32_t *pat_codes,
+                        const int32_t *col_indices, int n, const float *dense_panel, int K,
+                        srt_stream_t stream) {
+    if (!t) return fail(SRT_ERR_INVALID, "null tile");
+    if (panel_id < 0 || panel_id >= t->num_panels) return fail(SRT_ERR_INVALID, "panel_id out of range");
+    if (t->tile_packed || t->panels[panel_id].packed) return fail(SRT_ERR_INVALID, "panel already packed");
+    if (n < 0 || K < 0) return fail(SRT_ERR_INVALID, "negative size");
+    if (n > 0 && (!pat_codes || !col_indices || !dense_panel)) return fail(SRT_ERR_INVALID, "null array");
+    // Packed B-row indices are col + col_offset with col < K: they must stay
+    // inside the tile's columns (B has exactly `cols` rows, template:366).
+    if (col_offset < 0 || col_offset + (int64_t)K > t->cols)
+        return fail(SRT_ERR_INVALID, "col_offset + K exceeds the tile's columns");
+    hipStream_t s = (hipStream_t)stream;
+    MappingImpl &map = *t->map;
+    DeviceLut *lut = nullptr;
+    SRT_TRY(map.device_lut(&lut));
+    const int NK = lut->num_patterns;
+    PanelBuf pb;
+    Scratch sc;
+    int32_t *ids = nullptr, *bad = nullptr;
+    int64_t *tot = nullptr, *pad = nullptr, *nzc = nullptr;
+    SRT_TRY(sc.get(&ids, n));
+    SRT_TRY(sc.get(&bad, 1));
+    SRT_TRY(sc.get(&tot, 2));
+    SRT_TRY(sc.get(&pad, 1));
+    SRT_TRY(sc.get(&nzc, 1));
+    SRT_TRY(dmalloc(&pb.counts, NK + 1));
+    SRT_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+    SRT_HIP_TRY(hipMemsetAsync(pad, 0, sizeof(int64_t), s));
+    SRT_HIP_TRY(launch_encode_entries(n, *lut, pat_codes, col_indices, K, ids, bad, s));
+    SRT_HIP_TRY(launch_count_entries(n, *lut, ids, pb.counts, tot, tot + 1, s));
+    SRT_HIP_TRY(launch_count_nonzero(dense_panel, (int64_t)K * map.M_r, nzc, s));
+    int h_bad = 0;
+    int64_t h_tot[2] = {0, 0};
+    int64_t h_nz = 0;
+    SRT_HIP_TRY(hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(h_tot, tot, sizeof(h_tot), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&h_nz, nzc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    if (h_bad & 1) {
+        hipFree(pb.counts);
+        return fail(SRT_ERR_INVALID, "pattern not in the kernel pattern set");
+    }
+    if (h_bad & 2) {
+        hipFree(pb.counts);
+        return fail(SRT_ERR_INVALID, "column index out of range of dense_panel");
+    }
+    pb.ncols = h_tot[0];
+    pb.nvals = h_tot[1];
+    int rc = dmalloc(&pb.cols, (size_t)pb.ncols);
+    if (!rc) rc = dmalloc(&pb.vals, (size_t)pb.nvals);
+    if (rc) {
+        hipFree(pb.counts);
+        hipFree(pb.cols);
+        return rc;
+    }
+    SRT_HIP_TRY(launch_fill_entries(n, (int)col_offset, K, *lut, ids, col_indices, dense_panel,
+                                    pb.counts, pb.cols, pb.vals, pad, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&pb.padding, pad, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    // The reference's packing invariant (template:214).
+    if (pb.nvals != pb.padding + h_nz) {
+        hipFree(pb.counts);
+        hipFree(pb.cols);
+        hipFree(pb.vals);
+        return fail(SRT_ERR_INVALID,
+                    "packed values != padding + nonzeros of dense_panel: the patterns do not "
+                    "cover the panel's nonzeros exactly once");
+    }
+    pb.packed = true;
+    t->padding_total += pb.padding;
+    t->panels[panel_id] = pb;
+    return SRT_OK;
+}
+
+int srt_tile_pack_tile(srt_tile *t, srt_stream_t stream) {
+    if (!t) return fail(SRT_ERR_INVALID, "null tile");
+    if (t->tile_packed) return fail(SRT_ERR_INVALID, "tile already packed");
+    for (const PanelBuf &pb : t->panels)
+        if (!pb.packed) return fail(SRT_ERR_INVALID, "every panel must be packed before pack_tile");
+    hipStream_t s = (hipStream_t)stream;
+    Packed &pk = t->pk;
+    const int P = (int)t->num_panels;
+    const int NK = (int)t->map->patterns.size();
+    pk.M = (int)t->rows;
+    pk.K = (int)t->cols;
+    pk.M_r = t->map->M_r;
+    pk.P = P;
+    pk.NK = NK;
+    std::vector<int64_t> cp(P + 1, 0), vp(P + 1, 0);
+    for (int p = 0; p < P; ++p) {
+        cp[p + 1] = cp[p] + t->panels[p].ncols;
+        vp[p + 1] = vp[p] + t->panels[p].nvals;
+    }
+    pk.total_cols = cp[P];
+    pk.total_vals = vp[P];
+    pk.padding = t->padding_total;
+    SRT_TRY(dmalloc(&pk.counts, (size_t)P * NK));
+    SRT_TRY(dmalloc(&pk.col_ptr, (size_t)P + 1));
+    SRT_TRY(dmalloc(&pk.val_ptr, (size_t)P + 1));
+    SRT_TRY(dmalloc(&pk.cols, (size_t)pk.total_cols));
+    SRT_TRY(dmalloc(&pk.vals, (size_t)pk.total_vals));
+    SRT_HIP_TRY(hipMemcpyAsync(pk.col_ptr, cp.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
+    SRT_HIP_TRY(hipMemcpyAsync(pk.val_ptr, vp.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
+    for (int p = 0; p < P; ++p) {
+        const PanelBuf &pb = t->panels[p];
+        if (NK)
+            SRT_HIP_TRY(hipMemcpyAsync(pk.counts + (int64_t)p * NK, pb.counts, sizeof(int32_t) * NK,
+                                       hipMemcpyDeviceToDevice, s));
+        if (pb.ncols)
+            SRT_HIP_TRY(hipMemcpyAsync(pk.cols + cp[p], pb.cols, sizeof(int32_t) * pb.ncols,
+                                       hipMemcpyDeviceToDevice, s));
+        if (pb.nvals)
+            SRT_HIP_TRY(hipMemcpyAsync(pk.vals + vp[p], pb.vals, sizeof(float) * pb.nvals,
+                                       hipMemcpyDeviceToDevice, s));
+    }
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    SRT_TRY(build_groups(pk, s));
+    for (PanelBuf &pb : t->panels) {
+        hipFree(pb.counts);
+        hipFree(pb.cols);
+        hipFree(pb.vals);
+        pb.counts = nullptr;
+        pb.cols = nullptr;
+        pb.vals = nullptr;
+    }
+    t->tile_packed = true;
+    return SRT_OK;
+}
+
+int srt_tile_padding(const srt_tile *t, int64_t *padding) {
+    if (!t || !padding) return fail(SRT_ERR_INVALID, "null argument");
+    *padding = t->padding_total;
+    return SRT_OK;
+}
+
+int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, float *C, int64_t ldc,
+                     int load_c, const srt_tile_config *cfg, srt_stream_t stream) {
+    if (!t) return fail(SRT_ERR_INVALID, "null tile");
+    if (!t->tile_packed) return fail(SRT_ERR_INVALID, "pack_tile has not been called");
+    srt_tile_config c;
+    if (cfg) c = *cfg;
+    else srt_tile_config_default(&c);
+    realise_config(c, t->pk.M_r, t->pk.K, t->pk.P, std::max(N_c, 1));
+    return run_executor(t->pk, t->exec, c, B, ldb, C, ldc, N_c, load_c ? 1 : 0, (hipStream_t)stream);
+}
+
+int srt_tile_copy_packed(const srt_tile *t, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
+                         int32_t *cols, float *vals, int64_t *packed_cols, int64_t *packed_values) {
+    if (!t) return fail(SRT_ERR_INVALID, "null tile");
+    if (!t->tile_packed) return fail(SRT_ERR_INVALID, "pack_tile has not been called");
+    if (packed_cols) *packed_cols = t->pk.total_cols;
+    if (packed_values) *packed_values = t->pk.total_vals;
+    return copy_packed(t->pk, counts, col_ptr, val_ptr, cols, vals);
+}
+
+void srt_tile_destroy(srt_tile *t) {
+    if (!t) return;
+    for (PanelBuf &pb : t->panels) {
+        hipFree(pb.counts);
+        hipFree(pb.cols);
+        hipFree(pb.vals);
+    }
+    t->pk.release();
+    delete t;
+}
+
+// -------------------------------------------------------------------- CSR
+int srt_csr_spmm(int m, int k, int n, int64_t nnz, const void *rowptr, const void *colind,
+                 int index_bytes, const float *vals, const float *B, int64_t ldb, float *C,
+                 int64_t ldc, int n_tile, srt_stream_t stream) {
+    if (m < 0 || n < 0 || k < 0 || nnz < 0) return fail(SRT_ERR_INVALID, "negative dimension");
+    if (index_bytes == 4 && nnz > 0x7fffffffLL) return fail(SRT_ERR_INVALID, "nnz exceeds int32 indices");
+    if (index_bytes != 4 && index_bytes != 8) return fail(SRT_ERR_INVALID, "index_bytes must be 4 or 8");
+    if (m == 0 || n == 0) return SRT_OK;
+    if (!rowptr || !colind || !vals || !B || !C) return fail(SRT_ERR_INVALID, "null array");
+    if (ldb < n || ldc < n) return fail(SRT_ERR_INVALID, "ldb and ldc must be >= n");
+    if (n_tile != 16 && n_tile != 32 && n_tile != 64 && n_tile != 128 && n_tile != 256)
+        return fail(SRT_ERR_INVALID, "n_tile must be one of 16, 32, 64, 128, 256");
+    int V = n_tile <= 64 ? 1 : n_tile / 64;
+    while (V > 1 && (n % V || ldb % V || ldc % V || !aligned(B, 4 * V) || !aligned(C, 4 * V))) V >>= 1;
+    SRT_HIP_TRY(launch_csr(m, k, n, nnz, rowptr, colind, index_bytes, vals, B, ldb, C, ldc, V,
+                           (hipStream_t)stream));
+    return SRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,513 @@
+// Device-side packer: the reference's MicroKernelPacker / SOPTile::pack_panel
+// moved onto the GPU.
+//
+//  scatter_csr     per (panel, column) nonzero code (bit r = row r of the
+//                  panel, sbench/SOP/sop_utils.py:25-30) and a dense copy of
+//                  each panel's values laid out [panel][column][row]
+//  count_codes     per panel: split every code into kernel ids through the
+//                  mapping LUT (sbench/SOP/sop_driver.py:58-65) and count
+//                  packed columns per id (nkern_counts, template:163-164)
+//  exclusive_scan  per-panel packed column / value offsets
+//  fill_codes      stable placement by id (the std::stable_sort of
+//                  template:63-73) and value compression (template:158-205)
+//
+// Stability: columns are visited in chunks of 64 in ascending order.  Inside
+// a chunk the wave repeatedly takes the first lane that still has an entry
+// left, and places, for that entry's id, every lane's matching entry at
+// cursor[id] + (rank among those lanes).  Each (chunk, id) pair is placed in
+// one step, lanes in column order, so within an id the packed order is
+// ascending column order - the same permutation a stable sort by id gives.
+#include <hip/hip_runtime.h>
+
+#include "srt_internal.h"
+
+namespace srt {
+
+namespace {
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+template <typename I>
+__global__ void scatter_csr_kernel(int m, int k, int M_r, const float *__restrict__ Lx,
+                                   const I *__restrict__ Lp, const I *__restrict__ Li,
+                                   uint32_t *__restrict__ codes, float *__restrict__ dense,
+                                   int *__restrict__ bad) {
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= m) return;
+    const int p = row / M_r, r = row - p * M_r;
+    const I beg = Lp[row], end = Lp[row + 1];
+    for (I q = beg + lane; q < end; q += 64) {
+        const I col = Li[q];
+        if (col < 0 || col >= (I)k) {
+            atomicOr(bad, 1);
+            continue;
+        }
+        const int64_t pc = (int64_t)p * k + (int64_t)col;
+        atomicOr(&codes[pc], 1u << r);
+        dense[pc * M_r + r] = Lx[q];
+    }
+}
+
+__device__ __forceinline__ void block_add_i64(unsigned long long *dst, int64_t v) {
+    atomicAdd(dst, (unsigned long long)v);
+}
+
+// One workgroup per panel: LDS histogram over kernel ids.
+__global__ void count_codes_kernel(int K, int NK, const int32_t *__restrict__ lut_ptr,
+                                   const int32_t *__restrict__ lut_ids,
+                                   const int32_t *__restrict__ id_pattern,
+                                   const uint32_t *__restrict__ codes,
+                                   int32_t *__restrict__ counts, int64_t *__restrict__ ncols,
+                                   int64_t *__restrict__ nvals) {
+    extern __shared__ int hist[];
+    __shared__ unsigned long long tot_c, tot_v;
+    const int p = blockIdx.x;
+    for (int i = threadIdx.x; i < NK; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x == 0) tot_c = tot_v = 0;
+    __syncthreads();
+    const uint32_t *pc = codes + (int64_t)p * K;
+    for (int c = threadIdx.x; c < K; c += blockDim.x) {
+        const uint32_t code = pc[c];
+        if (!code) continue;
+        for (int e = lut_ptr[code]; e < lut_ptr[code + 1]; ++e) atomicAdd(&hist[lut_ids[e]], 1);
+    }
+    __syncthreads();
+    int64_t lc = 0, lv = 0;
+    for (int i = threadIdx.x; i < NK; i += blockDim.x) {
+        const int h = hist[i];
+        counts[(int64_t)p * NK + i] = h;
+        lc += h;
+        lv += (int64_t)h * __popc((unsigned)id_pattern[i]);
+    }
+    block_add_i64(&tot_c, lc);
+    block_add_i64(&tot_v, lv);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ncols[p] = (int64_t)tot_c;
+        nvals[p] = (int64_t)tot_v;
+    }
+}
+
+// Single-workgroup exclusive scan: out[0..n] with out[n] = total.
+__global__ void exclusive_scan_kernel(int n, const int64_t *__restrict__ in,
+                                      int64_t *__restrict__ out) {
+    __shared__ int64_t sums[1024];
+    const int t = threadIdx.x, T = blockDim.x;
+    const int chunk = (n + T - 1) / T;
+    const int beg = min(n, t * chunk), end = min(n, beg + chunk);
+    int64_t s = 0;
+    for (int i = beg; i < end; ++i) s += in[i];
+    sums[t] = s;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        const int64_t v = t >= off ? sums[t - off] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t ? sums[t - 1] : 0;
+    for (int i = beg; i < end; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (t == T - 1) out[n] = sums[T - 1];
+}
+
+// Per-id start cursors (exclusive prefix over ids) for one panel, one wave.
+__device__ void id_cursors(int NK, const int32_t *__restrict__ cnt,
+                           const int32_t *__restrict__ id_pattern, int *cur_c, int *cur_v) {
+    const int lane = threadIdx.x & 63;
+    int carry_c = 0, carry_v = 0;
+    for (int base = 0; base < NK; base += 64) {
+        const int i = base + lane;
+        const int c = i < NK ? cnt[i] : 0;
+        const int v = i < NK ? c * __popc((unsigned)id_pattern[i]) : 0;
+        int sc = c, sv = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int tc = __shfl_up(sc, off, 64);
+            const int tv = __shfl_up(sv, off, 64);
+            if (lane >= off) {
+                sc += tc;
+                sv += tv;
+            }
+        }
+        if (i < NK) {
+            cur_c[i] = carry_c + sc - c;
+            cur_v[i] = carry_v + sv - v;
+        }
+        carry_c += __shfl(sc, 63, 64);
+        carry_v += __shfl(sv, 63, 64);
+    }
+}
+
+// Place one chunk of up to 64 lanes, each with a bitmask of pending entries
+// (entry j has id ids_of(j)); emit(lane_entry j, id, rank) for every entry.
+// Cursors live in LDS; one wave per workgroup.
+template <typename IdOf, typename Emit>
+__device__ void place_chunk(uint32_t pending, IdOf ids_of, int *cur_c, int *cur_v,
+                            const int32_t *__restrict__ id_pattern, Emit emit) {
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        const uint64_t active = __ballot(pending != 0);
+        if (!active) break;
+        const int leader = __ffsll((long long)active) - 1;
+        const int mine = pending ? ids_of(__ffs(pending) - 1) : 0;
+        const int lid = __shfl(mine, leader, 64);
+        int jm = -1;
+        for (uint32_t q = pending; q; q &= q - 1) {
+            const int j = __ffs(q) - 1;
+            if (ids_of(j) == lid) {
+                jm = j;
+                break;
+            }
+        }
+        const uint64_t m = __ballot(jm >= 0);
+        const int rank = __popcll(m & lanemask_lt());
+        const int base_c = cur_c[lid];
+        const int base_v = cur_v[lid];
+        const int pc = __popc((unsigned)id_pattern[lid]);
+        if (jm >= 0) {
+            pending &= ~(1u << jm);
+            emit(jm, lid, base_c + rank, base_v + rank * pc);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const int nm = __popcll(m);
+            cur_c[lid] = base_c + nm;
+            cur_v[lid] = base_v + nm * pc;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int64_t wave_sum(int64_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One 64-lane workgroup per panel.
+__global__ void __launch_bounds__(64)
+    fill_codes_kernel(int K, int col_offset, int NK, int M_r, const int32_t *__restrict__ lut_ptr,
+                      const int32_t *__restrict__ lut_ids, const int32_t *__restrict__ id_pattern,
+                      const uint32_t *__restrict__ codes, const float *__restrict__ dense,
+                      const int64_t *__restrict__ col_ptr, const int64_t *__restrict__ val_ptr,
+                      const int32_t *__restrict__ counts, int32_t *__restrict__ cols,
+                      float *__restrict__ vals, int64_t *__restrict__ padding) {
+    extern __shared__ int cur[];
+    int *cur_c = cur, *cur_v = cur + NK;
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    id_cursors(NK, counts + (int64_t)p * NK, id_pattern, cur_c, cur_v);
+    __syncthreads();
+    const int64_t cbase = col_ptr[p], vbase = val_ptr[p];
+    const uint32_t *pcodes = codes + (int64_t)p * K;
+    int64_t zeros = 0;
+    for (int chunk = 0; chunk < K; chunk += 64) {
+        const int c = chunk + lane;
+        const uint32_t code = c < K ? pcodes[c] : 0u;
+        const int e0 = code ? lut_ptr[code] : 0;
+        const int ne = code ? lut_ptr[code + 1] - e0 : 0;
+        const uint32_t pending = ne >= 32 ? 0xffffffffu : ((1u << ne) - 1u);
+        const float *d = dense + ((int64_t)p * K + c) * M_r;
+        place_chunk(
+            pending, [&](int j) { return lut_ids[e0 + j]; }, cur_c, cur_v, id_pattern,
+            [&](int, int id, int pos_c, int pos_v) {
+                cols[cbase + pos_c] = c + col_offset;
+                const unsigned pat = (unsigned)id_pattern[id];
+                float *out = vals + vbase + pos_v;
+                int k = 0;
+                for (int r = 0; r < M_r; ++r) {
+                    if ((pat >> r) & 1u) {
+                        const float v = d[r];
+                        zeros += (v == 0.0f);
+                        out[k++] = v;
+                    }
+                }
+            });
+    }
+    zeros = wave_sum(zeros);
+    if (lane == 0) padding[p] = zeros;
+}
+
+// One thread per panel: number of ids with a nonzero count.
+__global__ void group_count_kernel(int P, int NK, const int32_t *__restrict__ counts,
+                                   int64_t *__restrict__ ngroups) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int32_t *c = counts + (int64_t)p * NK;
+    int64_t g = 0;
+    for (int i = 0; i < NK; ++i) g += (c[i] != 0);
+    ngroups[p] = g;
+}
+
+// One thread per panel: (id, count) of every non-empty id, id order.
+__global__ void group_fill_kernel(int P, int NK, const int32_t *__restrict__ counts,
+                                  const int64_t *__restrict__ grp_ptr,
+                                  int32_t *__restrict__ groups) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int32_t *c = counts + (int64_t)p * NK;
+    int32_t *out = groups + 2 * grp_ptr[p];
+    for (int i = 0; i < NK; ++i) {
+        if (c[i]) {
+            out[0] = i;
+            out[1] = c[i];
+            out += 2;
+        }
+    }
+}
+
+// ------------------------------------------------------- entry-list variants
+__global__ void encode_entries_kernel(int n, int M_r, const int32_t *__restrict__ pattern_id,
+                                      const int32_t *__restrict__ pat_codes,
+                                      const int32_t *__restrict__ col_indices, int K,
+                                      int32_t *__restrict__ ids, int32_t *__restrict__ bad) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int pc = pat_codes[e];
+    if (col_indices) {
+        const int c = col_indices[e];
+        if (c < 0 || c >= K) atomicOr(bad, 2);
+    }
+    int id;
+    if (pc == 0) {
+        id = SRT_ZERO_PATTERN_ID;
+    } else if (pc < 0 || pc >= (1 << M_r) || pattern_id[pc] < 0) {
+        atomicOr(bad, 1);
+        id = SRT_ZERO_PATTERN_ID;
+    } else {
+        id = pattern_id[pc];
+    }
+    ids[e] = id;
+}
+
+// counts[NK + 1]: the last bucket counts zero patterns.
+__global__ void count_entries_kernel(int n, int NK, const int32_t *__restrict__ ids,
+                                     const int32_t *__restrict__ id_pattern,
+                                     int32_t *__restrict__ counts, int64_t *__restrict__ ncols,
+                                     int64_t *__restrict__ nvals) {
+    extern __shared__ int hist[];
+    __shared__ unsigned long long tot_c, tot_v;
+    for (int i = threadIdx.x; i <= NK; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x == 0) tot_c = tot_v = 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int id = ids[e];
+        atomicAdd(&hist[id == SRT_ZERO_PATTERN_ID ? NK : id], 1);
+    }
+    __syncthreads();
+    int64_t lc = 0, lv = 0;
+    for (int i = threadIdx.x; i <= NK; i += blockDim.x) {
+        const int h = hist[i];
+        counts[i] = h;
+        if (i < NK) {
+            lc += h;
+            lv += (int64_t)h * __popc((unsigned)id_pattern[i]);
+        }
+    }
+    block_add_i64(&tot_c, lc);
+    block_add_i64(&tot_v, lv);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *ncols = (int64_t)tot_c;
+        *nvals = (int64_t)tot_v;
+    }
+}
+
+__global__ void __launch_bounds__(64)
+    fill_entries_kernel(int n, int col_offset, int K, int NK, int M_r,
+                        const int32_t *__restrict__ id_pattern, const int32_t *__restrict__ ids,
+                        const int32_t *__restrict__ col_indices,
+                        const float *__restrict__ dense_panel, const int32_t *__restrict__ counts,
+                        int32_t *__restrict__ cols, float *__restrict__ vals,
+                        int64_t *__restrict__ padding) {
+    extern __shared__ int cur[];
+    int *cur_c = cur, *cur_v = cur + NK;
+    const int lane = threadIdx.x & 63;
+    id_cursors(NK, counts, id_pattern, cur_c, cur_v);
+    __syncthreads();
+    int64_t zeros = 0;
+    for (int chunk = 0; chunk < n; chunk += 64) {
+        const int e = chunk + lane;
+        const int id = e < n ? ids[e] : SRT_ZERO_PATTERN_ID;
+        const uint32_t pending = (id != SRT_ZERO_PATTERN_ID) ? 1u : 0u;
+        const int col = e < n ? col_indices[e] : 0;
+        place_chunk(
+            pending, [&](int) { return id; }, cur_c, cur_v, id_pattern,
+            [&](int, int lid, int pos_c, int pos_v) {
+                cols[pos_c] = col + col_offset;
+                const unsigned pat = (unsigned)id_pattern[lid];
+                const float *d = dense_panel + (int64_t)col * M_r;
+                int k = 0;
+                for (int r = 0; r < M_r; ++r) {
+                    if ((pat >> r) & 1u) {
+                        const float v = d[r];
+                        zeros += (v == 0.0f);
+                        vals[pos_v + k++] = v;
+                    }
+                }
+            });
+    }
+    zeros = wave_sum(zeros);
+    if (lane == 0) *padding = zeros;
+}
+
+// Number of nonzero floats in x[0..n), single workgroup.
+__global__ void count_nonzero_kernel(const float *__restrict__ x, int64_t n,
+                                     int64_t *__restrict__ out) {
+    __shared__ unsigned long long tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    int64_t c = 0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) c += (x[i] != 0.0f);
+    atomicAdd(&tot, (unsigned long long)c);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (int64_t)tot;
+}
+
+// Stable order by id with the zero pattern last (sort_encoded_patterns,
+// template:63-73): bucket NK holds the zero pattern.
+__global__ void __launch_bounds__(64)
+    stable_order_kernel(int n, int NK, const int32_t *__restrict__ ids,
+                        const int32_t *__restrict__ counts, int32_t *__restrict__ encoded_sorted,
+                        int32_t *__restrict__ order) {
+    extern __shared__ int cur[];
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        int run = 0;
+        for (int i = 0; i <= NK; ++i) {
+            cur[i] = run;
+            run += counts[i];
+        }
+    }
+    __syncthreads();
+    for (int chunk = 0; chunk < n; chunk += 64) {
+        const int e = chunk + lane;
+        const int id = e < n ? ids[e] : -1;
+        const int bucket = id == SRT_ZERO_PATTERN_ID ? NK : id;
+        uint32_t pending = e < n ? 1u : 0u;
+        while (true) {
+            const uint64_t active = __ballot(pending != 0);
+            if (!active) break;
+            const int leader = __ffsll((long long)active) - 1;
+            const int lb = __shfl(bucket, leader, 64);
+            const bool mine = pending && bucket == lb;
+            const uint64_t m = __ballot(mine);
+            const int rank = __popcll(m & lanemask_lt());
+            const int base = cur[lb];
+            if (mine) {
+                pending = 0;
+                encoded_sorted[base + rank] = id;
+                order[base + rank] = e;
+            }
+            __syncthreads();
+            if (lane == 0) cur[lb] = base + __popcll(m);
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_scatter_csr(int m, int k, int M_r, const float *Lx, const void *Lp,
+                              const void *Li, int index_bytes, uint32_t *codes, float *dense,
+                              int *bad, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    const int waves = 4;
+    const dim3 grid((m + waves - 1) / waves), block(64 * waves);
+    if (index_bytes == 8)
+        hipLaunchKernelGGL(scatter_csr_kernel<int64_t>, grid, block, 0, s, m, k, M_r, Lx,
+                           (const int64_t *)Lp, (const int64_t *)Li, codes, dense, bad);
+    else
+        hipLaunchKernelGGL(scatter_csr_kernel<int32_t>, grid, block, 0, s, m, k, M_r, Lx,
+                           (const int32_t *)Lp, (const int32_t *)Li, codes, dense, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_codes(int P, int K, const DeviceLut &lut, const uint32_t *codes,
+                              int32_t *counts, int64_t *ncols, int64_t *nvals, hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    hipLaunchKernelGGL(count_codes_kernel, dim3(P), dim3(256), sizeof(int) * lut.num_patterns, s,
+                       K, lut.num_patterns, lut.lut_ptr, lut.lut_ids, lut.id_pattern, codes,
+                       counts, ncols, nvals);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_codes(int P, int K, int col_offset, const DeviceLut &lut,
+                             const uint32_t *codes, const float *dense, const int64_t *col_ptr,
+                             const int64_t *val_ptr, const int32_t *counts, int32_t *cols,
+                             float *vals, int64_t *padding, hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_codes_kernel, dim3(P), dim3(64), 2 * sizeof(int) * lut.num_patterns,
+                       s, K, col_offset, lut.num_patterns, lut.M_r, lut.lut_ptr, lut.lut_ids,
+                       lut.id_pattern, codes, dense, col_ptr, val_ptr, counts, cols, vals,
+                       padding);
+    return hipGetLastError();
+}
+
+hipError_t launch_exclusive_scan(int n, const int64_t *in, int64_t *out, hipStream_t s) {
+    hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, s, n, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_count(int P, int NK, const int32_t *counts, int64_t *ngroups,
+                              hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    hipLaunchKernelGGL(group_count_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, NK, counts,
+                       ngroups);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t *grp_ptr,
+                             int32_t *groups, hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    hipLaunchKernelGGL(group_fill_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, NK, counts,
+                       grp_ptr, groups);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode_entries(int n, const DeviceLut &lut, const int32_t *pat_codes,
+                                 const int32_t *col_indices, int K, int32_t *ids, int32_t *bad,
+                                 hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(encode_entries_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, lut.M_r,
+                       lut.pattern_id, pat_codes, col_indices, K, ids, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_nonzero(const float *x, int64_t n, int64_t *out, hipStream_t s) {
+    hipLaunchKernelGGL(count_nonzero_kernel, dim3(1), dim3(1024), 0, s, x, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_entries(int n, const DeviceLut &lut, const int32_t *ids, int32_t *counts,
+                                int64_t *ncols, int64_t *nvals, hipStream_t s) {
+    hipLaunchKernelGGL(count_entries_kernel, dim3(1), dim3(256),
+                       sizeof(int) * (lut.num_patterns + 1), s, n, lut.num_patterns, ids,
+                       lut.id_pattern, counts, ncols, nvals);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_entries(int n, int col_offset, int K, const DeviceLut &lut,
+                               const int32_t *ids, const int32_t *col_indices,
+                               const float *dense_panel, const int32_t *counts, int32_t *cols,
+                               float *vals, int64_t *padding, hipStream_t s) {
+    hipLaunchKernelGGL(fill_entries_kernel, dim3(1), dim3(64), 2 * sizeof(int) * lut.num_patterns,
+                       s, n, col_offset, K, lut.num_patterns, lut.M_r, lut.id_pattern, ids,
+                       col_indices, dense_panel, counts, cols, vals, padding);
+    return hipGetLastError();
+}
+
+hipError_t launch_stable_order(int n, int NK, const int32_t *ids, const int32_t *counts,
+                               int32_t *encoded_sorted, int32_t *order, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(stable_order_kernel, dim3(1), dim3(64), sizeof(int) * (NK + 1), s, n, NK,
+                       ids, counts, encoded_sorted, order);
+    return hipGetLastError();
+}
+
+}  // namespace srt

@@ -1,0 +1,128 @@
+// Internal declarations shared by the libsrt translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "srt.h"
+
+namespace srt {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define SRT_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return ::srt::fail(SRT_ERR_HIP, std::string(#expr " failed: ") +          \
+                                                hipGetErrorString(_e));                \
+    } while (0)
+
+// ------------------------------------------------------ generated executors
+// Arguments of one executor launch over a packed matrix.
+struct ExecLaunch {
+    const int32_t *groups;   // (id, count) pairs of non-empty groups, id order
+    const int64_t *grp_ptr;  // [P + 1] range of panel p in groups (in pairs)
+    const int64_t *col_ptr;  // [P + 1]
+    const int64_t *val_ptr;  // [P + 1]
+    const int32_t *cols;
+    const float *vals;
+    const float *B;
+    uint32_t b_bytes;  // bytes of B addressable through the buffer resource
+    float *C;
+    int64_t ldb, ldc;
+    int M, N, num_panels;
+    int load_c;
+    int vec_width;        // V
+    int waves_per_block;  // panels per workgroup
+    int xcd_remap;
+    int unroll;
+};
+
+typedef hipError_t (*ExecLaunchFn)(const ExecLaunch &, hipStream_t);
+
+// One generated executor (one pattern set).  launch[v][u] for V = 1,2,4 and
+// unroll = 1,2,4,8 (index log2); a null entry is a variant not generated.
+struct ExecutorDesc {
+    const char *id;
+    int M_r;
+    int num_patterns;
+    const int32_t *patterns;  // id order
+    ExecLaunchFn launch[3][4];
+};
+
+void register_executor(const ExecutorDesc *d);
+const std::vector<const ExecutorDesc *> &executors();
+const ExecutorDesc *find_executor(int M_r, const std::vector<int32_t> &patterns);
+const ExecutorDesc *find_executor_by_id(const char *id);
+
+// ---------------------------------------------------------------- packing
+struct DeviceLut {
+    int M_r = 0;
+    int num_patterns = 0;
+    int32_t *lut_ptr = nullptr;       // [2^M_r + 1]  code -> range in lut_ids
+    int32_t *lut_ids = nullptr;       // kernel ids in mapping-list order
+    int32_t *id_pattern = nullptr;    // [NK] id -> bitmask
+    int32_t *pattern_id = nullptr;    // [2^M_r] bitmask -> id or -1
+    int device = -1;
+};
+
+// Packed matrix in device memory (flat arrays over all panels).
+struct Packed {
+    int M = 0, K = 0, M_r = 0, P = 0, NK = 0;
+    int32_t *counts = nullptr;   // [P * NK]  nkern_counts of every panel
+    int64_t *col_ptr = nullptr;  // [P + 1]
+    int64_t *val_ptr = nullptr;  // [P + 1]
+    int32_t *cols = nullptr;
+    float *vals = nullptr;
+    int32_t *groups = nullptr;   // (id, count) pairs of non-empty ids per panel
+    int64_t *grp_ptr = nullptr;  // [P + 1]
+    int64_t total_cols = 0, total_vals = 0, total_groups = 0, padding = 0, nnz = 0;
+    void release();
+    size_t bytes() const;
+};
+
+// Kernel launchers (packer.hip).
+hipError_t launch_scatter_csr(int m, int k, int M_r, const float *Lx, const void *Lp,
+                              const void *Li, int index_bytes, uint32_t *codes, float *dense,
+                              int *bad, hipStream_t s);
+hipError_t launch_count_codes(int P, int K, const DeviceLut &lut, const uint32_t *codes,
+                              int32_t *counts, int64_t *ncols, int64_t *nvals, hipStream_t s);
+hipError_t launch_fill_codes(int P, int K, int col_offset, const DeviceLut &lut,
+                             const uint32_t *codes, const float *dense, const int64_t *col_ptr,
+                             const int64_t *val_ptr, const int32_t *counts, int32_t *cols,
+                             float *vals, int64_t *padding, hipStream_t s);
+hipError_t launch_exclusive_scan(int n, const int64_t *in, int64_t *out, hipStream_t s);
+// Non-empty group lists from counts: ngroups[p], then (after a scan into
+// grp_ptr) the (id, count) pairs.
+hipError_t launch_group_count(int P, int NK, const int32_t *counts, int64_t *ngroups,
+                              hipStream_t s);
+hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t *grp_ptr,
+                             int32_t *groups, hipStream_t s);
+// Entry-list variants (SOPTile path): one panel, explicit (pattern, column) entries.
+// bad |= 1 for a pattern outside the set, |= 2 for a column outside [0, K)
+// (col_indices may be null to skip the column check).
+hipError_t launch_encode_entries(int n, const DeviceLut &lut, const int32_t *pat_codes,
+                                 const int32_t *col_indices, int K, int32_t *ids, int32_t *bad,
+                                 hipStream_t s);
+hipError_t launch_count_nonzero(const float *x, int64_t n, int64_t *out, hipStream_t s);
+hipError_t launch_count_entries(int n, const DeviceLut &lut, const int32_t *ids,
+                                int32_t *counts, int64_t *ncols, int64_t *nvals, hipStream_t s);
+hipError_t launch_fill_entries(int n, int col_offset, int K, const DeviceLut &lut,
+                               const int32_t *ids, const int32_t *col_indices,
+                               const float *dense_panel, const int32_t *counts, int32_t *cols,
+                               float *vals, int64_t *padding, hipStream_t s);
+hipError_t launch_stable_order(int n, int NK, const int32_t *ids, const int32_t *counts,
+                               int32_t *encoded_sorted, int32_t *order, hipStream_t s);
+
+// CSR baseline (csr_spmm.hip).
+hipError_t launch_csr(int m, int k, int n, int64_t nnz, const void *rowptr, const void *colind,
+                      int index_bytes, const float *vals, const float *B, int64_t ldb, float *C,
+                      int64_t ldc, int vec_width, hipStream_t s);
+
+}  // namespace srt

@@ -1,0 +1,152 @@
+"""Host-side checks that need no GPU: the pattern-set generator and mapping IO
+of the package, the executor code generator, and the C ABI of libsrt.so
+(loads, exports every declared symbol, host-only calls behave)."""
+import ctypes as ct
+import glob
+import os
+import re
+
+import pytest
+
+from conftest import GOLDEN_MAPPINGS, ROOT
+
+
+# ----------------------------------------------------------- mapping module
+def test_shipped_strategies_reproduce_reference_ids(srt, tmp_path):
+    ids = srt.mapping.generate_shipped_mappings(str(tmp_path))
+    assert ids["M4_identity"] == "61fee"
+    assert ids["M4_orig"] == "da01e"
+    assert ids["M8_orig"] == "400fa"
+    assert ids["M8_alt"] == "747f9"
+    assert ids["M8_identity"] == "161fb"
+    assert ids["M2_identity"] == "4af51"
+    assert ids["M6_identity"] == "001e3"
+    for name, mid in ids.items():
+        golden = os.path.join(GOLDEN_MAPPINGS, "mapping_%s.txt" % mid)
+        if os.path.exists(golden):
+            assert open(tmp_path / ("mapping_%s.txt" % mid)).read() == open(golden).read()
+
+
+def test_named_mappings(srt):
+    assert srt.named_mapping(4, "identity").id == "61fee"
+    assert srt.named_mapping(4, "orig").id == "da01e"
+    assert srt.named_mapping(8, "orig").id == "400fa"
+    assert srt.named_mapping(8, "alt").id == "747f9"
+    m = srt.named_mapping(8, "orig")
+    assert m.patterns == [1, 2, 4, 8, 16, 32, 64, 128, 15, 60, 85, 170, 195, 240, 63, 207, 243, 252, 255]
+    assert m(0) == [] and m(3) == [195]
+
+
+def test_load_mapping_like_reference(srt):
+    acc, patterns, m = srt.load_mapping("400fa", [GOLDEN_MAPPINGS])
+    assert acc["M_r"] == 8 and acc["N_r"]["avx512"] == 2
+    assert patterns == set(m.patterns)
+
+
+def test_sop_orig_matches_generator_rules(srt):
+    s = srt.SOPOrig([0b1111], [1, 2, 4, 8], split_cost=10000)
+    assert s.map_pattern(0) == [0]
+    assert s.map_pattern(0b0001) == [0b0001]          # below min_merge
+    assert s.map_pattern(0b0101) == [0b1111]          # merge with padding
+    r = srt.SOPOrig([0b0011, 0b1100], [1, 2, 4, 8], split_cost=2.1, recursive=True)
+    assert r.map_pattern(0b1111) == [0b0011, 0b1100]
+    assert srt.gen_all_combos(4, 3) == [7, 11, 13, 14]
+
+
+def test_code_arrays(srt):
+    m = srt.named_mapping(4, "orig")
+    ptr, flat = m.code_arrays()
+    assert len(ptr) == 17 and ptr[0] == 0 and ptr[1] == 0
+    for code in range(1, 16):
+        assert flat[ptr[code]:ptr[code + 1]] == m.table[code]
+
+
+# ---------------------------------------------------------------- codegen
+def test_codegen_sources_are_current(srt):
+    cg = srt.mapping  # noqa: F841
+    import importlib
+    codegen = importlib.import_module("sparse-register-tiling_amd.codegen")
+    gen_dir = os.path.join(ROOT, "sparse-register-tiling_amd", "csrc", "generated")
+    files = sorted(glob.glob(os.path.join(gen_dir, "exec_*.hip")))
+    assert files, "build() has not generated executors"
+    for f in files:
+        text = open(f).read()
+        M_r = int(re.search(r"M_r = (\d+)", text).group(1))
+        pats = [int(x) for x in re.search(r"k_patterns\[\] = \{([^}]*)\}", text).group(1).split(",")]
+        assert text == codegen.gen_executor_source(M_r, pats)
+        assert os.path.basename(f) == "exec_%s.hip" % codegen.executor_id(M_r, pats)
+
+
+def test_codegen_rejects_bad_sets():
+    import importlib
+    codegen = importlib.import_module("sparse-register-tiling_amd.codegen")
+    with pytest.raises(ValueError):
+        codegen.gen_executor_source(4, [])
+    with pytest.raises(ValueError):
+        codegen.gen_executor_source(4, [16])
+    assert codegen.canonical_patterns([15, 1, 3, 2]) == [1, 2, 3, 15]
+
+
+# ------------------------------------------------------------------ C ABI
+def _declared_symbols():
+    hdr = open(os.path.join(ROOT, "include", "srt.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(srt_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_every_declared_symbol_is_exported(srt):
+    L = srt._lib.lib()
+    syms = _declared_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) <= set(srt._lib.SIGNATURES), set(syms) - set(srt._lib.SIGNATURES)
+
+
+def test_c_mapping_reader_on_golden_files(srt):
+    for f in sorted(glob.glob(os.path.join(GOLDEN_MAPPINGS, "mapping_*.txt"))):
+        mid = os.path.basename(f)[len("mapping_"):-4]
+        nm = srt.NativeMapping.from_id(mid, [GOLDEN_MAPPINGS])
+        py = srt.read_mapping_file(f)
+        assert nm.M_r == py.M_r
+        assert nm.patterns == py.patterns
+
+
+def test_shipped_executors_cover_named_mappings(srt):
+    ids = {e[0] for e in srt.executors()}
+    for M_r, name in [(4, "identity"), (4, "orig"), (8, "orig"), (8, "alt")]:
+        nm = srt.resolve_mapping(name, M_r=M_r)
+        assert nm.executor_id() in ids
+
+
+def test_c_errors(srt):
+    L = srt._lib.lib()
+    h = ct.c_void_p()
+    rc = L.srt_mapping_read(b"nonexistent", (ct.c_char_p * 1)(GOLDEN_MAPPINGS.encode()), 1, ct.byref(h))
+    assert rc == srt._lib.SRT_ERR_IO
+    assert "not found" in srt._lib.last_error()
+    # overlapping kernel patterns would add a value twice
+    ptr = [0, 0, 1, 2, 4] + [4] * 13
+    pats = [1, 2, 3, 1]
+    rc = L.srt_mapping_create(4, srt._lib.int32_array(ptr[:17]), srt._lib.int32_array(pats), ct.byref(h))
+    assert rc == srt._lib.SRT_ERR_INVALID
+    # a pattern set without a generated executor
+    m = srt.Mapping(3, {c: [c] for c in range(1, 8)})
+    nm = srt.NativeMapping.from_mapping(m)
+    with pytest.raises(srt.SrtError) as e:
+        nm.executor_id()
+    assert e.value.code == srt._lib.SRT_ERR_UNSUPPORTED
+    # plan creation argument checks happen before any device work
+    plan = ct.c_void_p()
+    rc = L.srt_plan_create(4, 4, 4, None, None, None, 3, None, 1, None, nm.handle, None, ct.byref(plan))
+    assert rc == srt._lib.SRT_ERR_INVALID
+    rc = L.srt_csr_spmm(4, 4, 4, 0, None, None, 4, None, None, 4, None, 4, 48, None)
+    assert rc == srt._lib.SRT_ERR_INVALID
+
+
+def test_default_tile_config(srt):
+    c = srt._lib.TileConfigC()
+    srt._lib.lib().srt_tile_config_default(ct.byref(c))
+    t = srt.TileConfig.from_c(c)
+    assert (t.M_c, t.K_c, t.N_c, t.beta, t.max_tlb_entries, t.tlb_page_size) == (16, 256, 64, 1.0, 64, 4096)
+    assert srt.TileConfig.from_c(t.to_c()) == t
