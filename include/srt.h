@@ -67,9 +67,17 @@ typedef struct srt_tile_config {
     int tlb_page_size;     /* recorded                                                */
     int runtime_schedule;  /* reference outer schedule id (KNM=0 ...) (recorded)      */
     int vec_width;         /* V in {1,2,4}: fp32 columns per lane; 0 = choose        */
-    int waves_per_block;   /* panels per workgroup, 1..8; 0 = choose                 */
+    int waves_per_block;   /* wave slots per workgroup, 1..16; a panel's waves share
+                              one workgroup, light panels are packed together;
+                              0 = choose                                            */
     int xcd_remap;         /* 1: group workgroups of one N tile on one XCD; -1 = choose */
     int unroll;            /* packed columns in flight per wave: 1,2,4,8; 0 = choose  */
+    int nsplit;            /* most waves one panel's packed columns are split over,
+                              1..waves_per_block (partial sums are added in chunk
+                              order; 1 = the reference's single chain); 0 = choose */
+    int chunk_cols;        /* target packed columns per wave: panel p runs on
+                              kp = min(nsplit, max(1, ceil(ncols_p / chunk_cols)))
+                              waves (load balance); 0 = choose                      */
 } srt_tile_config;
 
 /* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54). */
@@ -141,6 +149,14 @@ int srt_plan_allocate_executor(srt_plan *p, int n);
 int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, int64_t ldb,
                      int n, srt_stream_t stream);
 
+/* Waves each panel's packed columns are split over (num_panels entries) in
+ * the current launch schedule: chunk j of kp covers packed columns
+ * [floor(j*n/kp), floor((j+1)*n/kp)) and C = chain_0 + chain_1 + ... in
+ * order.  The load-balanced kernel descriptors of the reference
+ * (KD_IntelLoadBalanced, nano/runtime_yaml_config.register:85-87) have no
+ * published partition; this one is stated so callers can reproduce the order. */
+int srt_plan_split_counts(const srt_plan *p, int32_t *kp);
+
 typedef struct srt_plan_stats {
     int M, K, M_r, num_panels, num_patterns;
     int64_t nnz;            /* stored entries of A                           */
@@ -192,6 +208,9 @@ int srt_tile_padding(const srt_tile *t, int64_t *padding);
  * accumulates into C. */
 int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, float *C,
                      int64_t ldc, int load_c, const srt_tile_config *cfg, srt_stream_t stream);
+
+/* Split counts (as srt_plan_split_counts) of the tile's last executed configuration. */
+int srt_tile_split_counts(const srt_tile *t, int32_t *kp);
 
 int srt_tile_copy_packed(const srt_tile *t, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
                          int32_t *cols, float *vals, int64_t *packed_cols, int64_t *packed_values);

@@ -44,7 +44,8 @@ def lib():
     L.srt_oracle_pack_fill.restype = ct.c_longlong
     L.srt_oracle_exec.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, _i32p, _i32p, _i64p, _i64p,
                                   _i32p, _f32p, _f32p, ct.c_longlong, _f32p, ct.c_longlong,
-                                  ct.c_int, ct.c_int]
+                                  ct.c_int, _i32p, ct.c_int]
+    L.srt_oracle_isa.restype = ct.c_int
     L.srt_oracle_exec.restype = ct.c_int
     L.srt_oracle_csr.argtypes = [ct.c_int, ct.c_int, _i64p, _i32p, _f32p, _f32p, ct.c_longlong,
                                  _f32p, ct.c_longlong, ct.c_int]
@@ -100,11 +101,16 @@ def pack_dense(A: np.ndarray, M_r: int, table: Dict[int, List[int]], col_offset:
 
 
 def exec_packed(packed, B: np.ndarray, nthreads: int = 0, C: np.ndarray | None = None,
-                load_c: bool = False) -> np.ndarray:
-    """Run the restated executor over a ``pack_dense`` result."""
+                load_c: bool = False, nsplit=1) -> np.ndarray:
+    """Run the restated executor over a ``pack_dense`` result.  ``nsplit`` is
+    the number of chunks per panel, one int for every panel or an array with
+    one entry per panel: 1 is the reference's single chain per panel; k > 1
+    sums the chains of k contiguous chunks left to right (the GPU executor's
+    split order)."""
     B = np.ascontiguousarray(B, dtype=np.float32)
     M, M_r, NK = packed["M"], packed["M_r"], packed["NK"]
     N = B.shape[1]
+    kp = np.ascontiguousarray(np.broadcast_to(np.asarray(nsplit, dtype=np.int32), (max(1, packed["P"]),)))
     if C is None:
         C = np.zeros((M, N), dtype=np.float32)
     counts = np.ascontiguousarray(packed["counts"], dtype=np.int32).reshape(-1)
@@ -116,10 +122,15 @@ def exec_packed(packed, B: np.ndarray, nthreads: int = 0, C: np.ndarray | None =
     rc = lib().srt_oracle_exec(M, N, M_r, NK, id_pat, counts, packed["col_ptr"], packed["val_ptr"],
                                np.ascontiguousarray(cols, dtype=np.int32),
                                np.ascontiguousarray(vals, dtype=np.float32), B, B.shape[1], C, N,
-                               1 if load_c else 0, nthreads)
+                               1 if load_c else 0, kp, nthreads)
     if rc:
-        raise MemoryError("oracle executor allocation failed")
+        raise MemoryError("oracle executor failed (%d)" % rc)
     return C
+
+
+def isa() -> str:
+    """Inner-loop variant the C oracle runs on this host."""
+    return {2: "avx512", 1: "avx2+fma", 0: "x86-64"}[lib().srt_oracle_isa()]
 
 
 def csr_exec(rowptr, colind, vals, M: int, B: np.ndarray, nthreads: int = 0) -> np.ndarray:

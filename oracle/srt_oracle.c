@@ -120,7 +120,12 @@ long long srt_oracle_pack_fill(int M, int K, const float *A, long long lda, int 
                     if (!(pat & 1u)) continue;
                     int row = p * M_r + r;
                     float v = row < M ? A[(long long)row * lda + c] : 0.0f;
-                    if (v == 0.0f) padding++;
+                    /* A position the pattern pads is not a stored entry of A:
+                     * it holds +0.0f whatever the sign of a dense zero. */
+                    if (v == 0.0f) {
+                        padding++;
+                        v = 0.0f;
+                    }
                     vals[cur_v[id]++] = v;
                 }
             }
@@ -132,83 +137,144 @@ long long srt_oracle_pack_fill(int M, int K, const float *A, long long lda, int 
 }
 
 /*
- * Executor: C[rows, 0:N] (+)= packed A . B.  With load_c == 0 the
- * accumulators start at zero, otherwise at the current C (template:281-297).
+ * Executor: C[rows, 0:N] (+)= packed A . B (template:281-297).
+ *
+ * nsplit == 1 is the reference's order: one fused multiply-add chain per
+ * output element over the panel's packed columns, starting from C (load_c)
+ * or zero.  For nsplit > 1 the panel's packed columns are cut into nsplit
+ * contiguous chunks, chunk w = [floor(w*n/nsplit), floor((w+1)*n/nsplit)),
+ * each chunk runs its own chain (chunk 0 from C or zero, the others from
+ * zero) and the chunk results are added left to right - the order the GPU
+ * executor uses when it spreads one panel over nsplit wavefronts.
  * Rows of the last panel past M are computed and discarded.
+ *
+ * The inner loops are compiled for AVX-512, AVX2+FMA and baseline x86-64 and
+ * chosen by CPU feature at first use; fmaf is exact in every variant.
  */
-__attribute__((target_clones("arch=skylake-avx512", "arch=haswell", "default")))
-static void exec_panel(int M, int N, int M_r, int NK, const int *id_pattern, const int *cnt,
-                       const int *cols, const float *vals, const float *B, long long ldb,
-                       float *C, long long ldc, int load_c, int p, float *acc) {
-    for (int r = 0; r < M_r; ++r) {
-        int row = p * M_r + r;
-        float *a = acc + (long long)r * N;
-        if (load_c && row < M) memcpy(a, C + (long long)row * ldc, sizeof(float) * (size_t)N);
-        else memset(a, 0, sizeof(float) * (size_t)N);
+#define SRT_EXEC_PANEL(NAME, ATTR)                                                               \
+    ATTR static void NAME(int M, int N, int M_r, int NK, const int *id_pattern, const int *cnt,  \
+                          const int *cols, const float *vals, const float *B, long long ldb,     \
+                          float *C, long long ldc, int load_c, int p, int nsplit, long long ncols, \
+                          float *acc) {                                                          \
+        for (int w = 0; w < nsplit; ++w)                                                         \
+            for (int r = 0; r < M_r; ++r) {                                                      \
+                int row = p * M_r + r;                                                           \
+                float *a = acc + ((long long)w * M_r + r) * N;                                   \
+                if (w == 0 && load_c && row < M)                                                 \
+                    memcpy(a, C + (long long)row * ldc, sizeof(float) * (size_t)N);              \
+                else                                                                             \
+                    memset(a, 0, sizeof(float) * (size_t)N);                                     \
+            }                                                                                    \
+        long long c = 0, v = 0;                                                                  \
+        int w = 0;                                                                               \
+        long long w_end = ncols / nsplit;                                                        \
+        for (int id = 0; id < NK; ++id) {                                                        \
+            unsigned pattern = (unsigned)id_pattern[id];                                         \
+            for (int i = 0; i < cnt[id]; ++i, ++c) {                                             \
+                while (c >= w_end && w < nsplit - 1) {                                           \
+                    ++w;                                                                         \
+                    w_end = (long long)(w + 1) * ncols / nsplit;                                 \
+                }                                                                                \
+                const float *b = B + (long long)cols[c] * ldb;                                   \
+                unsigned pat = pattern;                                                          \
+                for (int r = 0; pat; ++r, pat >>= 1) {                                           \
+                    if (!(pat & 1u)) continue;                                                   \
+                    float av = vals[v++];                                                        \
+                    float *a = acc + ((long long)w * M_r + r) * N;                               \
+                    for (int j = 0; j < N; ++j) a[j] = fmaf(av, b[j], a[j]);                     \
+                }                                                                                \
+            }                                                                                    \
+        }                                                                                        \
+        for (int r = 0; r < M_r; ++r) {                                                          \
+            int row = p * M_r + r;                                                               \
+            float *a = acc + (long long)r * N;                                                   \
+            for (int u = 1; u < nsplit; ++u) {                                                   \
+                const float *q = acc + ((long long)u * M_r + r) * N;                             \
+                for (int j = 0; j < N; ++j) a[j] = a[j] + q[j];                                  \
+            }                                                                                    \
+            if (row < M) memcpy(C + (long long)row * ldc, a, sizeof(float) * (size_t)N);         \
+        }                                                                                        \
     }
-    long long c = 0, v = 0;
-    for (int id = 0; id < NK; ++id) {
-        unsigned pattern = (unsigned)id_pattern[id];
-        for (int i = 0; i < cnt[id]; ++i, ++c) {
-            const float *b = B + (long long)cols[c] * ldb;
-            unsigned pat = pattern;
-            for (int r = 0; pat; ++r, pat >>= 1) {
-                if (!(pat & 1u)) continue;
-                float av = vals[v++];
-                float *a = acc + (long long)r * N;
-                for (int j = 0; j < N; ++j) a[j] = fmaf(av, b[j], a[j]);
-            }
-        }
+
+#define SRT_CSR_ROW(NAME, ATTR)                                                                  \
+    ATTR static void NAME(int N, const long long *rowptr, const int *colind, const float *vals,  \
+                          const float *B, long long ldb, float *crow, int i) {                   \
+        memset(crow, 0, sizeof(float) * (size_t)N);                                              \
+        for (long long q = rowptr[i]; q < rowptr[i + 1]; ++q) {                                  \
+            const float *b = B + (long long)colind[q] * ldb;                                     \
+            float av = vals[q];                                                                  \
+            for (int j = 0; j < N; ++j) crow[j] = fmaf(av, b[j], crow[j]);                       \
+        }                                                                                        \
     }
-    for (int r = 0; r < M_r; ++r) {
-        int row = p * M_r + r;
-        if (row < M) memcpy(C + (long long)row * ldc, acc + (long long)r * N, sizeof(float) * (size_t)N);
-    }
+
+SRT_EXEC_PANEL(exec_panel_avx512, __attribute__((target("avx512f,avx512vl,fma"))))
+SRT_EXEC_PANEL(exec_panel_avx2, __attribute__((target("avx2,fma"))))
+SRT_EXEC_PANEL(exec_panel_base, )
+SRT_CSR_ROW(csr_row_avx512, __attribute__((target("avx512f,avx512vl,fma"))))
+SRT_CSR_ROW(csr_row_avx2, __attribute__((target("avx2,fma"))))
+SRT_CSR_ROW(csr_row_base, )
+
+typedef void (*exec_panel_fn)(int, int, int, int, const int *, const int *, const int *,
+                              const float *, const float *, long long, float *, long long, int, int,
+                              int, long long, float *);
+typedef void (*csr_row_fn)(int, const long long *, const int *, const float *, const float *,
+                           long long, float *, int);
+
+static int cpu_level(void) {
+    __builtin_cpu_init();
+    if (__builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512vl") &&
+        __builtin_cpu_supports("fma"))
+        return 2;
+    if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) return 1;
+    return 0;
 }
 
+/* Which inner-loop variant runs on this host: 2 AVX-512, 1 AVX2+FMA, 0 base. */
+SRT_ORACLE_API int srt_oracle_isa(void) { return cpu_level(); }
+
+/* kp[p] = number of chunks panel p is split into (NULL: one chunk each). */
 SRT_ORACLE_API
 int srt_oracle_exec(int M, int N, int M_r, int NK, const int *id_pattern,
                     const int *counts, const long long *col_ptr, const long long *val_ptr,
                     const int *cols, const float *vals, const float *B, long long ldb,
-                    float *C, long long ldc, int load_c, int nthreads) {
+                    float *C, long long ldc, int load_c, const int *kp, int nthreads) {
     int P = (M + M_r - 1) / M_r;
     if (nthreads <= 0) nthreads = omp_get_max_threads();
+    int kmax = 1;
+    for (int p = 0; kp && p < P; ++p) {
+        if (kp[p] < 1) return -2;
+        if (kp[p] > kmax) kmax = kp[p];
+    }
+    int lvl = cpu_level();
+    exec_panel_fn fn = lvl == 2 ? exec_panel_avx512 : lvl == 1 ? exec_panel_avx2 : exec_panel_base;
     int err = 0;
 #pragma omp parallel num_threads(nthreads)
     {
-        float *acc = (float *)aligned_alloc(64, ((sizeof(float) * (size_t)M_r * (size_t)N) + 63) / 64 * 64);
+        size_t bytes = sizeof(float) * (size_t)kmax * (size_t)M_r * (size_t)(N > 0 ? N : 1);
+        float *acc = (float *)aligned_alloc(64, (bytes + 63) / 64 * 64);
         if (!acc) {
 #pragma omp atomic write
             err = 1;
         } else {
 #pragma omp for schedule(static)
             for (int p = 0; p < P; ++p)
-                exec_panel(M, N, M_r, NK, id_pattern, counts + (long long)p * NK, cols + col_ptr[p],
-                           vals + val_ptr[p], B, ldb, C, ldc, load_c, p, acc);
+                fn(M, N, M_r, NK, id_pattern, counts + (long long)p * NK, cols + col_ptr[p],
+                   vals + val_ptr[p], B, ldb, C, ldc, load_c, p, kp ? kp[p] : 1,
+                   col_ptr[p + 1] - col_ptr[p], acc);
             free(acc);
         }
     }
     return err ? -1 : 0;
 }
 
-/* CSR executor: per row, one fmaf per stored entry in storage order. */
-__attribute__((target_clones("arch=skylake-avx512", "arch=haswell", "default")))
-static void csr_row(int N, const long long *rowptr, const int *colind, const float *vals,
-                    const float *B, long long ldb, float *crow, int i) {
-    memset(crow, 0, sizeof(float) * (size_t)N);
-    for (long long q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-        const float *b = B + (long long)colind[q] * ldb;
-        float av = vals[q];
-        for (int j = 0; j < N; ++j) crow[j] = fmaf(av, b[j], crow[j]);
-    }
-}
-
 SRT_ORACLE_API
 int srt_oracle_csr(int M, int N, const long long *rowptr, const int *colind, const float *vals,
                    const float *B, long long ldb, float *C, long long ldc, int nthreads) {
     if (nthreads <= 0) nthreads = omp_get_max_threads();
+    int lvl = cpu_level();
+    csr_row_fn fn = lvl == 2 ? csr_row_avx512 : lvl == 1 ? csr_row_avx2 : csr_row_base;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
-    for (int i = 0; i < M; ++i) csr_row(N, rowptr, colind, vals, B, ldb, C + (long long)i * ldc, i);
+    for (int i = 0; i < M; ++i) fn(N, rowptr, colind, vals, B, ldb, C + (long long)i * ldc, i);
     return 0;
 }
 

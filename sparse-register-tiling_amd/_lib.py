@@ -40,6 +40,7 @@ class TileConfigC(ct.Structure):
         ("max_tlb_entries", ct.c_int), ("tlb_page_size", ct.c_int),
         ("runtime_schedule", ct.c_int), ("vec_width", ct.c_int),
         ("waves_per_block", ct.c_int), ("xcd_remap", ct.c_int), ("unroll", ct.c_int),
+        ("nsplit", ct.c_int), ("chunk_cols", ct.c_int),
     ]
 
 
@@ -79,6 +80,8 @@ SIGNATURES = {
     "srt_plan_stats_get": (_i, [_vp, ct.POINTER(PlanStatsC)]),
     "srt_plan_copy_packed": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "srt_plan_destroy": (None, [_vp]),
+    "srt_plan_split_counts": (_i, [_vp, _vp]),
+    "srt_tile_split_counts": (_i, [_vp, _vp]),
     "srt_sort_and_encode_patterns": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "srt_tile_create": (_i, [_vp, _i64, _i64, _i64, ct.POINTER(_vp)]),
     "srt_tile_pack_panel": (_i, [_vp, _i64, _i64, _vp, _vp, _i, _vp, _i, _vp]),

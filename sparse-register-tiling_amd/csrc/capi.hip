@@ -61,16 +61,17 @@ void Packed::release() {
     hipFree(vals);
     hipFree(groups);
     hipFree(grp_ptr);
+    hipFree(id_pattern);
     counts = nullptr;
     col_ptr = val_ptr = grp_ptr = nullptr;
-    cols = groups = nullptr;
+    cols = groups = id_pattern = nullptr;
     vals = nullptr;
 }
 
 size_t Packed::bytes() const {
     return sizeof(int32_t) * (size_t)P * NK + 3 * sizeof(int64_t) * (size_t)(P + 1) +
            sizeof(int32_t) * (size_t)total_cols + sizeof(float) * (size_t)total_vals +
-           2 * sizeof(int32_t) * (size_t)total_groups;
+           2 * sizeof(int32_t) * (size_t)total_groups + sizeof(int32_t) * (size_t)NK;
 }
 
 // ------------------------------------------------------------------ mapping
@@ -229,8 +230,10 @@ struct srt_mapping {
 // ------------------------------------------------------------------- plans
 struct srt_plan {
     Packed pk;
+    Schedule sch;
     const ExecutorDesc *exec = nullptr;
-    srt_tile_config cfg{};
+    srt_tile_config cfg_user{};  // as given (0 / -1 fields = choose)
+    srt_tile_config cfg{};       // realised
     int n_max = 0;
     int n_alloc = 0;
     int device = 0;
@@ -253,6 +256,11 @@ struct srt_tile {
     const ExecutorDesc *exec = nullptr;
     int64_t padding_total = 0;
     int device = 0;
+    // Schedule of the last executed configuration (rebuilt when the realised
+    // nsplit / chunk_cols / waves_per_block change).
+    mutable std::mutex mu;
+    mutable Schedule sch;
+    mutable srt_tile_config sch_cfg{};
 };
 
 extern "C" {
@@ -277,6 +285,7 @@ void srt_tile_config_default(srt_tile_config *c) {
     c->waves_per_block = 0;
     c->xcd_remap = -1;
     c->unroll = 0;
+    c->nsplit = 0;
 }
 
 // ----------------------------------------------------------------- mapping
@@ -365,21 +374,38 @@ static int log2_small(int v) { return v >= 8 ? 3 : v >= 4 ? 2 : v >= 2 ? 1 : 0; 
 // Realise the GPU fields of a tile config for a matrix with P panels and B
 // width n (the choices the reference's heuristic JSON made per sparsity
 // bucket are made here from the launch geometry).
-static void realise_config(srt_tile_config &c, int M_r, int K, int P, int n) {
-    if (c.vec_width != 1 && c.vec_width != 2 && c.vec_width != 4) {
-        // Widest V that still gives >= 2048 wavefronts (2 per SIMD), else 1.
-        c.vec_width = 1;
-        for (int v : {4, 2}) {
-            long long waves = (long long)P * ((n + 64 * v - 1) / (64 * v));
-            if (n % v == 0 && waves >= 2048) {
-                c.vec_width = v;
-                break;
-            }
-        }
+static bool pow2_upto8(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
+
+// Realise the GPU fields of a tile config for a matrix with P panels, on
+// average `avg_cols` packed columns per panel, and B width n (the choices the
+// reference's heuristic JSON made per sparsity bucket are made here from the
+// launch geometry; the autotuner overrides them with measured choices).
+static constexpr int kMaxWaves = 16;   // wave slots per workgroup (launch bound 1024)
+static constexpr size_t kMaxLds = 65536;
+
+static size_t reduce_lds(int waves, int M_r, int V) { return (size_t)waves * M_r * V * 64 * sizeof(float); }
+
+static void realise_config(srt_tile_config &c, int M_r, int K, int64_t total_cols, int n) {
+    if (c.vec_width != 1 && c.vec_width != 2 && c.vec_width != 4)
+        c.vec_width = (n % 4 == 0 && n >= 256) ? 4 : (n % 2 == 0 && n >= 128) ? 2 : 1;
+    if (c.waves_per_block < 1 || c.waves_per_block > kMaxWaves) c.waves_per_block = 8;
+    if (c.nsplit < 1 || c.nsplit > c.waves_per_block) c.nsplit = c.waves_per_block;
+    // The chunk reduction keeps one partial tile per wave slot in LDS.
+    while (c.nsplit > 1 && c.vec_width > 1 && reduce_lds(c.waves_per_block, M_r, c.vec_width) > kMaxLds)
+        c.vec_width >>= 1;
+    while (c.nsplit > 1 && reduce_lds(c.waves_per_block, M_r, c.vec_width) > kMaxLds) {
+        c.waves_per_block >>= 1;
+        c.nsplit = std::min(c.nsplit, c.waves_per_block);
     }
-    if (c.waves_per_block < 1 || c.waves_per_block > 8) c.waves_per_block = 4;
+    if (c.chunk_cols < 1) {
+        // About four waves per SIMD over the whole launch: long enough chains
+        // to amortise the chunk start, short enough to hide each other's latency.
+        const int64_t n_tiles = (n + 64 * c.vec_width - 1) / (64 * c.vec_width);
+        const int64_t target_waves = 4096;
+        c.chunk_cols = (int)std::max<int64_t>(8, (total_cols * n_tiles + target_waves - 1) / target_waves);
+    }
     if (c.xcd_remap != 0 && c.xcd_remap != 1) c.xcd_remap = 1;
-    if (c.unroll != 1 && c.unroll != 2 && c.unroll != 4 && c.unroll != 8) c.unroll = 4;
+    if (!pow2_upto8(c.unroll)) c.unroll = 4;
     c.M_c = M_r * c.waves_per_block;
     c.N_c = 64 * c.vec_width;
     c.K_c = K;
@@ -387,17 +413,22 @@ static void realise_config(srt_tile_config &c, int M_r, int K, int P, int n) {
 
 static bool aligned(const void *p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
 
-static int run_executor(const Packed &pk, const ExecutorDesc *exec, const srt_tile_config &cfg,
-                        const float *B, int64_t ldb, float *C, int64_t ldc, int n, int load_c,
-                        hipStream_t s) {
+static int run_executor(const Packed &pk, const Schedule &sch, const ExecutorDesc *exec,
+                        const srt_tile_config &cfg, const float *B, int64_t ldb, float *C,
+                        int64_t ldc, int n, int load_c, hipStream_t s) {
     if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
     if (n == 0 || pk.M == 0) return SRT_OK;
     if (!B || !C) return fail(SRT_ERR_INVALID, "null B or C");
     if (ldb < n || ldc < n) return fail(SRT_ERR_INVALID, "ldb and ldc must be >= n");
+    if (!sch.desc || sch.waves_per_block < 1 || sch.waves_per_block > kMaxWaves)
+        return fail(SRT_ERR_INVALID, "no launch schedule");
     int V = cfg.vec_width;
     // Vector loads need V | n, V | ldb, V | ldc and 4V-byte aligned B and C.
     while (V > 1 && (n % V || ldb % V || ldc % V || !aligned(B, 4 * V) || !aligned(C, 4 * V)))
         V >>= 1;
+    while (sch.reduce && V > 1 && reduce_lds(sch.waves_per_block, pk.M_r, V) > kMaxLds) V >>= 1;
+    if (sch.reduce && reduce_lds(sch.waves_per_block, pk.M_r, V) > kMaxLds)
+        return fail(SRT_ERR_UNSUPPORTED, "chunk reduction needs more than 64 KiB of LDS");
     int ui = log2_small(cfg.unroll);
     const int vi = log2_small(V);
     ExecLaunchFn fn = nullptr;
@@ -411,6 +442,9 @@ static int run_executor(const Packed &pk, const ExecutorDesc *exec, const srt_ti
     ExecLaunch a;
     a.groups = pk.groups;
     a.grp_ptr = pk.grp_ptr;
+    a.desc = sch.desc;
+    a.n_wblocks = sch.n_wblocks;
+    a.reduce = sch.reduce;
     a.col_ptr = pk.col_ptr;
     a.val_ptr = pk.val_ptr;
     a.cols = pk.cols;
@@ -425,7 +459,7 @@ static int run_executor(const Packed &pk, const ExecutorDesc *exec, const srt_ti
     a.num_panels = pk.P;
     a.load_c = load_c;
     a.vec_width = V;
-    a.waves_per_block = cfg.waves_per_block;
+    a.waves_per_block = sch.waves_per_block;
     a.xcd_remap = cfg.xcd_remap;
     a.unroll = cfg.unroll;
     SRT_HIP_TRY(fn(a, s));
@@ -461,8 +495,9 @@ struct Scratch {
         if (_rc) return _rc;     \
     } while (0)
 
-// Per-panel lists of non-empty groups (id, count) from pk.counts.
-static int build_groups(Packed &pk, hipStream_t s) {
+// Per-panel lists of non-empty groups (id, count) from pk.counts, the id ->
+// pattern table, and a host copy of the column offsets.
+static int build_groups(Packed &pk, const int32_t *d_id_pattern, hipStream_t s) {
     Scratch sc;
     int64_t *ng = nullptr;
     SRT_TRY(sc.get(&ng, pk.P));
@@ -475,7 +510,86 @@ static int build_groups(Packed &pk, hipStream_t s) {
     pk.total_groups = tot;
     SRT_TRY(dmalloc(&pk.groups, 2 * (size_t)tot));
     SRT_HIP_TRY(launch_group_fill(pk.P, pk.NK, pk.counts, pk.grp_ptr, pk.groups, s));
+    SRT_TRY(dmalloc(&pk.id_pattern, (size_t)std::max(pk.NK, 1)));
+    if (pk.NK)
+        SRT_HIP_TRY(hipMemcpyAsync(pk.id_pattern, d_id_pattern, sizeof(int32_t) * pk.NK,
+                                   hipMemcpyDeviceToDevice, s));
+    pk.h_col_ptr.assign((size_t)pk.P + 1, 0);
+    SRT_HIP_TRY(hipMemcpyAsync(pk.h_col_ptr.data(), pk.col_ptr, sizeof(int64_t) * (pk.P + 1),
+                               hipMemcpyDeviceToHost, s));
     SRT_HIP_TRY(hipStreamSynchronize(s));
+    return SRT_OK;
+}
+
+void Schedule::release() {
+    hipFree(desc);
+    desc = nullptr;
+    n_wblocks = 0;
+    kp.clear();
+}
+
+// Load-balanced launch schedule for a realised config: panel p gets
+// kp = min(nsplit, max(1, ceil(ncols_p / chunk_cols))) waves, panels are laid
+// into workgroups of waves_per_block slots in panel order (a panel never
+// straddles two workgroups), and the wave descriptors are computed on the
+// device from the group lists.
+static int build_schedule(const Packed &pk, const srt_tile_config &cfg, Schedule &out,
+                          hipStream_t s) {
+    Schedule sch;
+    sch.waves_per_block = cfg.waves_per_block;
+    sch.nsplit = std::min(cfg.nsplit, cfg.waves_per_block);
+    sch.chunk_cols = std::max(1, cfg.chunk_cols);
+    sch.kp.assign(pk.P, 1);
+    std::vector<int32_t> slots;  // (p, j, kp) per slot
+    int used = 0;
+    auto close_block = [&]() {
+        for (; used > 0 && used < sch.waves_per_block; ++used) {
+            slots.push_back(-1);
+            slots.push_back(0);
+            slots.push_back(1);
+        }
+        used = 0;
+    };
+    for (int p = 0; p < pk.P; ++p) {
+        const int64_t nc = pk.h_col_ptr[p + 1] - pk.h_col_ptr[p];
+        const int64_t want = (nc + sch.chunk_cols - 1) / sch.chunk_cols;
+        const int kp = (int)std::max<int64_t>(1, std::min<int64_t>(sch.nsplit, want));
+        sch.kp[p] = kp;
+        if (kp > 1) sch.reduce = 1;
+        if (used + kp > sch.waves_per_block) close_block();
+        for (int j = 0; j < kp; ++j) {
+            slots.push_back(p);
+            slots.push_back(j);
+            slots.push_back(kp);
+        }
+        used += kp;
+    }
+    close_block();
+    const int nslots = (int)(slots.size() / 3);
+    sch.n_wblocks = nslots / std::max(1, sch.waves_per_block);
+    Scratch sc;
+    int32_t *d_slots = nullptr;
+    SRT_TRY(sc.get(&d_slots, slots.size()));
+    SRT_TRY(dmalloc(&sch.desc, 8 * (size_t)std::max(nslots, 1)));
+    if (nslots) {
+        SRT_HIP_TRY(hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * slots.size(),
+                                   hipMemcpyHostToDevice, s));
+        SRT_HIP_TRY(launch_wave_desc(nslots, d_slots, pk.groups, pk.grp_ptr, pk.col_ptr,
+                                     pk.id_pattern, sch.desc, s));
+    }
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    out.release();
+    out = std::move(sch);
+    return SRT_OK;
+}
+
+// Column / value arrays of pk.total_cols / pk.total_vals entries plus the
+// zeroed tail padding the executor's batch reads rely on.
+static int alloc_packed_arrays(Packed &pk, hipStream_t s) {
+    SRT_TRY(dmalloc(&pk.cols, (size_t)pk.total_cols + SRT_COL_PAD));
+    SRT_TRY(dmalloc(&pk.vals, (size_t)pk.total_vals + SRT_VAL_PAD));
+    SRT_HIP_TRY(hipMemsetAsync(pk.cols + pk.total_cols, 0, sizeof(int32_t) * SRT_COL_PAD, s));
+    SRT_HIP_TRY(hipMemsetAsync(pk.vals + pk.total_vals, 0, sizeof(float) * SRT_VAL_PAD, s));
     return SRT_OK;
 }
 
@@ -501,6 +615,7 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, const float *Lx,
         if (rp[0] < 0) return fail(SRT_ERR_INVALID, "CSR row pointer starts below zero");
         for (int i = 0; i < m; ++i)
             if (rp[i + 1] < rp[i]) return fail(SRT_ERR_INVALID, "CSR row pointer decreases at row " + std::to_string(i));
+        if (rp[m] > rp[0] && (!Lx || !Li)) return fail(SRT_ERR_INVALID, "null CSR value or index array");
     }
     const int M_r = map.M_r;
     const int NK = (int)map.patterns.size();
@@ -555,8 +670,7 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, const float *Lx,
     pk.nnz = lpm - lp0;
     pk.total_cols = tot_c;
     pk.total_vals = tot_v;
-    SRT_TRY(dmalloc(&pk.cols, (size_t)tot_c));
-    SRT_TRY(dmalloc(&pk.vals, (size_t)tot_v));
+    SRT_TRY(alloc_packed_arrays(pk, s));
     SRT_HIP_TRY(launch_fill_codes(P, k, 0, *lut, codes, dense, pk.col_ptr, pk.val_ptr, pk.counts,
                                   pk.cols, pk.vals, pad, s));
     std::vector<int64_t> h_pad(P);
@@ -564,7 +678,7 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, const float *Lx,
     SRT_HIP_TRY(hipStreamSynchronize(s));
     pk.padding = 0;
     for (int64_t v : h_pad) pk.padding += v;
-    return build_groups(pk, s);
+    return build_groups(pk, lut->id_pattern, s);
 }
 
 }  // namespace srt
@@ -580,7 +694,7 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
     *out = nullptr;
     if (m < 0 || k < 0 || n < 0) return fail(SRT_ERR_INVALID, "negative dimension");
     if (index_bytes != 4 && index_bytes != 8) return fail(SRT_ERR_INVALID, "index_bytes must be 4 or 8");
-    if (!Lp || ((!Lx || !Li) && m > 0)) return fail(SRT_ERR_INVALID, "null CSR array");
+    if (!Lp) return fail(SRT_ERR_INVALID, "null CSR row pointer");
     MappingImpl &map = *mapping->impl;
     const ExecutorDesc *exec = executor_id && executor_id[0] ? find_executor_by_id(executor_id)
                                                              : find_executor(map.M_r, map.patterns);
@@ -599,12 +713,22 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
         return rc;
     }
     plan->exec = exec;
-    if (cfg) plan->cfg = *cfg;
-    else srt_tile_config_default(&plan->cfg);
+    if (cfg) plan->cfg_user = *cfg;
+    else srt_tile_config_default(&plan->cfg_user);
     plan->n_max = n;
-    realise_config(plan->cfg, map.M_r, k, plan->pk.P, std::max(n, 1));
+    plan->cfg = plan->cfg_user;
+    realise_config(plan->cfg, map.M_r, k, plan->pk.total_cols, std::max(n, 1));
+    rc = build_schedule(plan->pk, plan->cfg, plan->sch, (hipStream_t)stream);
+    if (rc) {
+        plan->pk.release();
+        return rc;
+    }
     *out = plan.release();
     return SRT_OK;
+}
+
+static bool same_schedule(const srt_tile_config &a, const srt_tile_config &b) {
+    return a.nsplit == b.nsplit && a.chunk_cols == b.chunk_cols && a.waves_per_block == b.waves_per_block;
 }
 
 int srt_plan_get_config(const srt_plan *p, srt_tile_config *out) {
@@ -624,6 +748,12 @@ int srt_plan_allocate_executor(srt_plan *p, int n) {
     if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
     p->n_alloc = n;
     if (n > p->n_max) p->n_max = n;
+    // Re-realise the choices that depend on the B width and rebuild the
+    // schedule if they changed.  Not to be called while the plan executes.
+    srt_tile_config c = p->cfg_user;
+    realise_config(c, p->pk.M_r, p->pk.K, p->pk.total_cols, std::max(n, 1));
+    if (!same_schedule(c, p->cfg)) SRT_TRY(build_schedule(p->pk, c, p->sch, nullptr));
+    p->cfg = c;
     return SRT_OK;
 }
 
@@ -633,7 +763,13 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
     int dev = -1;
     SRT_HIP_TRY(hipGetDevice(&dev));
     if (dev != p->device) return fail(SRT_ERR_INVALID, "plan belongs to another device");
-    return run_executor(p->pk, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream);
+    return run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream);
+}
+
+int srt_plan_split_counts(const srt_plan *p, int32_t *kp) {
+    if (!p || !kp) return fail(SRT_ERR_INVALID, "null argument");
+    std::copy(p->sch.kp.begin(), p->sch.kp.end(), kp);
+    return SRT_OK;
 }
 
 int srt_plan_stats_get(const srt_plan *p, srt_plan_stats *o) {
@@ -672,6 +808,7 @@ int srt_plan_copy_packed(const srt_plan *p, int32_t *counts, int64_t *col_ptr, i
 void srt_plan_destroy(srt_plan *p) {
     if (!p) return;
     p->pk.release();
+    p->sch.release();
     delete p;
 }
 
@@ -824,8 +961,7 @@ int srt_tile_pack_tile(srt_tile *t, srt_stream_t stream) {
     SRT_TRY(dmalloc(&pk.counts, (size_t)P * NK));
     SRT_TRY(dmalloc(&pk.col_ptr, (size_t)P + 1));
     SRT_TRY(dmalloc(&pk.val_ptr, (size_t)P + 1));
-    SRT_TRY(dmalloc(&pk.cols, (size_t)pk.total_cols));
-    SRT_TRY(dmalloc(&pk.vals, (size_t)pk.total_vals));
+    SRT_TRY(alloc_packed_arrays(pk, s));
     SRT_HIP_TRY(hipMemcpyAsync(pk.col_ptr, cp.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
     SRT_HIP_TRY(hipMemcpyAsync(pk.val_ptr, vp.data(), sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice, s));
     for (int p = 0; p < P; ++p) {
@@ -841,7 +977,9 @@ int srt_tile_pack_tile(srt_tile *t, srt_stream_t stream) {
                                        hipMemcpyDeviceToDevice, s));
     }
     SRT_HIP_TRY(hipStreamSynchronize(s));
-    SRT_TRY(build_groups(pk, s));
+    DeviceLut *lut = nullptr;
+    SRT_TRY(t->map->device_lut(&lut));
+    SRT_TRY(build_groups(pk, lut->id_pattern, s));
     for (PanelBuf &pb : t->panels) {
         hipFree(pb.counts);
         hipFree(pb.cols);
@@ -867,8 +1005,22 @@ int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, fl
     srt_tile_config c;
     if (cfg) c = *cfg;
     else srt_tile_config_default(&c);
-    realise_config(c, t->pk.M_r, t->pk.K, t->pk.P, std::max(N_c, 1));
-    return run_executor(t->pk, t->exec, c, B, ldb, C, ldc, N_c, load_c ? 1 : 0, (hipStream_t)stream);
+    realise_config(c, t->pk.M_r, t->pk.K, t->pk.total_cols, std::max(N_c, 1));
+    std::lock_guard<std::mutex> g(t->mu);
+    if (!t->sch.desc || !same_schedule(c, t->sch_cfg)) {
+        SRT_TRY(build_schedule(t->pk, c, t->sch, (hipStream_t)stream));
+        t->sch_cfg = c;
+    }
+    return run_executor(t->pk, t->sch, t->exec, c, B, ldb, C, ldc, N_c, load_c ? 1 : 0,
+                        (hipStream_t)stream);
+}
+
+int srt_tile_split_counts(const srt_tile *t, int32_t *kp) {
+    if (!t || !kp) return fail(SRT_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(t->mu);
+    if (t->sch.kp.empty()) std::fill(kp, kp + t->num_panels, 1);
+    else std::copy(t->sch.kp.begin(), t->sch.kp.end(), kp);
+    return SRT_OK;
 }
 
 int srt_tile_copy_packed(const srt_tile *t, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
@@ -888,6 +1040,7 @@ void srt_tile_destroy(srt_tile *t) {
         hipFree(pb.vals);
     }
     t->pk.release();
+    t->sch.release();
     delete t;
 }
 
@@ -899,7 +1052,7 @@ int srt_csr_spmm(int m, int k, int n, int64_t nnz, const void *rowptr, const voi
     if (index_bytes == 4 && nnz > 0x7fffffffLL) return fail(SRT_ERR_INVALID, "nnz exceeds int32 indices");
     if (index_bytes != 4 && index_bytes != 8) return fail(SRT_ERR_INVALID, "index_bytes must be 4 or 8");
     if (m == 0 || n == 0) return SRT_OK;
-    if (!rowptr || !colind || !vals || !B || !C) return fail(SRT_ERR_INVALID, "null array");
+    if (!rowptr || !B || !C || (nnz > 0 && (!colind || !vals))) return fail(SRT_ERR_INVALID, "null array");
     if (ldb < n || ldc < n) return fail(SRT_ERR_INVALID, "ldb and ldc must be >= n");
     if (n_tile != 16 && n_tile != 32 && n_tile != 64 && n_tile != 128 && n_tile != 256)
         return fail(SRT_ERR_INVALID, "n_tile must be one of 16, 32, 64, 128, 256");

@@ -87,6 +87,37 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Reads N consecutive wave-uniform words as wide vector values.  Element-wise
+// loads would be sunk by LLVM into the per-column branches that use them, so
+// each column would wait for its own scalar load; a vector load used by
+// several branches stays in front of them (one s_load_dwordx{2..16}, one wait).
+template <typename T, int N>
+__device__ __forceinline__ void load_uniform(const T *__restrict__ p, T (&out)[N]) {
+    int i = 0;
+#pragma unroll
+    for (; i + 8 <= N; i += 8) {
+        T __attribute__((ext_vector_type(8))) v;
+        __builtin_memcpy(&v, p + i, sizeof(v));
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[i + j] = v[j];
+    }
+    if constexpr (N % 8 >= 4) {
+        T __attribute__((ext_vector_type(4))) v;
+        __builtin_memcpy(&v, p + i, sizeof(v));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[i + j] = v[j];
+        i += 4;
+    }
+    if constexpr (N % 4 >= 2) {
+        T __attribute__((ext_vector_type(2))) v;
+        __builtin_memcpy(&v, p + i, sizeof(v));
+        out[i] = v[0];
+        out[i + 1] = v[1];
+        i += 2;
+    }
+    if constexpr (N % 2 == 1) out[i] = p[i];
+}
+
 // acc[r][:] += av[k] * b for the k-th set bit r of PAT.
 template <int PAT, int MR, int V>
 __device__ __forceinline__ void fma_pattern(float (&acc)[MR][V], const float *__restrict__ av,
@@ -108,97 +139,182 @@ __device__ __forceinline__ void fma_pattern(float (&acc)[MR][V], const float *__
 // columns (cp[0 ..)).  On exit cp/vp are advanced past the group and
 // cur[0 .. min(U, next_count)) holds the rows of the next group's first
 // columns.
+//
+// Scalar metadata is read in whole batches, outside any branch, so the
+// compiler issues wide s_load_dwordx{4,8,16} and waits once per batch: the
+// column indices of the next U columns, then the values of the current
+// columns UG at a time (UG * popcount <= 32 SGPRs).  These batch reads may
+// run past the panel's last column; the packed arrays carry SRT_COL_PAD /
+// SRT_VAL_PAD zero entries at their end so they stay in bounds, and entries
+// past the group are never used.
+//
+// One batch: the FMAs of up to U columns held in `src` while the loads of the
+// following batch go into `dst`.  Returns true when the group is finished
+// (then `dst` holds the next group's first columns).
+template <int PAT, int MR, int V, int U>
+__device__ __forceinline__ bool group_batch(float (&acc)[MR][V], typename VecOf<V>::type (&src)[U],
+                                            typename VecOf<V>::type (&dst)[U], int &left,
+                                            int next_count, const int32_t *__restrict__ &cp,
+                                            const float *__restrict__ &vp, const BRows<V> &bb) {
+    constexpr int PC = __builtin_popcount(PAT);
+    constexpr int UG = PC * U <= 32 ? U : (32 / PC > 0 ? 32 / PC : 1);
+    const int vc = left < U ? left : U;
+    left -= vc;
+    const int want = left > 0 ? left : next_count;
+    const int vn = want < U ? want : U;
+    int col[U];
+    load_uniform(cp + vc, col);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (u < vn) dst[u] = bb.load(col[u]);
+#pragma unroll
+    for (int s = 0; s < U; s += UG) {
+        if (s < vc) {
+            float av[UG * PC];
+            load_uniform(vp + s * PC, av);
+#pragma unroll
+            for (int u = 0; u < UG; ++u)
+                if (s + u < U && s + u < vc) fma_pattern<PAT, MR, V>(acc, av + u * PC, src[s + u]);
+        }
+    }
+    cp += vc;
+    vp += vc * PC;
+    return left <= 0;
+}
+
+// Batches alternate between two register buffers, so no batch copies its
+// prefetched rows; a group that ends on the second buffer copies it once.
 template <int PAT, int MR, int V, int U>
 __device__ __forceinline__ void run_group(float (&acc)[MR][V], typename VecOf<V>::type (&cur)[U],
                                           int count, int next_count,
                                           const int32_t *__restrict__ &cp,
                                           const float *__restrict__ &vp, const BRows<V> &bb) {
-    constexpr int PC = __builtin_popcount(PAT);
     typedef typename VecOf<V>::type vec;
+    vec alt[U];
     int left = count;
     while (true) {
-        const int vc = left < U ? left : U;
-        left -= vc;
-        const int want = left > 0 ? left : next_count;
-        const int vn = want < U ? want : U;
-        vec nxt[U];
+        if (group_batch<PAT, MR, V, U>(acc, cur, alt, left, next_count, cp, vp, bb)) {
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u < vn) nxt[u] = bb.load(cp[vc + u]);
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u < vc) fma_pattern<PAT, MR, V>(acc, vp + u * PC, cur[u]);
-        cp += vc;
-        vp += vc * PC;
-#pragma unroll
-        for (int u = 0; u < U; ++u) cur[u] = nxt[u];
-        if (left <= 0) break;
+            for (int u = 0; u < U; ++u) cur[u] = alt[u];
+            return;
+        }
+        if (group_batch<PAT, MR, V, U>(acc, alt, cur, left, next_count, cp, vp, bb)) return;
     }
 }
 
 // D::run<V, U>(id, acc, cur, count, next_count, cp, vp, bb) is the generated
 // switch over the pattern set's ids.
+//
+// Work decomposition (load balanced, like the reference's KD_*LoadBalanced
+// kernel descriptors): panel p is cut into k_p contiguous chunks of its packed
+// columns, chunk j = [floor(j*n_p/k_p), floor((j+1)*n_p/k_p)), one wavefront
+// each; the chunks of a panel sit in consecutive wave slots of one workgroup
+// and light panels share workgroups.  Each wave reads its descriptor
+//   d0 = {panel, group of chunk start, skip in that group, first column}
+//   d1 = {first value, end column, k_p, j}
+// (panel -1 = idle slot).  Waves j > 0 park their partial accumulators in LDS
+// and wave j = 0 adds them in chunk order:
+//   C = ((chain_0 + chain_1) + chain_2) + ...
+// chain_0 starting from C (load_c) or 0, the others from 0.  With every
+// k_p = 1 this is exactly the reference's single chain per panel.
 template <int MR, int V, int U, class D>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
     exec_kernel(const int2 *__restrict__ groups, const int64_t *__restrict__ grp_ptr,
-                const int64_t *__restrict__ col_ptr, const int64_t *__restrict__ val_ptr,
-                const int32_t *__restrict__ cols, const float *__restrict__ vals,
-                const float *__restrict__ B, uint32_t b_bytes, uint32_t ldb,
-                float *__restrict__ C, int64_t ldc, int M, int N, int num_panels, int p_groups,
-                int load_c, int xcd_remap) {
+                const int4 *__restrict__ desc, const int64_t *__restrict__ col_ptr,
+                const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
+                const float *__restrict__ vals, const float *__restrict__ B, uint32_t b_bytes,
+                uint32_t ldb, float *__restrict__ C, int64_t ldc, int M, int N, int n_wblocks,
+                int reduce, int load_c, int xcd_remap) {
     typedef typename VecOf<V>::type vec;
+    extern __shared__ float red[];  // [waves][MR][V][64]
     const int waves = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     int b = blockIdx.x;
     if (xcd_remap) b = xcd_swizzle(b, gridDim.x);
-    const int ntile = b / p_groups;
-    const int pg = b - ntile * p_groups;
-    const int p = pg * waves + wave;
-    if (p >= num_panels) return;
+    const int ntile = b / n_wblocks;
+    const int sb = b - ntile * n_wblocks;
+    const int4 *__restrict__ dw = desc + ((int64_t)sb * waves + wave) * 2;
+    const int4 d0 = dw[0];
+    const int4 d1 = dw[1];
+    const int p = d0.x;
+    const bool live = p >= 0;  // wave-uniform
+    if (!reduce && !live) return;  // no barrier below when nothing is reduced
     const int n = ntile * (64 * V) + lane * V;
     const bool active = n < N;
     const int nl = active ? n : 0;  // inactive lanes read columns 0..V-1 (in bounds)
-
-    BRows<V> bb;
-    bb.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(B), (short)0, (int)b_bytes,
-                                                0x00020000);
-    bb.voff = (uint32_t)nl * 4u;
-    bb.row_bytes = ldb * 4u;
+    const int kp = d1.z, j = d1.w;
 
     float acc[MR][V];
 #pragma unroll
     for (int r = 0; r < MR; ++r)
 #pragma unroll
-        for (int j = 0; j < V; ++j) acc[r][j] = 0.0f;
-    if (load_c) {
+        for (int jj = 0; jj < V; ++jj) acc[r][jj] = 0.0f;
+
+    if (live) {
+        BRows<V> bb;
+        bb.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(B), (short)0, (int)b_bytes,
+                                                    0x00020000);
+        bb.voff = (uint32_t)nl * 4u;
+        bb.row_bytes = ldb * 4u;
+        if (load_c && j == 0) {
 #pragma unroll
-        for (int r = 0; r < MR; ++r) {
-            const int row = p * MR + r;
-            if (row < M && active) {
-                const vec c = load_vec<V>(C + (int64_t)row * ldc + nl);
+            for (int r = 0; r < MR; ++r) {
+                const int row = p * MR + r;
+                if (row < M && active) {
+                    const vec c = load_vec<V>(C + (int64_t)row * ldc + nl);
 #pragma unroll
-                for (int j = 0; j < V; ++j) acc[r][j] = lane_elem<V>(c, j);
+                    for (int jj = 0; jj < V; ++jj) acc[r][jj] = lane_elem<V>(c, jj);
+                }
+            }
+        }
+        int remaining = d1.y - d0.w;
+        if (remaining > 0) {
+            const int2 *__restrict__ gl = groups + grp_ptr[p] + d0.y;
+            const int32_t *__restrict__ cp = cols + col_ptr[p] + d0.w;
+            const float *__restrict__ vp = vals + val_ptr[p] + d1.x;
+            int2 e = gl[0];
+            int cnt = e.y - d0.z < remaining ? e.y - d0.z : remaining;
+            vec cur[U];
+            {
+                const int c0 = cnt < U ? cnt : U;
+                int col[U];
+                load_uniform(cp, col);  // padded: in bounds
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (u < c0) cur[u] = bb.load(col[u]);
+            }
+            while (true) {
+                remaining -= cnt;
+                int2 en = make_int2(0, 0);
+                if (remaining > 0) en = gl[1];
+                const int ncnt = en.y < remaining ? en.y : remaining;
+                D::template run<V, U>(e.x, acc, cur, cnt, ncnt, cp, vp, bb);
+                if (remaining <= 0) break;
+                e = en;
+                cnt = ncnt;
+                ++gl;
             }
         }
     }
 
-    const int64_t g0 = grp_ptr[p];
-    const int ng = (int)(grp_ptr[p + 1] - g0);
-    const int2 *__restrict__ gl = groups + g0;
-    const int32_t *__restrict__ cp = cols + col_ptr[p];
-    const float *__restrict__ vp = vals + val_ptr[p];
-    vec cur[U];
-    if (ng > 0) {
-        const int c0 = gl[0].y < U ? gl[0].y : U;
+    if (reduce) {
+        if (live && j > 0) {
+            float *mine = red + (int64_t)wave * MR * V * 64;
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (u < c0) cur[u] = bb.load(cp[u]);
-    }
-    for (int g = 0; g < ng; ++g) {
-        const int2 e = gl[g];
-        const int next = g + 1 < ng ? gl[g + 1].y : 0;
-        D::template run<V, U>(e.x, acc, cur, e.y, next, cp, vp, bb);
+            for (int r = 0; r < MR; ++r)
+#pragma unroll
+                for (int jj = 0; jj < V; ++jj) mine[(r * V + jj) * 64 + lane] = acc[r][jj];
+        }
+        __syncthreads();
+        if (!live || j > 0) return;
+        for (int w = 1; w < kp; ++w) {
+            const float *part = red + (int64_t)(wave + w) * MR * V * 64;
+#pragma unroll
+            for (int r = 0; r < MR; ++r)
+#pragma unroll
+                for (int jj = 0; jj < V; ++jj) acc[r][jj] = acc[r][jj] + part[(r * V + jj) * 64 + lane];
+        }
     }
 
     if (active) {
@@ -211,7 +327,7 @@ __global__ void __launch_bounds__(512)
                     c = acc[r][0];
                 } else {
 #pragma unroll
-                    for (int j = 0; j < V; ++j) c[j] = acc[r][j];
+                    for (int jj = 0; jj < V; ++jj) c[jj] = acc[r][jj];
                 }
                 __builtin_nontemporal_store(c, reinterpret_cast<vec *>(C + (int64_t)row * ldc + n));
             }
@@ -219,18 +335,24 @@ __global__ void __launch_bounds__(512)
     }
 }
 
+// LDS bytes the chunk reduction needs: one partial tile per wave slot.
+inline size_t exec_lds_bytes(int MR, int V, int waves, int reduce) {
+    return reduce ? (size_t)waves * MR * V * 64 * sizeof(float) : 0;
+}
+
 template <int MR, int V, int U, class D>
 hipError_t launch_exec(const ExecLaunch &a, hipStream_t s) {
-    const int waves = a.waves_per_block;
-    const int p_groups = (a.num_panels + waves - 1) / waves;
     const int n_tiles = (a.N + 64 * V - 1) / (64 * V);
-    const long long nblocks = (long long)p_groups * n_tiles;
+    const long long nblocks = (long long)a.n_wblocks * n_tiles;
     if (nblocks == 0) return hipSuccess;
-    if (nblocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((exec_kernel<MR, V, U, D>), dim3((unsigned)nblocks), dim3(64 * waves), 0, s,
-                       reinterpret_cast<const int2 *>(a.groups), a.grp_ptr, a.col_ptr, a.val_ptr,
-                       a.cols, a.vals, a.B, a.b_bytes, (uint32_t)a.ldb, a.C, a.ldc, a.M, a.N,
-                       a.num_panels, p_groups, a.load_c, a.xcd_remap);
+    if (nblocks > 0x7fffffffLL || a.waves_per_block < 1 || a.waves_per_block > 16)
+        return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((exec_kernel<MR, V, U, D>), dim3((unsigned)nblocks), dim3(64 * a.waves_per_block),
+                       exec_lds_bytes(MR, V, a.waves_per_block, a.reduce), s,
+                       reinterpret_cast<const int2 *>(a.groups), a.grp_ptr,
+                       reinterpret_cast<const int4 *>(a.desc), a.col_ptr, a.val_ptr, a.cols, a.vals,
+                       a.B, a.b_bytes, (uint32_t)a.ldb, a.C, a.ldc, a.M, a.N, a.n_wblocks, a.reduce,
+                       a.load_c, a.xcd_remap);
     return hipGetLastError();
 }
 

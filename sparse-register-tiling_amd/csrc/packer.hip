@@ -261,6 +261,42 @@ __global__ void group_fill_kernel(int P, int NK, const int32_t *__restrict__ cou
     }
 }
 
+// One thread per wave slot: where chunk j of kp of panel p starts and ends.
+//   d0 = {p, group index, skip in group, first column}
+//   d1 = {first value, end column, kp, j}      (panel-relative offsets)
+// Chunk j covers packed columns [floor(j*n/kp), floor((j+1)*n/kp)).
+__global__ void wave_desc_kernel(int nslots, const int32_t *__restrict__ slots,
+                                 const int2 *__restrict__ groups,
+                                 const int64_t *__restrict__ grp_ptr,
+                                 const int64_t *__restrict__ col_ptr,
+                                 const int32_t *__restrict__ id_pattern, int4 *__restrict__ desc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    const int p = slots[3 * i], j = slots[3 * i + 1], kp = slots[3 * i + 2];
+    if (p < 0) {
+        desc[2 * i] = make_int4(-1, 0, 0, 0);
+        desc[2 * i + 1] = make_int4(0, 0, 1, 0);
+        return;
+    }
+    const int64_t ncols = col_ptr[p + 1] - col_ptr[p];
+    const int64_t c0 = (int64_t)j * ncols / kp;
+    const int64_t c1 = (int64_t)(j + 1) * ncols / kp;
+    const int2 *gl = groups + grp_ptr[p];
+    const int ng = (int)(grp_ptr[p + 1] - grp_ptr[p]);
+    int64_t cs = 0, vs = 0;
+    int g = 0;
+    for (; g < ng; ++g) {
+        const int2 e = gl[g];
+        if (cs + e.y > c0) break;
+        cs += e.y;
+        vs += (int64_t)e.y * __popc((unsigned)id_pattern[e.x]);
+    }
+    const int64_t skip = c0 - cs;
+    const int pc = g < ng ? __popc((unsigned)id_pattern[gl[g].x]) : 0;
+    desc[2 * i] = make_int4(p, g, (int)skip, (int)c0);
+    desc[2 * i + 1] = make_int4((int)(vs + skip * pc), (int)c1, kp, j);
+}
+
 // ------------------------------------------------------- entry-list variants
 __global__ void encode_entries_kernel(int n, int M_r, const int32_t *__restrict__ pattern_id,
                                       const int32_t *__restrict__ pat_codes,
@@ -467,6 +503,16 @@ hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t
     if (P <= 0) return hipSuccess;
     hipLaunchKernelGGL(group_fill_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, NK, counts,
                        grp_ptr, groups);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_desc(int nslots, const int32_t *slots, const int32_t *groups,
+                            const int64_t *grp_ptr, const int64_t *col_ptr,
+                            const int32_t *id_pattern, int32_t *desc, hipStream_t s) {
+    if (nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wave_desc_kernel, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, slots,
+                       reinterpret_cast<const int2 *>(groups), grp_ptr, col_ptr, id_pattern,
+                       reinterpret_cast<int4 *>(desc));
     return hipGetLastError();
 }
 

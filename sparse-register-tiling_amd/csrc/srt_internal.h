@@ -28,6 +28,9 @@ int fail(int code, const std::string &msg);
 struct ExecLaunch {
     const int32_t *groups;   // (id, count) pairs of non-empty groups, id order
     const int64_t *grp_ptr;  // [P + 1] range of panel p in groups (in pairs)
+    const int32_t *desc;     // [n_wblocks * waves_per_block] pairs of int4 wave descriptors
+    int n_wblocks;           // workgroups per N tile
+    int reduce;              // some panel runs on more than one wave
     const int64_t *col_ptr;  // [P + 1]
     const int64_t *val_ptr;  // [P + 1]
     const int32_t *cols;
@@ -72,6 +75,12 @@ struct DeviceLut {
     int device = -1;
 };
 
+// Zero entries appended to the packed column / value arrays: the executor
+// reads column indices and values in fixed-size batches that may run past the
+// last panel's data (at most 2 * 8 indices and 8 * 16 + 32 values).
+constexpr int SRT_COL_PAD = 32;
+constexpr int SRT_VAL_PAD = 256;
+
 // Packed matrix in device memory (flat arrays over all panels).
 struct Packed {
     int M = 0, K = 0, M_r = 0, P = 0, NK = 0;
@@ -82,9 +91,25 @@ struct Packed {
     float *vals = nullptr;
     int32_t *groups = nullptr;   // (id, count) pairs of non-empty ids per panel
     int64_t *grp_ptr = nullptr;  // [P + 1]
+    int32_t *id_pattern = nullptr;  // [NK] kernel pattern of each id
+    std::vector<int64_t> h_col_ptr;  // host copy of col_ptr (scheduling)
     int64_t total_cols = 0, total_vals = 0, total_groups = 0, padding = 0, nnz = 0;
     void release();
     size_t bytes() const;
+};
+
+// Load-balanced launch schedule over a Packed matrix: panel p runs on
+// kp[p] = min(nsplit, max(1, ceil(ncols_p / chunk_cols))) waves; panels are
+// laid into workgroups of waves_per_block wave slots in panel order.
+struct Schedule {
+    int32_t *desc = nullptr;  // device: two int4 per wave slot (see exec_kernel.h)
+    int n_wblocks = 0;
+    int waves_per_block = 0;
+    int nsplit = 0;
+    int chunk_cols = 0;
+    int reduce = 0;
+    std::vector<int32_t> kp;  // per panel
+    void release();
 };
 
 // Kernel launchers (packer.hip).
@@ -104,6 +129,10 @@ hipError_t launch_group_count(int P, int NK, const int32_t *counts, int64_t *ngr
                               hipStream_t s);
 hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t *grp_ptr,
                              int32_t *groups, hipStream_t s);
+// Wave descriptors from (panel, j, kp) triples (slots[3 * i]; panel -1 = idle).
+hipError_t launch_wave_desc(int nslots, const int32_t *slots, const int32_t *groups,
+                            const int64_t *grp_ptr, const int64_t *col_ptr,
+                            const int32_t *id_pattern, int32_t *desc, hipStream_t s);
 // Entry-list variants (SOPTile path): one panel, explicit (pattern, column) entries.
 // bad |= 1 for a pattern outside the set, |= 2 for a column outside [0, K)
 // (col_indices may be null to skip the column check).

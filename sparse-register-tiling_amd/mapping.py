@@ -218,8 +218,10 @@ STRATEGIES: Dict[str, Callable[[], SOPOrig]] = {
 STRATEGY_M_R = {"M2_identity": 2, "M4_identity": 4, "M6_identity": 6, "M8_identity": 8,
                 "M4_orig": 4, "M8_orig": 8, "M8_alt": 8}
 
-# tools/paper/configs.py:84-87
-NAMED_IDS = {4: {"identity": "61fee", "orig": "da01e"}, 8: {"orig": "400fa", "alt": "747f9"}}
+# tools/paper/configs.py:84-87, plus the identity mappings of the other
+# panel heights (sbench/SOP/identity_mappings/{2,6,8}).
+NAMED_IDS = {2: {"identity": "4af51"}, 4: {"identity": "61fee", "orig": "da01e"},
+             6: {"identity": "001e3"}, 8: {"orig": "400fa", "alt": "747f9", "identity": "161fb"}}
 
 
 def generate_shipped_mappings(directory: str = MAPPINGS_DIR) -> Dict[str, str]:

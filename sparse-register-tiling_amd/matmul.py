@@ -36,8 +36,10 @@ SCHEDULES = {"KNM": 0, "NKM": 1, "MNK": 2, "NMK": 3, "KMN": 4, "MKN": 5, "KM": 6
 @dataclass
 class TileConfig:
     """sop::TileConfig fields set at SpMM_SOP.h:107-114 (defaults of SOPConfig,
-    :33-54) plus the GPU realisation (vec_width, waves_per_block, xcd_remap,
-    unroll; 0 / -1 = let the library choose)."""
+    :33-54) plus the GPU realisation (0 / -1 = let the library choose):
+    vec_width (fp32 columns per lane), waves_per_block (panels per
+    workgroup), xcd_remap, unroll (packed columns in flight per wave) and
+    nsplit (waves sharing one panel; 1 keeps the reference's single chain)."""
     M_c: int = 16
     K_c: int = 256
     N_c: int = 64
@@ -51,6 +53,8 @@ class TileConfig:
     waves_per_block: int = 0
     xcd_remap: int = -1
     unroll: int = 0
+    nsplit: int = 0
+    chunk_cols: int = 0
 
     def to_c(self) -> _lib.TileConfigC:
         c = _lib.TileConfigC()
@@ -152,6 +156,13 @@ class MatMulSpecialized:
         _lib.call("srt_plan_stats_get", self._h, ct.byref(s))
         return {f[0]: getattr(s, f[0]) for f in s._fields_}
 
+    def split_counts(self) -> np.ndarray:
+        """Waves each panel is split over in the current schedule (the chunk
+        order C = chain_0 + chain_1 + ... is reproducible from it)."""
+        kp = np.zeros(max(1, self.stats()["num_panels"]), dtype=np.int32)
+        _lib.call("srt_plan_split_counts", self._h, kp.ctypes.data_as(ct.c_void_p))
+        return kp[:self.stats()["num_panels"]]
+
     def packed(self) -> Dict[str, np.ndarray]:
         """Packed arrays copied to the host: counts [P, NK] (nkern_counts),
         col_ptr / val_ptr [P+1], cols, vals."""
@@ -221,8 +232,8 @@ class SpMM_SOP:
                           tiling_strategy=c["tiling_strategy"], beta=c["beta_10x"] / 10.0,
                           sparse_a=c["sparse_a"], max_tlb_entries=c["max_tlb_entries"],
                           tlb_page_size=c["tlb_page_size"], runtimeSchedule=self.schedule,
-                          **{k: c[k] for k in ("vec_width", "waves_per_block", "xcd_remap", "unroll")
-                             if k in c})
+                          **{k: c[k] for k in ("vec_width", "waves_per_block", "xcd_remap", "unroll",
+                                               "nsplit") if k in c})
 
     def _build(self) -> None:
         A = self.task["A"]

@@ -113,6 +113,12 @@ class SOPTile:
         _lib.call("srt_tile_execute", self._h, int(N_c), ptr(B), B.stride(0), ptr(C), C.stride(0),
                   1 if load_c else 0, ct.byref(cfg), stream_handle(stream))
 
+    def split_counts(self) -> np.ndarray:
+        """Waves per panel of the last executed configuration."""
+        kp = np.zeros(max(1, self.num_panels), dtype=np.int32)
+        _lib.call("srt_tile_split_counts", self._h, kp.ctypes.data_as(ct.c_void_p))
+        return kp[:self.num_panels]
+
     def packed(self) -> Dict[str, np.ndarray]:
         """Packed arrays on the host: per-panel nkern_counts, col/val offsets,
         col_indices, values (the MicroKernelPackedData fields, template:169-212)."""
@@ -222,10 +228,12 @@ class SOPModule:
         return sop_tile
 
     def execute_tile(self, tile: SOPTile, B: torch.Tensor, num_runs: int = 1024, N_c: Optional[int] = None,
-                     measured_iterations: int = MEASURED_ITERATIONS) -> Tuple[float, torch.Tensor]:
+                     measured_iterations: int = MEASURED_ITERATIONS,
+                     tile_config: Optional[TileConfig] = None) -> Tuple[float, torch.Tensor]:
         N_c = B.shape[-1] if N_c is None else N_c
-        vec_hint = self.acc.N if self.acc.N in (1, 2, 4) else 0
-        return execute_tile(N_c, tile, B, num_runs, measured_iterations, TileConfig(vec_width=vec_hint))
+        if tile_config is None:
+            tile_config = TileConfig(vec_width=self.acc.N if self.acc.N in (1, 2, 4) else 0)
+        return execute_tile(N_c, tile, B, num_runs, measured_iterations, tile_config)
 
 
 def make_sop_module(acc: Acc, patterns: Iterable[int], pattern_mapping: Union[Callable, Mapping],

@@ -87,7 +87,7 @@ def test_cpu_config_full_size(srt, oracle, torch_cuda, M_r, name):
     st = plan.stats()
     assert st["padding"] == pk["padding"] and st["nnz"] == np.count_nonzero(A)
     C = _run(srt, torch, plan, B)
-    _bitexact(C, cb.exec_packed(pk, B))
+    _bitexact(C, cb.exec_packed(pk, B, nsplit=plan.split_counts()))
     C64 = cb.dense_f64(A, B)
     assert np.all(np.abs(C - C64) <= so.fp64_error_bound(A, B))
 
@@ -107,25 +107,50 @@ def test_pruned_ffn_cases(srt, oracle, torch_cuda, case, M_r, name):
     pk = cb.pack_dense(A, M_r, _oracle_table(so, M_r, name))
     _check_packed(plan.packed(), pk)
     C = _run(srt, torch, plan, B)
-    _bitexact(C, cb.exec_packed(pk, B))
+    _bitexact(C, cb.exec_packed(pk, B, nsplit=plan.split_counts()))
     assert np.all(np.abs(C - cb.dense_f64(A, B)) <= so.fp64_error_bound(A, B))
 
 
 @pytest.mark.parametrize("V", [1, 2, 4])
 @pytest.mark.parametrize("U", [1, 2, 4, 8])
-@pytest.mark.parametrize("waves,xcd", [(1, 0), (4, 1), (8, 1)])
-def test_launch_variants_bitexact(srt, oracle, torch_cuda, V, U, waves, xcd):
+@pytest.mark.parametrize("waves,nsplit,chunk,xcd", [(1, 1, 0, 0), (4, 1, 0, 1), (8, 8, 16, 1),
+                                                    (16, 16, 8, 1), (8, 4, 32, 0), (16, 2, 0, 1),
+                                                    (2, 2, 0, 1)])
+def test_launch_variants_bitexact(srt, oracle, torch_cuda, V, U, waves, nsplit, chunk, xcd):
     so, cb = oracle
     torch = torch_cuda
-    _log("variant V=%d U=%d waves=%d xcd=%d" % (V, U, waves, xcd))
+    _log("variant V=%d U=%d waves=%d nsplit=%d chunk=%d xcd=%d" % (V, U, waves, nsplit, chunk, xcd))
     A, B = srt.workloads.case_inputs(srt.workloads.SpMMCase(520, 300, 0.85, 264), seed=3)
     plan = _plan(srt, torch, A, 264, 8, "orig", vec_width=V, unroll=U, waves_per_block=waves,
-                 xcd_remap=xcd)
+                 xcd_remap=xcd, nsplit=nsplit, chunk_cols=chunk)
     cfg = plan.get_config()
-    assert cfg.vec_width == V and cfg.unroll == U and cfg.waves_per_block == waves
-    assert cfg.N_c == 64 * V and cfg.M_c == 8 * waves
+    assert cfg.unroll == U and cfg.nsplit <= cfg.waves_per_block
+    assert cfg.N_c == 64 * cfg.vec_width and cfg.M_c == 8 * cfg.waves_per_block
+    kp = plan.split_counts()
+    assert kp.min() >= 1 and kp.max() <= cfg.nsplit
     pk = cb.pack_dense(A, 8, _oracle_table(so, 8, "orig"))
-    _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B))
+    _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=kp))
+
+
+@pytest.mark.parametrize("nsplit,chunk", [(1, 0), (2, 64), (4, 32), (8, 16), (16, 8)])
+@pytest.mark.parametrize("M_r,name", [(8, "identity"), (4, "orig")])
+def test_split_chains_on_uneven_rows(srt, oracle, torch_cuda, nsplit, chunk, M_r, name):
+    """Heavy and empty panels (magnitude pruning with lognormal row scales),
+    load-balanced splits, against the oracle's chunked chains."""
+    so, cb = oracle
+    torch = torch_cuda
+    _log("split", nsplit, chunk, M_r, name)
+    A, B = srt.workloads.case_inputs(srt.workloads.SpMMCase(2048, 512, 0.9, 512), seed=0)
+    plan = _plan(srt, torch, A, 512, M_r, name, nsplit=nsplit, waves_per_block=16, chunk_cols=chunk,
+                 vec_width=1)
+    kp = plan.split_counts()
+    assert kp.max() <= nsplit
+    if nsplit > 1:
+        assert kp.max() > kp.min()          # heavy panels get more waves
+    pk = cb.pack_dense(A, M_r, _oracle_table(so, M_r, name))
+    C = _run(srt, torch, plan, B)
+    _bitexact(C, cb.exec_packed(pk, B, nsplit=kp))
+    assert np.all(np.abs(C - cb.dense_f64(A, B)) <= so.fp64_error_bound(A, B))
 
 
 @pytest.mark.parametrize("shape", [(37, 50, 24), (1, 1, 1), (9, 3, 7), (1000, 300, 100), (64, 64, 65),
@@ -144,7 +169,7 @@ def test_ragged_shapes(srt, oracle, torch_cuda, shape, index_dtype):
         plan = _plan(srt, torch, A, N, M_r, name, index_dtype=index_dtype)
         pk = cb.pack_dense(A, M_r, _oracle_table(so, M_r, name))
         _check_packed(plan.packed(), pk)
-        _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B))
+        _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=plan.split_counts()))
 
 
 def test_empty_and_sparse_rows(srt, oracle, torch_cuda):
@@ -158,8 +183,11 @@ def test_empty_and_sparse_rows(srt, oracle, torch_cuda):
     assert np.array_equal(_run(srt, torch, plan, B), np.zeros((40, 16), np.float32))
     A[17, 3] = 2.0
     A[39, 29] = -1.0
-    plan = _plan(srt, torch, A, 16, 8, "orig")
-    _bitexact(_run(srt, torch, plan, B), cb.exec_packed(cb.pack_dense(A, 8, _oracle_table(so, 8, "orig")), B))
+    for ns in (1, 8):
+        plan = _plan(srt, torch, A, 16, 8, "orig", nsplit=ns, waves_per_block=8, chunk_cols=1)
+        _bitexact(_run(srt, torch, plan, B),
+                  cb.exec_packed(cb.pack_dense(A, 8, _oracle_table(so, 8, "orig")), B,
+                                 nsplit=plan.split_counts()))
 
 
 def test_strided_b_c_and_stream(srt, oracle, torch_cuda):
@@ -179,7 +207,8 @@ def test_strided_b_c_and_stream(srt, oracle, torch_cuda):
         plan(Cv, Bv, n=136)
     s.synchronize()
     C = Cv.cpu().numpy()
-    ref = cb.exec_packed(cb.pack_dense(A, 8, _oracle_table(so, 8, "orig")), Bfull[:, :136])
+    ref = cb.exec_packed(cb.pack_dense(A, 8, _oracle_table(so, 8, "orig")), Bfull[:, :136],
+                         nsplit=plan.split_counts())
     _bitexact(np.ascontiguousarray(C[:, :136]), ref)
     assert np.all(C[:, 136:] == 0)
     assert torch.all(Cbig[:, :8] == 0) and torch.all(Cbig[:, 208:] == 0)
@@ -229,11 +258,14 @@ def test_soptile_matches_reference_packing(srt, oracle, torch_cuda, M_r, name, s
         assert np.array_equal(got["cols"][got["col_ptr"][p]:got["col_ptr"][p + 1]], pd["col_indices"])
         assert np.array_equal(got["vals"][got["val_ptr"][p]:got["val_ptr"][p + 1]], pd["values"])
     assert st.padding() == sum(pd["padding"] for pd in panels)
-    ns, C = module.execute_tile(st, torch.from_numpy(B).cuda(), num_runs=2, measured_iterations=3)
-    assert ns > 0
-    C = C.cpu().numpy()
     pk = cb.pack_dense(tile, M_r, table)
-    _bitexact(C, cb.exec_packed(pk, B))
+    for nsplit in (1, 4):
+        ns, C = module.execute_tile(st, torch.from_numpy(B).cuda(), num_runs=2, measured_iterations=3,
+                                    tile_config=srt.TileConfig(nsplit=nsplit, waves_per_block=8,
+                                                               chunk_cols=8))
+        assert ns > 0
+        C = C.cpu().numpy()
+        _bitexact(C, cb.exec_packed(pk, B, nsplit=st.split_counts()))
     assert torch.allclose(torch.from_numpy(C), torch.from_numpy(tile) @ torch.from_numpy(B), atol=1e-4)
 
 
