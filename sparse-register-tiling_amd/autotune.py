@@ -1,31 +1,33 @@
-"""Autotuner: picks panel height / pattern set, lane width, unroll and
-workgroup shape for one sparse matrix and B width by timing the candidates.
+"""Autotuner: picks the pattern set (panel height M_r and mapping) and the GPU
+realisation of the tile config (lane width, unroll, waves per workgroup,
+panel split) for one sparse matrix and B width by timing the candidates.
 
 Replaces the reference's empirical heuristic (tools/empirical_heuristic_*.json,
 built by tools/paper/heuristic.py:26-50 and consumed by
 tools/experiment_runner.py:86-113: sparsity bucket x bcols -> up to 6 nano
 configs) and the testbed's grid search (cpp_testbed/src/spmm/SpMMFunctor.h:
 56-185: set_config, setup, warm-up, median of timed runs, keep the best).
-Timings are device times from HIP events on the launch stream.  Results are
-cached per (M, K, sparsity bucket, N) in a JSON file whose layout follows the
-heuristic file: bucket -> bcols -> ranked configs.
+Each candidate is timed as a HIP graph of back-to-back launches replayed on
+one stream, bracketed by HIP events, so host launch cost does not enter.
+Results are cached per (M, K, sparsity bucket, N) in a JSON file whose layout
+follows the heuristic file: bucket -> bcols -> ranked configs.
 """
 from __future__ import annotations
 
 import json
 import os
 import statistics
-from dataclasses import asdict
-from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
-import numpy as np
 import torch
 
 from .matmul import MatMulSpecialized, TileConfig
 from .mapping import named_mapping
 
 DEFAULT_MAPPINGS = [(8, "orig"), (8, "identity"), (4, "identity"), (4, "orig")]
-DEFAULT_SHAPES = [(v, u, w) for v in (1, 2, 4) for u in (4, 8) for w in (1, 2, 4)]
+# GPU realisations tried per mapping: lane width x unroll x (waves, split).
+DEFAULT_SHAPES = [dict(vec_width=v, unroll=u, waves_per_block=w, nsplit=s)
+                  for v in (1, 2, 4) for u in (4, 8) for (w, s) in ((8, 1), (4, 4), (8, 8), (16, 16))]
 BUCKETS = [(0.0, 0.7), (0.7, 0.8), (0.8, 0.9), (0.9, 1.0)]
 
 
@@ -37,28 +39,44 @@ def sparsity_bucket(sparsity: float) -> str:
     return "(0.9,1]"
 
 
-def time_plan(plan: MatMulSpecialized, B: torch.Tensor, C: torch.Tensor, reps: int = 50,
-              blocks: int = 3) -> float:
-    """Median over ``blocks`` of the mean device time of ``reps`` launches, in seconds."""
-    for _ in range(5):
-        plan(C, B)
+def time_graph(launch, reps: int = 50, blocks: int = 3) -> float:
+    """Median over ``blocks`` replays of a graph holding ``reps`` calls of
+    ``launch()``; returns device seconds per call."""
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            launch()
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for _ in range(reps):
+            launch()
+    g.replay()
+    torch.cuda.synchronize()
     times = []
     for _ in range(blocks):
-        s = torch.cuda.Event(enable_timing=True)
-        e = torch.cuda.Event(enable_timing=True)
-        s.record()
-        for _ in range(reps):
-            plan(C, B)
-        e.record()
-        e.synchronize()
-        times.append(s.elapsed_time(e) / 1e3 / reps)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        cur = torch.cuda.current_stream()
+        a.record(cur)
+        g.replay()
+        b.record(cur)
+        b.synchronize()
+        times.append(a.elapsed_time(b) / 1e3 / reps)
+    del g
     return statistics.median(times)
 
 
+def time_plan(plan: MatMulSpecialized, B: torch.Tensor, C: torch.Tensor, reps: int = 50,
+              blocks: int = 3) -> float:
+    return time_graph(lambda: plan(C, B), reps, blocks)
+
+
 def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAULT_MAPPINGS,
-         shapes: Sequence[Tuple[int, int, int]] = DEFAULT_SHAPES, reps: int = 50,
+         shapes: Sequence[Dict] = DEFAULT_SHAPES, reps: int = 50,
          B: Optional[torch.Tensor] = None) -> List[Dict]:
-    """Time every (mapping, V, U, waves) candidate; returns records sorted fastest first."""
+    """Time every (mapping, shape) candidate; returns records sorted fastest
+    first.  A shape is a dict of TileConfig fields."""
     M, K = csr.shape
     if B is None:
         B = torch.rand((K, N), device="cuda") * 2 - 1
@@ -66,14 +84,15 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
     out = []
     for M_r, name in mappings:
         base = MatMulSpecialized.from_csr(csr, N, mapping_id=named_mapping(M_r, name))
-        for V, U, W in shapes:
-            if N % V:
+        for shape in shapes:
+            if N % max(1, shape.get("vec_width", 1)):
                 continue
-            cfg = TileConfig(vec_width=V, unroll=U, waves_per_block=W, xcd_remap=1)
+            cfg = TileConfig(xcd_remap=1, **shape)
             plan = MatMulSpecialized.from_csr(csr, N, mapping_id=base.mapping, tile_config=cfg)
             t = time_plan(plan, B, C, reps)
-            out.append({"M_r": M_r, "mapping": name, "vec_width": V, "unroll": U, "waves_per_block": W,
-                        "seconds": t, "padding": plan.stats()["padding"]})
+            rec = {"M_r": M_r, "mapping": name, "seconds": t, "padding": plan.stats()["padding"]}
+            rec.update(shape)
+            out.append(rec)
             plan.close()
         base.close()
     out.sort(key=lambda r: r["seconds"])
@@ -104,5 +123,5 @@ class TuningCache:
 
 def best_config(records: List[Dict]) -> Tuple[Tuple[int, str], TileConfig]:
     r = records[0]
-    return (r["M_r"], r["mapping"]), TileConfig(vec_width=r["vec_width"], unroll=r["unroll"],
-                                                waves_per_block=r["waves_per_block"], xcd_remap=1)
+    kw = {k: v for k, v in r.items() if k in TileConfig.__dataclass_fields__}
+    return (r["M_r"], r["mapping"]), TileConfig(xcd_remap=1, **kw)
