@@ -52,7 +52,7 @@ COPY_BYTES = 256 << 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "executor_north_star_traffic.txt")
 EXEC_KERNEL = "exec_kernel"
 # realised tile-config fields that name a measured configuration
-GPU_FIELDS = ("K_c", "vec_width", "unroll", "waves_per_block", "nsplit", "chunk_cols")
+GPU_FIELDS = ("K_c", "vec_width", "unroll", "waves_per_block", "nsplit", "chunk_cols", "b_staging")
 
 
 def log(*a):
@@ -70,6 +70,7 @@ def parse():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--nsplit", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--staging", type=int, default=0, help="B path: 1 LDS tile, 2 global rows, 0 choose")
     ap.add_argument("--config", help="JSON file written by --save-config")
     ap.add_argument("--save-config", help="write the chosen configuration here")
     ap.add_argument("--replicas", type=int, default=0, help="0 = enough to exceed 2x the 256 MB cache")
@@ -225,7 +226,7 @@ def choose_config(args, srt, csr, N, Bd):
         recs = autotune.tune(csr, N, reps=args.tune_reps, B=Bd)
         (M_r, name), cfg = autotune.best_config(recs)
         tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
-                                             "chunk_cols", "xcd_remap")}
+                                             "chunk_cols", "xcd_remap", "b_staging")}
         tune_top = [{k: (round(v * 1e6, 3) if k == "seconds" else v) for k, v in r.items()}
                     for r in recs[:5]]
         log("autotune best (us):", tune_top[0])
@@ -233,11 +234,11 @@ def choose_config(args, srt, csr, N, Bd):
         mr, name = args.mapping.split(":")
         M_r = int(mr)
         tile = dict(vec_width=args.vec_width, unroll=args.unroll, waves_per_block=args.waves,
-                    nsplit=args.nsplit, chunk_cols=args.chunk, xcd_remap=1)
+                    nsplit=args.nsplit, chunk_cols=args.chunk, xcd_remap=1, b_staging=args.staging)
     if args.save_config:
-        flags = ("--mapping %d:%s --vec-width %d --unroll %d --waves %d --nsplit %d --chunk %d"
+        flags = ("--mapping %d:%s --vec-width %d --unroll %d --waves %d --nsplit %d --chunk %d --staging %d"
                  % (M_r, name, tile["vec_width"], tile["unroll"], tile["waves_per_block"], tile["nsplit"],
-                    tile["chunk_cols"]))
+                    tile["chunk_cols"], tile.get("b_staging", 0)))
         with open(args.save_config, "w") as f:
             json.dump({"M_r": M_r, "mapping": name, "tile": tile, "flags": flags}, f)
     return M_r, name, srt.TileConfig(**tile), tune_top

@@ -78,6 +78,10 @@ typedef struct srt_tile_config {
     int chunk_cols;        /* target packed columns per wave: panel p runs on
                               kp = min(nsplit, max(1, ceil(ncols_p / chunk_cols)))
                               waves (load balance); 0 = choose                      */
+    int b_staging;         /* B operand path: 1 = copy the workgroup's B tile (K rows x
+                              N_c columns) into LDS once and read packed columns
+                              from there; 2 = read B rows from global memory per
+                              packed column; 0 = choose (global memory)          */
 } srt_tile_config;
 
 /* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54). */

@@ -25,9 +25,11 @@ from .matmul import MatMulSpecialized, TileConfig
 from .mapping import named_mapping
 
 DEFAULT_MAPPINGS = [(8, "orig"), (8, "identity"), (4, "identity"), (4, "orig")]
-# GPU realisations tried per mapping: lane width x unroll x (waves, split).
-DEFAULT_SHAPES = [dict(vec_width=v, unroll=u, waves_per_block=w, nsplit=s)
-                  for v in (1, 2, 4) for u in (4, 8) for (w, s) in ((8, 1), (4, 4), (8, 8), (16, 16))]
+# GPU realisations tried per mapping: B path x lane width x unroll x (waves, split).
+DEFAULT_SHAPES = ([dict(b_staging=1, vec_width=1, unroll=u, waves_per_block=16, nsplit=s)
+                   for u in (4, 8) for s in (2, 4, 16)]
+                  + [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nsplit=s)
+                     for v in (2, 4) for u in (4, 8) for (w, s) in ((8, 8), (16, 16), (4, 4))])
 BUCKETS = [(0.0, 0.7), (0.7, 0.8), (0.8, 0.9), (0.9, 1.0)]
 
 
@@ -89,9 +91,12 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
                 continue
             cfg = TileConfig(xcd_remap=1, **shape)
             plan = MatMulSpecialized.from_csr(csr, N, mapping_id=base.mapping, tile_config=cfg)
+            real = plan.get_config()
             t = time_plan(plan, B, C, reps)
             rec = {"M_r": M_r, "mapping": name, "seconds": t, "padding": plan.stats()["padding"]}
             rec.update(shape)
+            rec["realised"] = {k: getattr(real, k) for k in ("b_staging", "vec_width", "waves_per_block",
+                                                             "nsplit", "chunk_cols")}
             out.append(rec)
             plan.close()
         base.close()

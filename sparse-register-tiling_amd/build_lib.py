@@ -32,6 +32,10 @@ CFLAGS = [
     # Keep every multiply-add explicit: results must reproduce the reference's
     # fmaf chains bit for bit.
     "-ffp-contract=off",
+    # Leave uniform (wave-invariant) branches unstructurized: the executors
+    # dispatch on wave-uniform kernel ids, and structurizing those switches
+    # chains the cases through flag blocks (register copies, extra waits).
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I" + INCLUDE, "-I" + CSRC,
 ]

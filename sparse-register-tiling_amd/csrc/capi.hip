@@ -286,6 +286,8 @@ void srt_tile_config_default(srt_tile_config *c) {
     c->xcd_remap = -1;
     c->unroll = 0;
     c->nsplit = 0;
+    c->chunk_cols = 0;
+    c->b_staging = 0;
 }
 
 // ----------------------------------------------------------------- mapping
@@ -371,35 +373,60 @@ namespace srt {
 
 static int log2_small(int v) { return v >= 8 ? 3 : v >= 4 ? 2 : v >= 2 ? 1 : 0; }
 
-// Realise the GPU fields of a tile config for a matrix with P panels and B
-// width n (the choices the reference's heuristic JSON made per sparsity
-// bucket are made here from the launch geometry).
 static bool pow2_upto8(int v) { return v == 1 || v == 2 || v == 4 || v == 8; }
 
-// Realise the GPU fields of a tile config for a matrix with P panels, on
-// average `avg_cols` packed columns per panel, and B width n (the choices the
-// reference's heuristic JSON made per sparsity bucket are made here from the
-// launch geometry; the autotuner overrides them with measured choices).
-static constexpr int kMaxWaves = 16;   // wave slots per workgroup (launch bound 1024)
-static constexpr size_t kMaxLds = 65536;
+static constexpr int kMaxWaves = 16;        // wave slots per workgroup (launch bound 1024)
+static constexpr size_t kMaxLdsL2 = 65536;  // reduction LDS of the global-memory B path
 
 static size_t reduce_lds(int waves, int M_r, int V) { return (size_t)waves * M_r * V * 64 * sizeof(float); }
+static size_t tile_lds(int K, int V) { return (size_t)K * 64 * V * sizeof(float); }
 
+size_t lds_limit() {
+    static int cached[64];  // per device, 0 = not queried
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return kMaxLdsL2;
+    if (!cached[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = (int)kMaxLdsL2;
+        cached[dev] = v;
+    }
+    return (size_t)cached[dev];
+}
+
+// Whether a launch of lane width V and `waves` slots fits the LDS budget of
+// its B path (stage: staged tile and reduction share one allocation).
+static bool lds_fits(bool stage, int K, int waves, int M_r, int V, bool reduce) {
+    const size_t red = reduce ? reduce_lds(waves, M_r, V) : 0;
+    if (!stage) return red <= kMaxLdsL2;
+    return std::max(red, tile_lds(K, V)) <= lds_limit();
+}
+
+// Realise the GPU fields of a tile config for a matrix of K columns with
+// `total_cols` packed columns and B width n (the choices the reference's
+// heuristic JSON made per sparsity bucket are made here from the launch
+// geometry; the autotuner overrides them with measured choices).
 static void realise_config(srt_tile_config &c, int M_r, int K, int64_t total_cols, int n) {
     if (c.vec_width != 1 && c.vec_width != 2 && c.vec_width != 4)
         c.vec_width = (n % 4 == 0 && n >= 256) ? 4 : (n % 2 == 0 && n >= 128) ? 2 : 1;
     if (c.waves_per_block < 1 || c.waves_per_block > kMaxWaves) c.waves_per_block = 8;
     if (c.nsplit < 1 || c.nsplit > c.waves_per_block) c.nsplit = c.waves_per_block;
-    // The chunk reduction keeps one partial tile per wave slot in LDS.
-    while (c.nsplit > 1 && c.vec_width > 1 && reduce_lds(c.waves_per_block, M_r, c.vec_width) > kMaxLds)
+    // B path: global-memory rows unless the LDS tile is asked for and its
+    // narrowest form (V = 1) fits (measured faster on the pruned-FFN shapes).
+    if (c.b_staging != 1 && c.b_staging != 2) c.b_staging = 2;
+    const bool can_stage = tile_lds(std::max(K, 1), 1) <= lds_limit();
+    if (c.b_staging == 1 && !can_stage) c.b_staging = 2;
+    const bool stage = c.b_staging == 1;
+    const bool split = c.nsplit > 1;
+    while (c.vec_width > 1 && !lds_fits(stage, K, c.waves_per_block, M_r, c.vec_width, split))
         c.vec_width >>= 1;
-    while (c.nsplit > 1 && reduce_lds(c.waves_per_block, M_r, c.vec_width) > kMaxLds) {
+    while (c.waves_per_block > 1 && !lds_fits(stage, K, c.waves_per_block, M_r, c.vec_width, split)) {
         c.waves_per_block >>= 1;
         c.nsplit = std::min(c.nsplit, c.waves_per_block);
     }
     if (c.chunk_cols < 1) {
         // About four waves per SIMD over the whole launch: long enough chains
-        // to amortise the chunk start, short enough to hide each other's latency.
+        // to amortise the chunk start, short enough to hide latency.
         const int64_t n_tiles = (n + 64 * c.vec_width - 1) / (64 * c.vec_width);
         const int64_t target_waves = 4096;
         c.chunk_cols = (int)std::max<int64_t>(8, (total_cols * n_tiles + target_waves - 1) / target_waves);
@@ -426,14 +453,16 @@ static int run_executor(const Packed &pk, const Schedule &sch, const ExecutorDes
     // Vector loads need V | n, V | ldb, V | ldc and 4V-byte aligned B and C.
     while (V > 1 && (n % V || ldb % V || ldc % V || !aligned(B, 4 * V) || !aligned(C, 4 * V)))
         V >>= 1;
-    while (sch.reduce && V > 1 && reduce_lds(sch.waves_per_block, pk.M_r, V) > kMaxLds) V >>= 1;
-    if (sch.reduce && reduce_lds(sch.waves_per_block, pk.M_r, V) > kMaxLds)
-        return fail(SRT_ERR_UNSUPPORTED, "chunk reduction needs more than 64 KiB of LDS");
-    int ui = log2_small(cfg.unroll);
+    bool stage = cfg.b_staging == 1 && tile_lds(pk.K, 1) <= lds_limit();
+    while (V > 1 && !lds_fits(stage, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce)) V >>= 1;
+    if (stage && !lds_fits(true, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce)) stage = false;
+    if (!lds_fits(stage, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce))
+        return fail(SRT_ERR_UNSUPPORTED, "chunk reduction needs more LDS than a workgroup has");
+    const int ui = log2_small(cfg.unroll);
     const int vi = log2_small(V);
     ExecLaunchFn fn = nullptr;
-    for (int u = ui; u >= 0 && !fn; --u) fn = exec->launch[vi][u];
-    for (int u = ui + 1; u < 4 && !fn; ++u) fn = exec->launch[vi][u];
+    for (int u = ui; u >= 0 && !fn; --u) fn = exec->launch[stage][vi][u];
+    for (int u = ui + 1; u < 4 && !fn; ++u) fn = exec->launch[stage][vi][u];
     if (!fn) return fail(SRT_ERR_UNSUPPORTED, "executor variant not generated");
     // B rows are addressed with a 32-bit byte offset (buffer loads).
     const int64_t b_extent = (int64_t)pk.K * ldb * (int64_t)sizeof(float);
@@ -456,6 +485,7 @@ static int run_executor(const Packed &pk, const Schedule &sch, const ExecutorDes
     a.ldc = ldc;
     a.M = pk.M;
     a.N = n;
+    a.K = pk.K;
     a.num_panels = pk.P;
     a.load_c = load_c;
     a.vec_width = V;
@@ -508,8 +538,9 @@ static int build_groups(Packed &pk, const int32_t *d_id_pattern, hipStream_t s) 
     SRT_HIP_TRY(hipMemcpyAsync(&tot, pk.grp_ptr + pk.P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     SRT_HIP_TRY(hipStreamSynchronize(s));
     pk.total_groups = tot;
-    SRT_TRY(dmalloc(&pk.groups, 2 * (size_t)tot));
-    SRT_HIP_TRY(launch_group_fill(pk.P, pk.NK, pk.counts, pk.grp_ptr, pk.groups, s));
+    SRT_TRY(dmalloc(&pk.groups, 2 * ((size_t)tot + SRT_GROUP_PAD)));
+    SRT_HIP_TRY(hipMemsetAsync(pk.groups + 2 * tot, 0, sizeof(int32_t) * 2 * SRT_GROUP_PAD, s));
+    SRT_HIP_TRY(launch_group_fill(pk.P, pk.NK, pk.counts, pk.grp_ptr, d_id_pattern, pk.groups, s));
     SRT_TRY(dmalloc(&pk.id_pattern, (size_t)std::max(pk.NK, 1)));
     if (pk.NK)
         SRT_HIP_TRY(hipMemcpyAsync(pk.id_pattern, d_id_pattern, sizeof(int32_t) * pk.NK,

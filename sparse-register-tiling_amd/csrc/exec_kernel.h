@@ -27,6 +27,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "srt_internal.h"
 
 namespace srt {
@@ -73,6 +75,18 @@ struct BRows {
             return __builtin_bit_cast(typename VecOf<4>::type,
                                       __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, so, 0));
         }
+    }
+};
+
+// LDS view of a staged B tile: row k of the workgroup's 64*V columns sits at
+// lds[k * 64 * V ..), lane l reads its V columns at offset l * V.  One wave
+// reads one contiguous 256*V-byte row: no bank conflicts.
+template <int V>
+struct LdsRows {
+    const float *base;  // staged tile + lane * V
+
+    __device__ __forceinline__ typename VecOf<V>::type load(int col) const {
+        return *reinterpret_cast<const typename VecOf<V>::type *>(base + col * (64 * V));
     }
 };
 
@@ -134,76 +148,123 @@ __device__ __forceinline__ void fma_pattern(float (&acc)[MR][V], const float *__
     }
 }
 
-// All `count` (>= 1) packed columns of one kernel pattern.
-// On entry cur[0 .. min(U, count)) holds the B rows of the group's first
-// columns (cp[0 ..)).  On exit cp/vp are advanced past the group and
-// cur[0 .. min(U, next_count)) holds the rows of the next group's first
-// columns.
+// The packed columns of a wave's chunk form a stream of batches: each group
+// (kernel id, count) is cut into batches of U columns, the last one partial.
+// A batch is run by one case of the generated switch, keyed id * U + n - 1
+// for n columns: n unrolled columns of PC = popcount(pattern) fused
+// multiply-adds each, accumulator rows fixed at compile time.
 //
-// Scalar metadata is read in whole batches, outside any branch, so the
-// compiler issues wide s_load_dwordx{4,8,16} and waits once per batch: the
-// column indices of the next U columns, then the values of the current
-// columns UG at a time (UG * popcount <= 32 SGPRs).  These batch reads may
-// run past the panel's last column; the packed arrays carry SRT_COL_PAD /
-// SRT_VAL_PAD zero entries at their end so they stay in bounds, and entries
-// past the group are never used.
+// Software pipeline.  Column indices and values are read through the scalar
+// unit (wave-uniform SGPR operands).  Scalar loads and LDS reads share one
+// wait counter and scalar loads may return out of order, so a wait for a
+// scalar load also waits for every LDS read in flight.  Each iteration
+// therefore first consumes what the previous one loaded (one wait), then
+// issues everything the next iterations need, then runs its FMAs on
+// registers that have already arrived:
 //
-// One batch: the FMAs of up to U columns held in `src` while the loads of the
-// following batch go into `dst`.  Returns true when the group is finished
-// (then `dst` holds the next group's first columns).
-template <int PAT, int MR, int V, int U>
-__device__ __forceinline__ bool group_batch(float (&acc)[MR][V], typename VecOf<V>::type (&src)[U],
-                                            typename VecOf<V>::type (&dst)[U], int &left,
-                                            int next_count, const int32_t *__restrict__ &cp,
-                                            const float *__restrict__ &vp, const BRows<V> &bb) {
+//   iteration i:  rows'  <- B rows of batch i+1   (indices loaded by iteration i-1)
+//                 cols'' <- column indices of batch i+2
+//                 vals'  <- first W values of batch i+1
+//                 FMAs of batch i from rows and vals
+//
+// All U rows of a batch are loaded whatever its size (indices past the chunk
+// are valid indices of later columns or zero padding), so the loads carry no
+// branches.  A batch that needs more than W values reads the rest itself.
+// The packed arrays carry SRT_COL_PAD / SRT_VAL_PAD zero entries at their
+// end, and group lists SRT_GROUP_PAD zero entries, so every look-ahead read
+// stays in bounds; entries past the chunk are never used.
+constexpr int kValWindow = 16;
+
+// FMAs of one batch of NCOL columns of pattern PAT.
+template <int PAT, int MR, int V, int U, int NCOL>
+__device__ __forceinline__ void batch_fmas(float (&acc)[MR][V], const typename VecOf<V>::type (&rows)[U],
+                                           const float (&av)[kValWindow], const float *__restrict__ vp) {
     constexpr int PC = __builtin_popcount(PAT);
-    constexpr int UG = PC * U <= 32 ? U : (32 / PC > 0 ? 32 / PC : 1);
-    const int vc = left < U ? left : U;
-    left -= vc;
-    const int want = left > 0 ? left : next_count;
-    const int vn = want < U ? want : U;
-    int col[U];
-    load_uniform(cp + vc, col);
+    constexpr int UW = kValWindow / PC < NCOL ? kValWindow / PC : NCOL;  // columns in the window
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-        if (u < vn) dst[u] = bb.load(col[u]);
+    for (int u = 0; u < UW; ++u) fma_pattern<PAT, MR, V>(acc, av + u * PC, rows[u]);
+    if constexpr (UW < NCOL) {
+        float rest[(NCOL - UW) * PC];
+        load_uniform(vp + UW * PC, rest);
 #pragma unroll
-    for (int s = 0; s < U; s += UG) {
-        if (s < vc) {
-            float av[UG * PC];
-            load_uniform(vp + s * PC, av);
-#pragma unroll
-            for (int u = 0; u < UG; ++u)
-                if (s + u < U && s + u < vc) fma_pattern<PAT, MR, V>(acc, av + u * PC, src[s + u]);
-        }
-    }
-    cp += vc;
-    vp += vc * PC;
-    return left <= 0;
-}
-
-// Batches alternate between two register buffers, so no batch copies its
-// prefetched rows; a group that ends on the second buffer copies it once.
-template <int PAT, int MR, int V, int U>
-__device__ __forceinline__ void run_group(float (&acc)[MR][V], typename VecOf<V>::type (&cur)[U],
-                                          int count, int next_count,
-                                          const int32_t *__restrict__ &cp,
-                                          const float *__restrict__ &vp, const BRows<V> &bb) {
-    typedef typename VecOf<V>::type vec;
-    vec alt[U];
-    int left = count;
-    while (true) {
-        if (group_batch<PAT, MR, V, U>(acc, cur, alt, left, next_count, cp, vp, bb)) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) cur[u] = alt[u];
-            return;
-        }
-        if (group_batch<PAT, MR, V, U>(acc, alt, cur, left, next_count, cp, vp, bb)) return;
+        for (int u = UW; u < NCOL; ++u) fma_pattern<PAT, MR, V>(acc, rest + (u - UW) * PC, rows[u]);
     }
 }
 
-// D::run<V, U>(id, acc, cur, count, next_count, cp, vp, bb) is the generated
-// switch over the pattern set's ids.
+// Batch generator over a wave's chunk: (id, pc, n) of successive batches.
+// Group entries are (id | pc << 16, count); the entry after the current one
+// is already in registers and the one after that is being loaded.
+template <int U>
+struct BatchGen {
+    const int2 *__restrict__ gl;  // entry of the current group
+    int2 nxt;                     // entry gl[1]
+    int gleft;                    // columns of the current group not yet batched
+    int rem;                      // columns of the chunk not yet batched
+    int key;                      // id * U of the current group
+    int pc;
+
+    __device__ __forceinline__ void start(const int2 *__restrict__ g, int skip, int total) {
+        gl = g;
+        const int2 e = g[0];
+        nxt = g[1];  // padded: in bounds
+        rem = total;
+        gleft = e.y - skip < total ? e.y - skip : total;
+        key = (e.x & 0xffff) * U;
+        pc = e.x >> 16;
+    }
+    // Next batch: n columns (0 at the end of the chunk), key and pc.
+    __device__ __forceinline__ int next(int &bkey, int &bpc) {
+        if (rem <= 0) {
+            bkey = 0;
+            bpc = 0;
+            return 0;
+        }
+        if (gleft <= 0) {
+            ++gl;
+            const int2 e = nxt;
+            nxt = gl[1];
+            gleft = e.y < rem ? e.y : rem;
+            key = (e.x & 0xffff) * U;
+            pc = e.x >> 16;
+        }
+        const int n = gleft < U ? gleft : U;
+        gleft -= n;
+        rem -= n;
+        bkey = key + n - 1;
+        bpc = pc;
+        return n;
+    }
+};
+
+// Pipeline registers of one iteration.
+template <int V, int U>
+struct Stage {
+    typename VecOf<V>::type rows[U];  // B rows of the batch
+    float vals[kValWindow];           // its first values
+    int cols_next[U];                 // column indices of the following batch
+};
+
+// One pipeline iteration: batch (n, key, pc) in `in`, the following batch
+// of n1 columns is prepared in `out`, and the generator yields the batch
+// after it (n2).
+template <int MR, int V, int U, class D, class R>
+__device__ __forceinline__ void pipe_step(float (&acc)[MR][V], const Stage<V, U> &in, Stage<V, U> &out,
+                                          int n, int key, int pc, int n1, int &n2, int &key2, int &pc2,
+                                          BatchGen<U> &gen, const int32_t *__restrict__ &cp,
+                                          const float *__restrict__ &vp, const R &bb) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) out.rows[u] = bb.load(in.cols_next[u]);
+    n2 = gen.next(key2, pc2);
+    load_uniform(cp + n + n1, out.cols_next);
+    load_uniform(vp + pc * n, out.vals);
+    D::template fmas<V, U>(key, acc, in.rows, in.vals, vp);
+    cp += n;
+    vp += pc * n;
+}
+
+// D::fmas<V, U>(key, acc, rows, vals, vp) is the generated switch over
+// (kernel id, batch size); R reads B rows (BRows from global memory, LdsRows
+// from the staged tile).
 //
 // Work decomposition (load balanced, like the reference's KD_*LoadBalanced
 // kernel descriptors): panel p is cut into k_p contiguous chunks of its packed
@@ -217,16 +278,23 @@ __device__ __forceinline__ void run_group(float (&acc)[MR][V], typename VecOf<V>
 //   C = ((chain_0 + chain_1) + chain_2) + ...
 // chain_0 starting from C (load_c) or 0, the others from 0.  With every
 // k_p = 1 this is exactly the reference's single chain per panel.
-template <int MR, int V, int U, class D>
+//
+// STAGE = true: the workgroup first copies its whole B tile (K rows x 64*V
+// columns) into LDS with coalesced 16-byte loads, so every B row is fetched
+// from L2 once per workgroup instead of once per packed column of every
+// panel; all packed-column reads then hit LDS.  The partial tiles of the chunk
+// reduction reuse the same LDS after a barrier.  Every wave, idle slots
+// included, takes part in the copy and reaches every barrier.
+template <int MR, int V, int U, bool STAGE, class D>
 __global__ void __launch_bounds__(1024)
     exec_kernel(const int2 *__restrict__ groups, const int64_t *__restrict__ grp_ptr,
                 const int4 *__restrict__ desc, const int64_t *__restrict__ col_ptr,
                 const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
                 const float *__restrict__ vals, const float *__restrict__ B, uint32_t b_bytes,
-                uint32_t ldb, float *__restrict__ C, int64_t ldc, int M, int N, int n_wblocks,
+                uint32_t ldb, float *__restrict__ C, int64_t ldc, int M, int N, int K, int n_wblocks,
                 int reduce, int load_c, int xcd_remap) {
     typedef typename VecOf<V>::type vec;
-    extern __shared__ float red[];  // [waves][MR][V][64]
+    extern __shared__ float lds[];  // STAGE: B tile [K][64*V]; reduction: [waves][MR][V][64]
     const int waves = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -239,11 +307,49 @@ __global__ void __launch_bounds__(1024)
     const int4 d1 = dw[1];
     const int p = d0.x;
     const bool live = p >= 0;  // wave-uniform
-    if (!reduce && !live) return;  // no barrier below when nothing is reduced
+    if (!STAGE && !reduce && !live) return;  // no barrier below
     const int n = ntile * (64 * V) + lane * V;
     const bool active = n < N;
     const int nl = active ? n : 0;  // inactive lanes read columns 0..V-1 (in bounds)
     const int kp = d1.z, j = d1.w;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(B), (short)0, (int)b_bytes, 0x00020000);
+
+    if constexpr (STAGE) {
+        // Row k, 16-byte chunk c of the tile <- B[k][n0 + 4c .. n0 + 4c + 4).
+        // Chunks past column N of the last tile read the next row (or 0 past
+        // the end of the buffer); only inactive lanes see them.
+        // Eight loads in flight per lane before their LDS writes.
+        constexpr int CH = 16 * V;
+        constexpr int BATCH = 8;
+        const uint32_t n0_bytes = (uint32_t)(ntile * (64 * V)) * 4u;
+        const int total = K * CH;
+        const int nthr = blockDim.x;
+        for (int i0 = threadIdx.x; i0 < total; i0 += BATCH * nthr) {
+            float4 v[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int i = i0 + u * nthr;
+                if (i < total) {
+                    // The row offset differs between lanes: it goes into the
+                    // per-lane offset (a scalar offset would be waterfalled).
+                    const int k = i / CH, c = i % CH;
+                    v[u] = __builtin_bit_cast(
+                        float4, __builtin_amdgcn_raw_buffer_load_b128(
+                                    rsrc, (uint32_t)k * ldb * 4u + n0_bytes + (uint32_t)c * 16u, 0, 0));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int i = i0 + u * nthr;
+                if (i < total) {
+                    const int k = i / CH, c = i % CH;
+                    *reinterpret_cast<float4 *>(lds + k * (64 * V) + c * 4) = v[u];
+                }
+            }
+        }
+        __syncthreads();
+    }
 
     float acc[MR][V];
 #pragma unroll
@@ -252,11 +358,6 @@ __global__ void __launch_bounds__(1024)
         for (int jj = 0; jj < V; ++jj) acc[r][jj] = 0.0f;
 
     if (live) {
-        BRows<V> bb;
-        bb.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(B), (short)0, (int)b_bytes,
-                                                    0x00020000);
-        bb.voff = (uint32_t)nl * 4u;
-        bb.row_bytes = ldb * 4u;
         if (load_c && j == 0) {
 #pragma unroll
             for (int r = 0; r < MR; ++r) {
@@ -270,37 +371,53 @@ __global__ void __launch_bounds__(1024)
         }
         int remaining = d1.y - d0.w;
         if (remaining > 0) {
-            const int2 *__restrict__ gl = groups + grp_ptr[p] + d0.y;
+            typedef typename std::conditional<STAGE, LdsRows<V>, BRows<V>>::type Rows;
+            Rows bb;
+            if constexpr (STAGE) {
+                bb.base = lds + lane * V;
+            } else {
+                bb.rsrc = rsrc;
+                bb.voff = (uint32_t)nl * 4u;
+                bb.row_bytes = ldb * 4u;
+            }
             const int32_t *__restrict__ cp = cols + col_ptr[p] + d0.w;
             const float *__restrict__ vp = vals + val_ptr[p] + d1.x;
-            int2 e = gl[0];
-            int cnt = e.y - d0.z < remaining ? e.y - d0.z : remaining;
-            vec cur[U];
+            BatchGen<U> gen;
+            gen.start(groups + grp_ptr[p] + d0.y, d0.z, remaining);
+            Stage<V, U> s0, s1;
+            int k0, q0, k1, q1, k2, q2;
+            int n0 = gen.next(k0, q0);
+            int n1 = gen.next(k1, q1);
             {
-                const int c0 = cnt < U ? cnt : U;
                 int col[U];
-                load_uniform(cp, col);  // padded: in bounds
+                load_uniform(cp, col);  // padded: in bounds, valid column indices
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (u < c0) cur[u] = bb.load(col[u]);
+                for (int u = 0; u < U; ++u) s0.rows[u] = bb.load(col[u]);
+                load_uniform(cp + n0, s0.cols_next);
+                load_uniform(vp, s0.vals);
             }
+            // Two iterations per trip so the stages alternate without copies.
             while (true) {
-                remaining -= cnt;
-                int2 en = make_int2(0, 0);
-                if (remaining > 0) en = gl[1];
-                const int ncnt = en.y < remaining ? en.y : remaining;
-                D::template run<V, U>(e.x, acc, cur, cnt, ncnt, cp, vp, bb);
-                if (remaining <= 0) break;
-                e = en;
-                cnt = ncnt;
-                ++gl;
+                int n2;
+                pipe_step<MR, V, U, D, Rows>(acc, s0, s1, n0, k0, q0, n1, n2, k2, q2, gen, cp, vp, bb);
+                if (n1 <= 0) break;
+                int n3, k3, q3;
+                pipe_step<MR, V, U, D, Rows>(acc, s1, s0, n1, k1, q1, n2, n3, k3, q3, gen, cp, vp, bb);
+                if (n2 <= 0) break;
+                n0 = n2;
+                k0 = k2;
+                q0 = q2;
+                n1 = n3;
+                k1 = k3;
+                q1 = q3;
             }
         }
     }
 
     if (reduce) {
+        if constexpr (STAGE) __syncthreads();  // every wave is done with the staged tile
         if (live && j > 0) {
-            float *mine = red + (int64_t)wave * MR * V * 64;
+            float *mine = lds + (int64_t)wave * MR * V * 64;
 #pragma unroll
             for (int r = 0; r < MR; ++r)
 #pragma unroll
@@ -309,13 +426,14 @@ __global__ void __launch_bounds__(1024)
         __syncthreads();
         if (!live || j > 0) return;
         for (int w = 1; w < kp; ++w) {
-            const float *part = red + (int64_t)(wave + w) * MR * V * 64;
+            const float *part = lds + (int64_t)(wave + w) * MR * V * 64;
 #pragma unroll
             for (int r = 0; r < MR; ++r)
 #pragma unroll
                 for (int jj = 0; jj < V; ++jj) acc[r][jj] = acc[r][jj] + part[(r * V + jj) * 64 + lane];
         }
     }
+    if (!live) return;
 
     if (active) {
 #pragma unroll
@@ -335,24 +453,39 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
-// LDS bytes the chunk reduction needs: one partial tile per wave slot.
-inline size_t exec_lds_bytes(int MR, int V, int waves, int reduce) {
+// LDS bytes of the chunk reduction: one partial tile per wave slot.
+inline size_t reduce_lds_bytes(int MR, int V, int waves, int reduce) {
     return reduce ? (size_t)waves * MR * V * 64 * sizeof(float) : 0;
 }
+// LDS bytes of one launch: the staged B tile (STAGE) and the reduction share it.
+inline size_t exec_lds_bytes(int MR, int V, int waves, int reduce, bool stage, int K) {
+    const size_t red = reduce_lds_bytes(MR, V, waves, reduce);
+    const size_t tile = stage ? (size_t)K * 64 * V * sizeof(float) : 0;
+    return red > tile ? red : tile;
+}
 
-template <int MR, int V, int U, class D>
+template <int MR, int V, int U, bool STAGE, class D>
 hipError_t launch_exec(const ExecLaunch &a, hipStream_t s) {
     const int n_tiles = (a.N + 64 * V - 1) / (64 * V);
     const long long nblocks = (long long)a.n_wblocks * n_tiles;
     if (nblocks == 0) return hipSuccess;
     if (nblocks > 0x7fffffffLL || a.waves_per_block < 1 || a.waves_per_block > 16)
         return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((exec_kernel<MR, V, U, D>), dim3((unsigned)nblocks), dim3(64 * a.waves_per_block),
-                       exec_lds_bytes(MR, V, a.waves_per_block, a.reduce), s,
-                       reinterpret_cast<const int2 *>(a.groups), a.grp_ptr,
-                       reinterpret_cast<const int4 *>(a.desc), a.col_ptr, a.val_ptr, a.cols, a.vals,
-                       a.B, a.b_bytes, (uint32_t)a.ldb, a.C, a.ldc, a.M, a.N, a.n_wblocks, a.reduce,
-                       a.load_c, a.xcd_remap);
+    const size_t lds = exec_lds_bytes(MR, V, a.waves_per_block, a.reduce, STAGE, a.K);
+    if (lds > 65536) {
+        static bool raised = false;  // once per instantiation
+        if (!raised) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&exec_kernel<MR, V, U, STAGE, D>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
+            if (e != hipSuccess) return e;
+            raised = true;
+        }
+    }
+    hipLaunchKernelGGL((exec_kernel<MR, V, U, STAGE, D>), dim3((unsigned)nblocks),
+                       dim3(64 * a.waves_per_block), lds, s, reinterpret_cast<const int2 *>(a.groups),
+                       a.grp_ptr, reinterpret_cast<const int4 *>(a.desc), a.col_ptr, a.val_ptr, a.cols,
+                       a.vals, a.B, a.b_bytes, (uint32_t)a.ldb, a.C, a.ldc, a.M, a.N, a.K, a.n_wblocks,
+                       a.reduce, a.load_c, a.xcd_remap);
     return hipGetLastError();
 }
 

@@ -244,9 +244,11 @@ __global__ void group_count_kernel(int P, int NK, const int32_t *__restrict__ co
     ngroups[p] = g;
 }
 
-// One thread per panel: (id, count) of every non-empty id, id order.
+// One thread per panel: (id | popcount(pattern) << 16, count) of every
+// non-empty id, id order.
 __global__ void group_fill_kernel(int P, int NK, const int32_t *__restrict__ counts,
                                   const int64_t *__restrict__ grp_ptr,
+                                  const int32_t *__restrict__ id_pattern,
                                   int32_t *__restrict__ groups) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -254,7 +256,7 @@ __global__ void group_fill_kernel(int P, int NK, const int32_t *__restrict__ cou
     int32_t *out = groups + 2 * grp_ptr[p];
     for (int i = 0; i < NK; ++i) {
         if (c[i]) {
-            out[0] = i;
+            out[0] = i | (__popc((unsigned)id_pattern[i]) << 16);
             out[1] = c[i];
             out += 2;
         }
@@ -289,10 +291,10 @@ __global__ void wave_desc_kernel(int nslots, const int32_t *__restrict__ slots,
         const int2 e = gl[g];
         if (cs + e.y > c0) break;
         cs += e.y;
-        vs += (int64_t)e.y * __popc((unsigned)id_pattern[e.x]);
+        vs += (int64_t)e.y * (e.x >> 16);
     }
     const int64_t skip = c0 - cs;
-    const int pc = g < ng ? __popc((unsigned)id_pattern[gl[g].x]) : 0;
+    const int pc = g < ng ? (gl[g].x >> 16) : 0;
     desc[2 * i] = make_int4(p, g, (int)skip, (int)c0);
     desc[2 * i + 1] = make_int4((int)(vs + skip * pc), (int)c1, kp, j);
 }
@@ -499,10 +501,10 @@ hipError_t launch_group_count(int P, int NK, const int32_t *counts, int64_t *ngr
 }
 
 hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t *grp_ptr,
-                             int32_t *groups, hipStream_t s) {
+                             const int32_t *id_pattern, int32_t *groups, hipStream_t s) {
     if (P <= 0) return hipSuccess;
     hipLaunchKernelGGL(group_fill_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, NK, counts,
-                       grp_ptr, groups);
+                       grp_ptr, id_pattern, groups);
     return hipGetLastError();
 }
 

@@ -26,7 +26,7 @@ int fail(int code, const std::string &msg);
 // ------------------------------------------------------ generated executors
 // Arguments of one executor launch over a packed matrix.
 struct ExecLaunch {
-    const int32_t *groups;   // (id, count) pairs of non-empty groups, id order
+    const int32_t *groups;   // (id | pc << 16, count) pairs of non-empty groups, id order
     const int64_t *grp_ptr;  // [P + 1] range of panel p in groups (in pairs)
     const int32_t *desc;     // [n_wblocks * waves_per_block] pairs of int4 wave descriptors
     int n_wblocks;           // workgroups per N tile
@@ -39,7 +39,7 @@ struct ExecLaunch {
     uint32_t b_bytes;  // bytes of B addressable through the buffer resource
     float *C;
     int64_t ldb, ldc;
-    int M, N, num_panels;
+    int M, N, K, num_panels;
     int load_c;
     int vec_width;        // V
     int waves_per_block;  // panels per workgroup
@@ -49,15 +49,19 @@ struct ExecLaunch {
 
 typedef hipError_t (*ExecLaunchFn)(const ExecLaunch &, hipStream_t);
 
-// One generated executor (one pattern set).  launch[v][u] for V = 1,2,4 and
+// One generated executor (one pattern set).  launch[s][v][u]: s = 0 reads B
+// rows from global memory, s = 1 stages the B tile in LDS; V = 1,2,4 and
 // unroll = 1,2,4,8 (index log2); a null entry is a variant not generated.
 struct ExecutorDesc {
     const char *id;
     int M_r;
     int num_patterns;
     const int32_t *patterns;  // id order
-    ExecLaunchFn launch[3][4];
+    ExecLaunchFn launch[2][3][4];
 };
+
+// Largest dynamic LDS allocation of one workgroup on the current device.
+size_t lds_limit();
 
 void register_executor(const ExecutorDesc *d);
 const std::vector<const ExecutorDesc *> &executors();
@@ -80,6 +84,9 @@ struct DeviceLut {
 // last panel's data (at most 2 * 8 indices and 8 * 16 + 32 values).
 constexpr int SRT_COL_PAD = 32;
 constexpr int SRT_VAL_PAD = 256;
+// Zero (id, count) pairs after the last group: the executor reads the group
+// list two entries ahead.
+constexpr int SRT_GROUP_PAD = 4;
 
 // Packed matrix in device memory (flat arrays over all panels).
 struct Packed {
@@ -89,7 +96,7 @@ struct Packed {
     int64_t *val_ptr = nullptr;  // [P + 1]
     int32_t *cols = nullptr;
     float *vals = nullptr;
-    int32_t *groups = nullptr;   // (id, count) pairs of non-empty ids per panel
+    int32_t *groups = nullptr;   // (id | popcount << 16, count) of non-empty ids per panel
     int64_t *grp_ptr = nullptr;  // [P + 1]
     int32_t *id_pattern = nullptr;  // [NK] kernel pattern of each id
     std::vector<int64_t> h_col_ptr;  // host copy of col_ptr (scheduling)
@@ -124,11 +131,11 @@ hipError_t launch_fill_codes(int P, int K, int col_offset, const DeviceLut &lut,
                              float *vals, int64_t *padding, hipStream_t s);
 hipError_t launch_exclusive_scan(int n, const int64_t *in, int64_t *out, hipStream_t s);
 // Non-empty group lists from counts: ngroups[p], then (after a scan into
-// grp_ptr) the (id, count) pairs.
+// grp_ptr) the (id | popcount(pattern) << 16, count) pairs.
 hipError_t launch_group_count(int P, int NK, const int32_t *counts, int64_t *ngroups,
                               hipStream_t s);
 hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t *grp_ptr,
-                             int32_t *groups, hipStream_t s);
+                             const int32_t *id_pattern, int32_t *groups, hipStream_t s);
 // Wave descriptors from (panel, j, kp) triples (slots[3 * i]; panel -1 = idle).
 hipError_t launch_wave_desc(int nslots, const int32_t *slots, const int32_t *groups,
                             const int64_t *grp_ptr, const int64_t *col_ptr,

@@ -2,7 +2,7 @@
 
 The reference JIT-compiles one torch extension per pattern set
 (sbench/SOP/sop_driver.py:100-127).  Here the generator writes the CDNA4
-translation unit, hipcc builds it into ``_jit/libsrt_exec_<id>.so`` (linked
+translation units, hipcc builds them into ``_jit/libsrt_exec_<id>.so`` (linked
 against libsrt.so), and loading that library registers the executor with
 libsrt.  Built libraries are reused across processes.
 """
@@ -13,25 +13,28 @@ import os
 import subprocess
 
 from . import _lib
-from .codegen import executor_id, gen_executor_source
+from .codegen import executor_id, gen_executor_sources
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 JIT_DIR = os.path.join(PKG_DIR, "_jit")
 _loaded = {}
 
 
-def _compile(eid: str, source: str) -> str:
+def _compile(eid: str, sources) -> str:
     from importlib import util
     spec = util.spec_from_file_location("_srt_build_lib", os.path.join(PKG_DIR, "build_lib.py"))
     bl = util.module_from_spec(spec)
     spec.loader.exec_module(bl)
     os.makedirs(JIT_DIR, exist_ok=True)
-    src = os.path.join(JIT_DIR, "exec_%s.hip" % eid)
+    srcs = []
+    for name, text in sorted(sources.items()):
+        path = os.path.join(JIT_DIR, name)
+        with open(path, "w") as f:
+            f.write(text)
+        srcs.append(path)
     out = os.path.join(JIT_DIR, "libsrt_exec_%s.so" % eid)
-    with open(src, "w") as f:
-        f.write(source)
-    cmd = [bl.HIPCC] + bl.CFLAGS + ["-shared", src, "-o", out + ".tmp", "-L" + PKG_DIR, "-lsrt",
-                                   "-Wl,-rpath," + PKG_DIR]
+    cmd = [bl.HIPCC] + bl.CFLAGS + ["-shared"] + srcs + ["-o", out + ".tmp", "-L" + PKG_DIR, "-lsrt",
+                                                        "-Wl,-rpath," + PKG_DIR]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("executor build failed:\n%s\n%s" % (r.stdout, r.stderr))
@@ -50,7 +53,7 @@ def ensure_executor(native, regen: bool = False) -> str:
     eid = executor_id(native.M_r, native.patterns)
     path = os.path.join(JIT_DIR, "libsrt_exec_%s.so" % eid)
     if regen or not os.path.exists(path):
-        path = _compile(eid, gen_executor_source(native.M_r, native.patterns))
+        path = _compile(eid, gen_executor_sources(native.M_r, native.patterns))
     if path not in _loaded:
         _lib.lib()
         _loaded[path] = ct.CDLL(path)

@@ -38,8 +38,10 @@ class TileConfig:
     """sop::TileConfig fields set at SpMM_SOP.h:107-114 (defaults of SOPConfig,
     :33-54) plus the GPU realisation (0 / -1 = let the library choose):
     vec_width (fp32 columns per lane), waves_per_block (panels per
-    workgroup), xcd_remap, unroll (packed columns in flight per wave) and
-    nsplit (waves sharing one panel; 1 keeps the reference's single chain)."""
+    workgroup), xcd_remap, unroll (packed columns in flight per wave),
+    nsplit (waves sharing one panel; 1 keeps the reference's single chain),
+    chunk_cols (target packed columns per wave) and b_staging (1: B tile
+    staged in LDS, 2: B rows read from global memory)."""
     M_c: int = 16
     K_c: int = 256
     N_c: int = 64
@@ -55,6 +57,7 @@ class TileConfig:
     unroll: int = 0
     nsplit: int = 0
     chunk_cols: int = 0
+    b_staging: int = 0
 
     def to_c(self) -> _lib.TileConfigC:
         c = _lib.TileConfigC()

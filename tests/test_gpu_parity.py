@@ -116,20 +116,44 @@ def test_pruned_ffn_cases(srt, oracle, torch_cuda, case, M_r, name):
 @pytest.mark.parametrize("waves,nsplit,chunk,xcd", [(1, 1, 0, 0), (4, 1, 0, 1), (8, 8, 16, 1),
                                                     (16, 16, 8, 1), (8, 4, 32, 0), (16, 2, 0, 1),
                                                     (2, 2, 0, 1)])
-def test_launch_variants_bitexact(srt, oracle, torch_cuda, V, U, waves, nsplit, chunk, xcd):
+@pytest.mark.parametrize("staging", [1, 2])
+def test_launch_variants_bitexact(srt, oracle, torch_cuda, V, U, waves, nsplit, chunk, xcd, staging):
     so, cb = oracle
     torch = torch_cuda
-    _log("variant V=%d U=%d waves=%d nsplit=%d chunk=%d xcd=%d" % (V, U, waves, nsplit, chunk, xcd))
+    _log("variant V=%d U=%d waves=%d nsplit=%d chunk=%d xcd=%d staging=%d"
+         % (V, U, waves, nsplit, chunk, xcd, staging))
     A, B = srt.workloads.case_inputs(srt.workloads.SpMMCase(520, 300, 0.85, 264), seed=3)
     plan = _plan(srt, torch, A, 264, 8, "orig", vec_width=V, unroll=U, waves_per_block=waves,
-                 xcd_remap=xcd, nsplit=nsplit, chunk_cols=chunk)
+                 xcd_remap=xcd, nsplit=nsplit, chunk_cols=chunk, b_staging=staging)
     cfg = plan.get_config()
+    assert cfg.b_staging == staging
     assert cfg.unroll == U and cfg.nsplit <= cfg.waves_per_block
     assert cfg.N_c == 64 * cfg.vec_width and cfg.M_c == 8 * cfg.waves_per_block
     kp = plan.split_counts()
     assert kp.min() >= 1 and kp.max() <= cfg.nsplit
     pk = cb.pack_dense(A, 8, _oracle_table(so, 8, "orig"))
     _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=kp))
+
+
+@pytest.mark.parametrize("K,V,waves,nsplit", [(640, 1, 16, 16), (320, 2, 16, 4), (160, 4, 8, 8),
+                                               (641, 1, 8, 2), (64, 4, 16, 16)])
+def test_lds_tile_limits(srt, oracle, torch_cuda, K, V, waves, nsplit):
+    """B tiles up to the full 160 KiB of LDS a workgroup can hold (and one row
+    past it, which falls back to global-memory B rows); the chunk reduction
+    reuses the tile's LDS."""
+    so, cb = oracle
+    torch = torch_cuda
+    _log("lds limit K=%d V=%d waves=%d nsplit=%d" % (K, V, waves, nsplit))
+    A, B = srt.workloads.case_inputs(srt.workloads.SpMMCase(1030, K, 0.9, 64 * V * 3 - 4 * V), seed=K)
+    N = B.shape[1]
+    plan = _plan(srt, torch, A, N, 8, "orig", vec_width=V, waves_per_block=waves, nsplit=nsplit,
+                 chunk_cols=8, b_staging=1)
+    cfg = plan.get_config()
+    _log("realised b_staging=%d V=%d waves=%d" % (cfg.b_staging, cfg.vec_width, cfg.waves_per_block))
+    if K * 64 * 4 > 160 * 1024:
+        assert cfg.b_staging == 2
+    pk = cb.pack_dense(A, 8, _oracle_table(so, 8, "orig"))
+    _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=plan.split_counts()))
 
 
 @pytest.mark.parametrize("nsplit,chunk", [(1, 0), (2, 64), (4, 32), (8, 16), (16, 8)])

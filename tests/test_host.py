@@ -67,23 +67,26 @@ def test_codegen_sources_are_current(srt):
     import importlib
     codegen = importlib.import_module("sparse-register-tiling_amd.codegen")
     gen_dir = os.path.join(ROOT, "sparse-register-tiling_amd", "csrc", "generated")
-    files = sorted(glob.glob(os.path.join(gen_dir, "exec_*.hip")))
+    files = sorted(f for f in glob.glob(os.path.join(gen_dir, "exec_*.hip")) if "__" not in f)
     assert files, "build() has not generated executors"
     for f in files:
         text = open(f).read()
         M_r = int(re.search(r"M_r = (\d+)", text).group(1))
         pats = [int(x) for x in re.search(r"k_patterns\[\] = \{([^}]*)\}", text).group(1).split(",")]
-        assert text == codegen.gen_executor_source(M_r, pats)
+        units = codegen.gen_executor_sources(M_r, pats)
         assert os.path.basename(f) == "exec_%s.hip" % codegen.executor_id(M_r, pats)
+        for name, want in units.items():
+            with open(os.path.join(gen_dir, name)) as g:
+                assert g.read() == want, name
 
 
 def test_codegen_rejects_bad_sets():
     import importlib
     codegen = importlib.import_module("sparse-register-tiling_amd.codegen")
     with pytest.raises(ValueError):
-        codegen.gen_executor_source(4, [])
+        codegen.gen_executor_sources(4, [])
     with pytest.raises(ValueError):
-        codegen.gen_executor_source(4, [16])
+        codegen.gen_executor_sources(4, [16])
     assert codegen.canonical_patterns([15, 1, 3, 2]) == [1, 2, 3, 15]
 
 
