@@ -23,6 +23,8 @@ Prints one JSON line with the driver's fields plus:
 
 Profiling helpers (used to make the files under profiles/):
   --config FILE / --save-config FILE   fix / record the tuned configuration
+  --tune                               time the candidates instead of using
+                                       the shipped measured table
   --traffic-probe                      only the executor launches plus a copy
                                        kernel of known size (for --pmc runs)
   --summarize DIR                      turn rocprofv3 CSV output into a text
@@ -77,6 +79,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune-reps", type=int, default=40)
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune even when the shipped table has this workload")
     ap.add_argument("--traffic-probe", action="store_true")
     ap.add_argument("--summarize", metavar="DIR")
     ap.add_argument("--summary-out")
@@ -222,6 +226,14 @@ def choose_config(args, srt, csr, N, Bd):
         with open(args.config) as f:
             c = json.load(f)
         M_r, name, tile = c["M_r"], c["mapping"], c["tile"]
+    elif args.mapping == "auto" and not args.tune and autotune.TuningCache(autotune.SHIPPED).get(
+            csr.shape[0], csr.shape[1], 1.0 - csr.values().numel() / float(csr.shape[0] * csr.shape[1]), N):
+        recs = autotune.TuningCache(autotune.SHIPPED).get(
+            csr.shape[0], csr.shape[1], 1.0 - csr.values().numel() / float(csr.shape[0] * csr.shape[1]), N)
+        (M_r, name), cfg = autotune.best_config(recs)
+        tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
+                                             "chunk_cols", "xcd_remap", "b_staging")}
+        log("shipped tuning record:", recs[0])
     elif args.mapping == "auto":
         recs = autotune.tune(csr, N, reps=args.tune_reps, B=Bd)
         (M_r, name), cfg = autotune.best_config(recs)

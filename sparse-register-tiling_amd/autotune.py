@@ -104,8 +104,13 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
     return out
 
 
+# Measured choices shipped with the package (same layout as the cache).
+SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "heuristics", "mi355x.json")
+
+
 class TuningCache:
-    """JSON cache: {"MxK": {bucket: {N: [records...]}}}."""
+    """JSON cache: {"MxK": {bucket: {N: [records...]}}}.  ``path=SHIPPED``
+    reads the measured table that ships with the package."""
 
     def __init__(self, path: Optional[str] = None):
         self.path = path or os.path.join(os.path.expanduser("~"), ".cache", "srt_autotune.json")
