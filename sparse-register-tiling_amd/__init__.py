@@ -11,6 +11,7 @@ Public surface (each mirrors a reference interface):
   Acc, SOPModule, SOPTile, make_sop_module, execute_tile,
   sort_and_encode_patterns, pattern_code                      (sop)
   csr.executor, csr.spmm                                      (csr)
+  precision.PanelPlan, precision.spmm  (BF16 / FP8 on MFMA 16-row panels)
   SOPOrig, save_mapping, load_mapping, named_mapping, Mapping  (mapping)
 """
 from . import mapping, workloads
@@ -25,6 +26,6 @@ from .native import NativeMapping, executors, resolve_mapping
 from .matmul import MatMulSpecialized, SpMM_SOP, TileConfig, get_executor_id
 from .sop import (Acc, SOPModule, SOPTile, execute_tile, make_sop_module, panel_pattern_codes,
                   pattern_code, sort_and_encode_patterns)
-from . import csr
+from . import csr, precision
 
 __version__ = "0.1.0"

@@ -44,6 +44,15 @@ class TileConfigC(ct.Structure):
     ]
 
 
+class MfmaStatsC(ct.Structure):
+    _fields_ = [("M", ct.c_int), ("K", ct.c_int), ("num_panels", ct.c_int), ("k_blocks", ct.c_int),
+                ("dtype", ct.c_int), ("nnz", ct.c_int64), ("stored_blocks", ct.c_int64)]
+
+
+SRT_DT_BF16 = 1
+SRT_DT_FP8 = 2
+
+
 class PlanStatsC(ct.Structure):
     _fields_ = [
         ("M", ct.c_int), ("K", ct.c_int), ("M_r", ct.c_int), ("num_panels", ct.c_int),
@@ -91,6 +100,14 @@ SIGNATURES = {
     "srt_tile_copy_packed": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _p_i64, _p_i64]),
     "srt_tile_destroy": (None, [_vp]),
     "srt_csr_spmm": (_i, [_i, _i, _i, _i64, _vp, _vp, _i, _vp, _vp, _i64, _vp, _i64, _i, _vp]),
+    "srt_mfma_plan_create": (_i, [_i, _i, _vp, _vp, _vp, _i, _i, _vp, ct.POINTER(_vp)]),
+    "srt_mfma_plan_reserve": (_i, [_vp, _i, _i]),
+    "srt_mfma_convert_b": (_i, [_i, _i64, _i, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "srt_mfma_plan_execute": (_i, [_vp, _vp, _i64, _vp, _i64, _vp, _i, _i, _i, _vp]),
+    "srt_mfma_plan_stats_get": (_i, [_vp, ct.POINTER(MfmaStatsC)]),
+    "srt_mfma_plan_row_scales": (_i, [_vp, _vp]),
+    "srt_mfma_plan_copy_blocks": (_i, [_vp, _vp, _vp, _vp]),
+    "srt_mfma_plan_destroy": (None, [_vp]),
 }
 
 _lib = None

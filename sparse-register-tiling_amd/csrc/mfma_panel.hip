@@ -1,0 +1,430 @@
+// Precision form: BF16 and FP8-e4m3 SpMM on MFMA over 16-row panels.
+//
+// A is cut into 16-row panels and 32-column K blocks.  Every block holding a
+// nonzero is stored dense (zeros filled: the panel is padded to a dense 16-row
+// block, the extreme of the reference's ILP padding, sbench/ilp_pad/
+// nano_solver.py:76-116, where each column pattern is padded up to all rows);
+// all-zero blocks are skipped.  A stored block is laid out exactly as the
+// v_mfma_f32_16x16x32_{bf16,fp8_fp8} A operand wants it: lane l holds row
+// l % 16, columns 8 * (l / 16) .. + 8, so one lane reads one 16-byte (BF16) or
+// 8-byte (FP8) word per block.
+//
+// A workgroup owns WP panels x NT columns of C and walks the K blocks of its
+// K range: the B tile (32 rows x NT columns) is copied from row-major B into
+// LDS transposed to [column][k] (the MFMA B operand wants eight consecutive k
+// of one column per lane), then each wave runs NT / 16 MFMAs for its panel if
+// the panel stores that block.  Accumulation is FP32.  FP8: A rows carry FP32
+// scales s_i = max_k |A_ik| / 448 and B columns t_j = max_k |B_kj| / 448, both
+// computed in FP64; C_ij = s_i * t_j * sum_k q(A)_ik q(B)_kj.
+//
+// A K range split over KS workgroups writes partial tiles to a workspace that
+// a second kernel adds in split order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "srt_internal.h"
+
+namespace srt {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kPanelRows = 16;
+constexpr int kBlockK = 32;
+constexpr int kBlockElems = kPanelRows * kBlockK;  // 512
+
+// ------------------------------------------------------------ conversions
+// float -> bfloat16, round to nearest even (NaN stays a quiet NaN).
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// float -> FP8 e4m3 (OCP "fn": bias 7, no infinities, max 448), round to
+// nearest even, saturating at +-448; NaN -> 0x7f.
+__device__ __forceinline__ uint8_t to_e4m3(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    const uint32_t a = u & 0x7fffffffu;
+    if (a > 0x7f800000u) return 0x7f;
+    if (a >= 0x43e00000u) return sign | 0x7e;  // >= 448
+    const int e = (int)(a >> 23) - 127;
+    if (e < -6) {
+        // subnormal target: multiples of 2^-9; a * 512 is exact, rint is RNE
+        const float q = __builtin_rintf(__builtin_bit_cast(float, a) * 512.0f);
+        return sign | (uint8_t)q;  // q in [0, 8]; 8 is the smallest normal
+    }
+    uint32_t m = a & 0x7fffffu;
+    uint32_t r = m >> 20;
+    const uint32_t rem = m & 0xfffffu;
+    uint32_t ex = (uint32_t)(e + 7);
+    if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) {
+        if (++r == 8u) {
+            r = 0;
+            ++ex;
+        }
+    }
+    if (ex > 15u || (ex == 15u && r == 7u)) return sign | 0x7e;  // rounded past 448
+    return sign | (uint8_t)((ex << 3) | r);
+}
+
+// ------------------------------------------------------------ packing
+// One wave per CSR row: mark the K blocks holding a nonzero.
+template <typename I>
+__global__ void blk_mark_kernel(int m, int k, int KB, const I *__restrict__ Lp,
+                                const I *__restrict__ Li, uint8_t *__restrict__ present,
+                                int *__restrict__ bad) {
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= m) return;
+    const int lane = threadIdx.x & 63;
+    const int p = row / kPanelRows;
+    for (int64_t q = (int64_t)Lp[row] + lane; q < (int64_t)Lp[row + 1]; q += 64) {
+        const int64_t c = (int64_t)Li[q];
+        if (c < 0 || c >= k) {
+            atomicOr(bad, 1);
+            continue;
+        }
+        present[(int64_t)p * KB + c / kBlockK] = 1;
+    }
+}
+
+// One thread per panel: number of stored blocks.
+__global__ void blk_count_kernel(int P, int KB, const uint8_t *__restrict__ present,
+                                 int64_t *__restrict__ nblk) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int64_t n = 0;
+    for (int b = 0; b < KB; ++b) n += present[(int64_t)p * KB + b];
+    nblk[p] = n;
+}
+
+// One thread per panel: slot of every stored block (K order), -1 elsewhere.
+__global__ void blk_slot_kernel(int P, int KB, const uint8_t *__restrict__ present,
+                                const int64_t *__restrict__ blk_ptr, int32_t *__restrict__ slot,
+                                int32_t *__restrict__ blk_kb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int64_t s = blk_ptr[p];
+    for (int b = 0; b < KB; ++b) {
+        if (present[(int64_t)p * KB + b]) {
+            slot[(int64_t)p * KB + b] = (int32_t)s;
+            blk_kb[s] = b;
+            ++s;
+        } else {
+            slot[(int64_t)p * KB + b] = -1;
+        }
+    }
+}
+
+// One wave per row: FP8 row scale s = max|A_i| / 448 computed in FP64.
+template <typename I>
+__global__ void row_scale_kernel(int m, const I *__restrict__ Lp, const float *__restrict__ Lx,
+                                 float *__restrict__ scale) {
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= m) return;
+    const int lane = threadIdx.x & 63;
+    double mx = 0.0;
+    for (int64_t q = (int64_t)Lp[row] + lane; q < (int64_t)Lp[row + 1]; q += 64)
+        mx = fmax(mx, fabs((double)Lx[q]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    if (lane == 0) scale[row] = mx > 0.0 ? (float)(mx / 448.0) : 1.0f;
+}
+
+__device__ __forceinline__ int64_t operand_offset(int64_t slot, int r, int kk) {
+    return slot * kBlockElems + (int64_t)(((kk >> 3) * kPanelRows + r) * 8 + (kk & 7));
+}
+
+// One wave per row: every nonzero into its block, converted.
+template <typename I, bool FP8>
+__global__ void blk_fill_kernel(int m, int k, int KB, const I *__restrict__ Lp,
+                                const I *__restrict__ Li, const float *__restrict__ Lx,
+                                const int32_t *__restrict__ slot, const float *__restrict__ rscale,
+                                void *__restrict__ blocks) {
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= m) return;
+    const int lane = threadIdx.x & 63;
+    const int p = row / kPanelRows, r = row % kPanelRows;
+    const double s = FP8 ? (double)rscale[row] : 1.0;
+    for (int64_t q = (int64_t)Lp[row] + lane; q < (int64_t)Lp[row + 1]; q += 64) {
+        const int64_t c = (int64_t)Li[q];
+        if (c < 0 || c >= k) continue;
+        const int32_t sl = slot[(int64_t)p * KB + c / kBlockK];
+        const int64_t off = operand_offset(sl, r, (int)(c % kBlockK));
+        if constexpr (FP8) {
+            static_cast<uint8_t *>(blocks)[off] = to_e4m3((float)((double)Lx[q] / s));
+        } else {
+            static_cast<uint16_t *>(blocks)[off] = to_bf16(Lx[q]);
+        }
+    }
+}
+
+// ------------------------------------------------------------ B operand prep
+// One thread per element: fp32 B -> bf16 (row-major, ld_out).
+__global__ void b_to_bf16_kernel(int64_t rows, int cols, const float *__restrict__ src, int64_t ld_src,
+                                 uint16_t *__restrict__ dst, int64_t ld_dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    const int64_t r = i / cols, c = i % cols;
+    dst[r * ld_dst + c] = to_bf16(src[r * ld_src + c]);
+}
+
+// One block of 256 threads per 256 columns: FP8 column scales in FP64.
+__global__ void col_scale_kernel(int64_t rows, int cols, const float *__restrict__ src, int64_t ld,
+                                 float *__restrict__ scale) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    double mx = 0.0;
+    for (int64_t r = 0; r < rows; ++r) mx = fmax(mx, fabs((double)src[r * ld + c]));
+    scale[c] = mx > 0.0 ? (float)(mx / 448.0) : 1.0f;
+}
+
+__global__ void b_to_e4m3_kernel(int64_t rows, int cols, const float *__restrict__ src, int64_t ld_src,
+                                 const float *__restrict__ scale, uint8_t *__restrict__ dst,
+                                 int64_t ld_dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    const int64_t r = i / cols, c = i % cols;
+    dst[r * ld_dst + c] = to_e4m3((float)((double)src[r * ld_src + c] / (double)scale[c]));
+}
+
+// ------------------------------------------------------------ executor
+template <bool FP8>
+struct Elt {
+    typedef uint16_t T;
+    static constexpr int kPerChunk = 8;  // elements per 16-byte chunk
+};
+template <>
+struct Elt<true> {
+    typedef uint8_t T;
+    static constexpr int kPerChunk = 16;
+};
+
+// LDS tile [NT][kLdsK] of B transposed; 8 elements of padding per column row
+// keep the 16-byte operand reads aligned and spread the transposed writes.
+constexpr int kLdsK = kBlockK + 8;
+
+template <bool FP8, int NT, int WP>
+__global__ void __launch_bounds__(64 * WP)
+    mfma_exec_kernel(int M, int N, int K, int KB, int P, const typename Elt<FP8>::T *__restrict__ B,
+                     int64_t ldb, const void *__restrict__ blocks, const int32_t *__restrict__ slot,
+                     const float *__restrict__ rscale, const float *__restrict__ cscale,
+                     float *__restrict__ C, int64_t ldc, int ks, float *__restrict__ work, int vec_ok) {
+    typedef typename Elt<FP8>::T T;
+    constexpr int PER = Elt<FP8>::kPerChunk;
+    constexpr int CHUNKS = kBlockK * NT / PER;  // 16-byte chunks of one B tile
+    constexpr int NTHR = 64 * WP;
+    constexpr int CPT = (CHUNKS + NTHR - 1) / NTHR;
+    constexpr int TN = NT / 16;
+    __shared__ __attribute__((aligned(16))) T tile[NT * kLdsK];
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int pg = blockIdx.x, nt = blockIdx.y, z = blockIdx.z;
+    const int p = pg * WP + wave;
+    const int n0 = nt * NT;
+    const int kb0 = (int)((int64_t)z * KB / ks), kb1 = (int)((int64_t)(z + 1) * KB / ks);
+
+    f32x4 acc[TN];
+#pragma unroll
+    for (int t = 0; t < TN; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int kb = kb0; kb < kb1; ++kb) {
+        // B tile rows kb*32 .. +32, columns n0 .. n0+NT -> tile[col][k]
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const int ch = threadIdx.x + c * NTHR;
+            if (ch < CHUNKS) {
+                const int kk = ch / (NT / PER), cn = (ch % (NT / PER)) * PER;
+                const int64_t kr = (int64_t)kb * kBlockK + kk;
+                T v[PER];
+                if (vec_ok && kr < K && n0 + cn + PER <= N) {
+                    const uint4 w = *reinterpret_cast<const uint4 *>(B + kr * ldb + n0 + cn);
+                    __builtin_memcpy(v, &w, 16);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < PER; ++e)
+                        v[e] = (kr < K && n0 + cn + e < N) ? B[kr * ldb + n0 + cn + e] : (T)0;
+                }
+#pragma unroll
+                for (int e = 0; e < PER; ++e) tile[(cn + e) * kLdsK + kk] = v[e];
+            }
+        }
+        __syncthreads();
+        if (p < P) {
+            const int32_t sl = slot[(int64_t)p * KB + kb];
+            if (sl >= 0) {
+                const int col = lane & 15, kg = lane >> 4;
+                if constexpr (FP8) {
+                    const long a = reinterpret_cast<const long *>(blocks)[(int64_t)sl * 64 + lane];
+#pragma unroll
+                    for (int t = 0; t < TN; ++t) {
+                        const long b = *reinterpret_cast<const long *>(&tile[(t * 16 + col) * kLdsK + kg * 8]);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc[t], 0, 0, 0);
+                    }
+                } else {
+                    const uint4 aw = reinterpret_cast<const uint4 *>(blocks)[(int64_t)sl * 64 + lane];
+                    const bf16x8 a = __builtin_bit_cast(bf16x8, aw);
+#pragma unroll
+                    for (int t = 0; t < TN; ++t) {
+                        const uint4 bw = *reinterpret_cast<const uint4 *>(&tile[(t * 16 + col) * kLdsK + kg * 8]);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, bw),
+                                                                         acc[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (p >= P) return;
+    // D layout: lane l holds rows 4 * (l / 16) .. + 4 of column l % 16.
+    const int col = lane & 15, rg = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < TN; ++t) {
+        const int n = n0 + t * 16 + col;
+        if (n >= N) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = p * kPanelRows + rg * 4 + i;
+            if (row >= M) continue;
+            float v = acc[t][i];
+            if (ks > 1) {
+                work[((int64_t)z * M + row) * N + n] = v;
+            } else {
+                if constexpr (FP8) v = v * rscale[row] * cscale[n];
+                C[(int64_t)row * ldc + n] = v;
+            }
+        }
+    }
+}
+
+// Adds the ks partial tiles in split order (and applies the FP8 scales).
+template <bool FP8>
+__global__ void split_reduce_kernel(int M, int N, int ks, const float *__restrict__ work,
+                                    const float *__restrict__ rscale, const float *__restrict__ cscale,
+                                    float *__restrict__ C, int64_t ldc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)M * N) return;
+    const int64_t row = i / N, n = i % N;
+    float v = work[i];
+    for (int z = 1; z < ks; ++z) v = v + work[(int64_t)z * M * N + i];
+    if constexpr (FP8) v = v * rscale[row] * cscale[n];
+    C[row * ldc + n] = v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_mfma_pack(int m, int k, const void *Lp, const void *Li, int index_bytes,
+                            const float *Lx, bool fp8, int P, int KB, uint8_t *present,
+                            int64_t *nblk, int *bad, hipStream_t s) {
+    const dim3 grid((m + 3) / 4), block(256);
+    if (m <= 0) return hipSuccess;
+    if (index_bytes == 4)
+        hipLaunchKernelGGL(blk_mark_kernel<int32_t>, grid, block, 0, s, m, k, KB,
+                           (const int32_t *)Lp, (const int32_t *)Li, present, bad);
+    else
+        hipLaunchKernelGGL(blk_mark_kernel<int64_t>, grid, block, 0, s, m, k, KB,
+                           (const int64_t *)Lp, (const int64_t *)Li, present, bad);
+    hipLaunchKernelGGL(blk_count_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, KB, present, nblk);
+    (void)Lx;
+    (void)fp8;
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_slots(int P, int KB, const uint8_t *present, const int64_t *blk_ptr,
+                             int32_t *slot, int32_t *blk_kb, hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    hipLaunchKernelGGL(blk_slot_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, KB, present, blk_ptr,
+                       slot, blk_kb);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_fill(int m, int k, int KB, const void *Lp, const void *Li, int index_bytes,
+                            const float *Lx, bool fp8, const int32_t *slot, float *rscale,
+                            void *blocks, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    const dim3 grid((m + 3) / 4), block(256);
+    if (index_bytes == 4) {
+        if (fp8) {
+            hipLaunchKernelGGL(row_scale_kernel<int32_t>, grid, block, 0, s, m, (const int32_t *)Lp, Lx, rscale);
+            hipLaunchKernelGGL((blk_fill_kernel<int32_t, true>), grid, block, 0, s, m, k, KB,
+                               (const int32_t *)Lp, (const int32_t *)Li, Lx, slot, rscale, blocks);
+        } else {
+            hipLaunchKernelGGL((blk_fill_kernel<int32_t, false>), grid, block, 0, s, m, k, KB,
+                               (const int32_t *)Lp, (const int32_t *)Li, Lx, slot, rscale, blocks);
+        }
+    } else {
+        if (fp8) {
+            hipLaunchKernelGGL(row_scale_kernel<int64_t>, grid, block, 0, s, m, (const int64_t *)Lp, Lx, rscale);
+            hipLaunchKernelGGL((blk_fill_kernel<int64_t, true>), grid, block, 0, s, m, k, KB,
+                               (const int64_t *)Lp, (const int64_t *)Li, Lx, slot, rscale, blocks);
+        } else {
+            hipLaunchKernelGGL((blk_fill_kernel<int64_t, false>), grid, block, 0, s, m, k, KB,
+                               (const int64_t *)Lp, (const int64_t *)Li, Lx, slot, rscale, blocks);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_b_convert(int64_t rows, int cols, const float *src, int64_t ld_src, bool fp8,
+                            float *cscale, void *dst, int64_t ld_dst, hipStream_t s) {
+    const int64_t n = rows * cols;
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    if (fp8) {
+        hipLaunchKernelGGL(col_scale_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, rows, cols, src, ld_src,
+                           cscale);
+        hipLaunchKernelGGL(b_to_e4m3_kernel, dim3(grid), dim3(256), 0, s, rows, cols, src, ld_src, cscale,
+                           (uint8_t *)dst, ld_dst);
+    } else {
+        hipLaunchKernelGGL(b_to_bf16_kernel, dim3(grid), dim3(256), 0, s, rows, cols, src, ld_src,
+                           (uint16_t *)dst, ld_dst);
+    }
+    return hipGetLastError();
+}
+
+template <bool FP8, int NT, int WP>
+static hipError_t launch_exec_t(const MfmaLaunch &a, hipStream_t s) {
+    const dim3 grid((a.P + WP - 1) / WP, (a.N + NT - 1) / NT, a.ks);
+    hipLaunchKernelGGL((mfma_exec_kernel<FP8, NT, WP>), grid, dim3(64 * WP), 0, s, a.M, a.N, a.K, a.KB, a.P,
+                       static_cast<const typename Elt<FP8>::T *>(a.B), a.ldb, a.blocks, a.slot, a.rscale,
+                       a.cscale, a.C, a.ldc, a.ks, a.work,
+                       ((uintptr_t)a.B % 16 == 0 && (a.ldb * (int64_t)sizeof(typename Elt<FP8>::T)) % 16 == 0) ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.ks <= 1) return e;
+    const int64_t n = (int64_t)a.M * a.N;
+    hipLaunchKernelGGL((split_reduce_kernel<FP8>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.M,
+                       a.N, a.ks, a.work, a.rscale, a.cscale, a.C, a.ldc);
+    return hipGetLastError();
+}
+
+template <bool FP8>
+static hipError_t launch_exec_dt(const MfmaLaunch &a, hipStream_t s) {
+    switch (a.shape) {
+        case 0: return launch_exec_t<FP8, 64, 4>(a, s);
+        case 1: return launch_exec_t<FP8, 128, 4>(a, s);
+        case 2: return launch_exec_t<FP8, 64, 8>(a, s);
+        case 3: return launch_exec_t<FP8, 128, 2>(a, s);
+        case 4: return launch_exec_t<FP8, 32, 4>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_mfma_exec(const MfmaLaunch &a, hipStream_t s) {
+    if (a.M <= 0 || a.N <= 0) return hipSuccess;
+    return a.fp8 ? launch_exec_dt<true>(a, s) : launch_exec_dt<false>(a, s);
+}
+
+int mfma_shape_tile(int shape, int *nt, int *wp) {
+    static const int t[5][2] = {{64, 4}, {128, 4}, {64, 8}, {128, 2}, {32, 4}};
+    if (shape < 0 || shape >= 5) return -1;
+    *nt = t[shape][0];
+    *wp = t[shape][1];
+    return 0;
+}
+
+}  // namespace srt

@@ -156,6 +156,35 @@ hipError_t launch_fill_entries(int n, int col_offset, int K, const DeviceLut &lu
 hipError_t launch_stable_order(int n, int NK, const int32_t *ids, const int32_t *counts,
                                int32_t *encoded_sorted, int32_t *order, hipStream_t s);
 
+// Precision form on MFMA (mfma_panel.hip): 16-row panels, 32-column blocks.
+struct MfmaLaunch {
+    bool fp8;
+    int shape;  // (N tile, panels per workgroup) index, see mfma_shape_tile
+    int M, N, K, KB, P;
+    const void *B;  // bf16 or e4m3, row-major, ldb elements
+    int64_t ldb;
+    const void *blocks;     // stored blocks in MFMA A-operand layout
+    const int32_t *slot;    // [P][KB] block slot or -1
+    const float *rscale;    // FP8 row scales (M)
+    const float *cscale;    // FP8 column scales of B (N)
+    float *C;
+    int64_t ldc;
+    int ks;       // K split
+    float *work;  // ks * M * N partial tiles when ks > 1
+};
+hipError_t launch_mfma_pack(int m, int k, const void *Lp, const void *Li, int index_bytes,
+                            const float *Lx, bool fp8, int P, int KB, uint8_t *present,
+                            int64_t *nblk, int *bad, hipStream_t s);
+hipError_t launch_mfma_slots(int P, int KB, const uint8_t *present, const int64_t *blk_ptr,
+                             int32_t *slot, int32_t *blk_kb, hipStream_t s);
+hipError_t launch_mfma_fill(int m, int k, int KB, const void *Lp, const void *Li, int index_bytes,
+                            const float *Lx, bool fp8, const int32_t *slot, float *rscale,
+                            void *blocks, hipStream_t s);
+hipError_t launch_b_convert(int64_t rows, int cols, const float *src, int64_t ld_src, bool fp8,
+                            float *cscale, void *dst, int64_t ld_dst, hipStream_t s);
+hipError_t launch_mfma_exec(const MfmaLaunch &a, hipStream_t s);
+int mfma_shape_tile(int shape, int *nt, int *wp);
+
 // CSR baseline (csr_spmm.hip).
 hipError_t launch_csr(int m, int k, int n, int64_t nnz, const void *rowptr, const void *colind,
                       int index_bytes, const float *vals, const float *B, int64_t ldb, float *C,

@@ -1,0 +1,137 @@
+"""GPU parity of the precision form (BF16 / FP8-e4m3 on MFMA 16-row panels).
+
+Bar: the packed blocks, the FP8 row scales and the converted B operand are
+bit-identical to the oracle's; C is within the FP32-accumulation bound of the
+FP64 product of the quantised operands (oracle.precision_oracle.reference_f64)
+for every launch shape and K split; the normwise error against the FP64
+product of the unquantised inputs is reported and bounded (BF16 <= 1e-2).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import precision_oracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    torch.cuda.init()
+    return torch
+
+
+def _sparse(M, K, density, seed):
+    rng = np.random.default_rng(seed)
+    return ((rng.random((M, K)) < density) * rng.standard_normal((M, K))).astype(np.float32)
+
+
+def _run(srt, torch, A, B, dtype, shape=None, ksplit=None, check_pack=True):
+    plan = srt.precision.PanelPlan.from_dense(torch.from_numpy(A), dtype)
+    if check_pack:
+        slot, kb, blocks = po.pack_blocks(A, dtype)
+        got = plan.blocks()
+        assert np.array_equal(got["slot"], slot)
+        assert np.array_equal(got["blk_kb"], kb)
+        assert np.array_equal(got["blocks"], blocks)
+        if dtype == "fp8":
+            assert np.array_equal(plan.row_scales(), po.row_scales(A))
+    Bd = torch.from_numpy(B).cuda()
+    Bq, t = plan.prepare_b(Bd)
+    if check_pack:
+        qb, tref = po.quantize_b(B, dtype)
+        gotb = Bq.cpu().view(torch.int16).numpy().view(np.uint16) if dtype == "bf16" else Bq.cpu().numpy()
+        assert np.array_equal(gotb, qb)
+        if dtype == "fp8":
+            assert np.array_equal(t.cpu().numpy(), tref)
+    C = torch.full((A.shape[0], B.shape[1]), float("nan"), device="cuda")
+    plan(C, Bq, t, shape=shape, ksplit=ksplit)
+    torch.cuda.synchronize()
+    return plan, C.cpu().numpy()
+
+
+def _within(C, A, B, dtype):
+    ref, bound = po.reference_f64(A, B, dtype)
+    ok = np.abs(C.astype(np.float64) - ref) <= bound + 1e-30
+    assert ok.all(), "max excess %g" % (np.abs(C - ref) - bound).max()
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("shape_mkn", [(16, 32, 16), (37, 70, 24), (1000, 300, 100), (130, 257, 3), (1, 1, 1)])
+def test_small_shapes(srt, torch_cuda, dtype, shape_mkn):
+    M, K, N = shape_mkn
+    _log("precision", dtype, shape_mkn)
+    A = _sparse(M, K, 0.3, M + K)
+    B = np.random.default_rng(N).uniform(-1, 1, (K, N)).astype(np.float32)
+    _, C = _run(srt, torch_cuda, A, B, dtype)
+    _within(C, A, B, dtype)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("ksplit", [1, 2, 5])
+def test_launch_shapes_and_splits(srt, torch_cuda, dtype, shape, ksplit):
+    _log("precision shape", dtype, shape, ksplit)
+    A = _sparse(200, 600, 0.4, 7)
+    A[32:48, :] = 0.0            # an empty panel
+    A[:, 64:128] = 0.0           # empty blocks in every panel
+    B = np.random.default_rng(8).uniform(-1, 1, (600, 136)).astype(np.float32)
+    _, C = _run(srt, torch_cuda, A, B, dtype, shape=shape, ksplit=ksplit, check_pack=(shape == 0 and ksplit == 1))
+    _within(C, A, B, dtype)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+def test_unaligned_b_and_empty(srt, torch_cuda, dtype):
+    torch = torch_cuda
+    _log("precision unaligned", dtype)
+    A = _sparse(64, 96, 0.5, 9)
+    B = np.random.default_rng(10).uniform(-1, 1, (96, 50)).astype(np.float32)
+    plan = srt.precision.PanelPlan.from_dense(torch.from_numpy(A), dtype)
+    Bq, t = plan.prepare_b(torch.from_numpy(B).cuda())
+    big = torch.zeros((96, 53), dtype=Bq.dtype, device="cuda")
+    view = big[:, 1:51]           # 2- or 1-byte offset: not 16-byte aligned, odd row stride
+    view.copy_(Bq)
+    C = torch.empty((64, 50), device="cuda")
+    plan(C, view, t)
+    torch.cuda.synchronize()
+    _within(C.cpu().numpy(), A, B, dtype)
+    Z = np.zeros((40, 30), dtype=np.float32)
+    pz = srt.precision.PanelPlan.from_dense(torch.from_numpy(Z), dtype)
+    assert pz.stats()["stored_blocks"] == 0
+    Bz, tz = pz.prepare_b(torch.ones((30, 8), device="cuda"))
+    Cz = torch.full((40, 8), 5.0, device="cuda")
+    pz(Cz, Bz, tz)
+    torch.cuda.synchronize()
+    assert (Cz.cpu().numpy() == 0).all()
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("mk,s", [((4096, 1024), 0.5), ((1024, 4096), 0.7)])
+def test_precision_workloads_error_vs_fp64(srt, torch_cuda, dtype, mk, s):
+    """The precision configurations (magnitude-pruned, N = 512)."""
+    M, K = mk
+    _log("precision workload", dtype, mk, s)
+    case = srt.workloads.SpMMCase(M, K, s, 512)
+    A, B = srt.workloads.case_inputs(case, seed=0)
+    _, C = _run(srt, torch_cuda, A, B, dtype, check_pack=False)
+    _within(C, A, B, dtype)
+    err = po.rel_error_vs_fp64(C, A, B)
+    _log("  rel error vs FP64 (unquantised inputs): %.3e" % err)
+    assert err <= (1e-2 if dtype == "bf16" else 1e-1)
+
+
+def test_invalid_inputs(srt, torch_cuda):
+    torch = torch_cuda
+    with pytest.raises(srt.SrtError):
+        srt.precision.PanelPlan(2, 4, torch.ones(1), torch.tensor([0, 1, 1]), torch.tensor([7]), "bf16")
+    with pytest.raises(srt.SrtError):
+        srt.precision.PanelPlan(2, 4, torch.ones(2), torch.tensor([0, 2, 1]), torch.tensor([0, 1]), "bf16")
+    with pytest.raises(ValueError):
+        srt.precision.PanelPlan(2, 4, torch.ones(1), torch.tensor([0, 1, 1]), torch.tensor([0]), "fp16")
