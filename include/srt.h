@@ -226,8 +226,9 @@ void srt_tile_destroy(srt_tile *t);
  * No counterpart in the reference (its executors are FP32 AVX-512 only,
  * tools/codegen/gen_vector_specialized.py:266-349); the packing follows the
  * reference's ILP padding idea (sbench/ilp_pad/nano_solver.py:76-116) at its
- * dense extreme: 16-row panels x 32-column blocks, every block holding a
- * nonzero stored dense in MFMA operand layout, empty blocks skipped.
+ * dense extreme: 16-row panels x 32-column blocks stored dense in MFMA operand
+ * layout (block (p, kb) at slot p * KB + kb), for panels dense enough for
+ * matrix cores.
  * FP8: A rows scaled by s_i = max|A_i| / 448, B columns by t_j = max|B_:j| /
  * 448, both computed in FP64 on the device; C = diag(s) (qA qB) diag(t).
  * ---------------------------------------------------------------------- */
@@ -237,7 +238,7 @@ typedef struct srt_mfma_plan srt_mfma_plan;
 
 typedef struct srt_mfma_stats {
     int M, K, num_panels, k_blocks, dtype;
-    int64_t nnz, stored_blocks;
+    int64_t nnz, stored_blocks, nonzero_blocks;
 } srt_mfma_stats;
 
 /* Pack CSR A (m x k, device Lp/Li/Lx, 4- or 8-byte indices) into panels. */
@@ -246,18 +247,21 @@ int srt_mfma_plan_create(int m, int k, const float *Lx, const void *Lp, const vo
 /* Workspace for K-split launches: up to ksplit_max partial tiles of m x n_max.
  * Allocates; call it before capturing launches in a graph. */
 int srt_mfma_plan_reserve(srt_mfma_plan *p, int n_max, int ksplit_max);
-/* fp32 B (rows x cols, ldb) -> BF16 or e4m3 (ld_out); FP8 also writes the
- * column scales (cols floats). */
+/* fp32 B (rows = K x cols = N, row-major, ldb) -> Bt: N x ld_out in BF16 or
+ * e4m3, k contiguous (the MFMA B operand order), ld_out >= K rounded up to 32,
+ * k in [K, ld_out) zero; FP8 also writes the column scales (N floats). */
 int srt_mfma_convert_b(int dtype, int64_t rows, int cols, const float *B, int64_t ldb, void *out,
                        int64_t ld_out, float *col_scales, srt_stream_t stream);
-/* C[:m, :n] = A . B (B in the plan's dtype, ldb elements; col_scales for FP8).
- * shape selects (N tile, panels per workgroup): 0 (64,4) 1 (128,4) 2 (64,8)
- * 3 (128,2) 4 (32,4); ksplit K ranges are added in order. */
+/* C[:m, :n] = A . B with B given transposed as produced by srt_mfma_convert_b
+ * (Bt: n x ldb in the plan's dtype, ldb >= k rounded up to 32; col_scales for
+ * FP8).  shape selects (N tile, waves, panels per wave): 0 (64,4,2)
+ * 1 (128,4,2) 2 (64,4,1) 3 (128,2,2) 4 (32,4,1) 5 (128,4,1); the ksplit K
+ * ranges are added in order. */
 int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
                           const float *col_scales, int n, int shape, int ksplit, srt_stream_t stream);
 int srt_mfma_plan_stats_get(const srt_mfma_plan *p, srt_mfma_stats *out);
 int srt_mfma_plan_row_scales(const srt_mfma_plan *p, float *out);
-/* Host copies: slot[P*KB] (block slot or -1), blk_kb[stored], blocks (512 elements each). */
+/* Host copies: slot[P*KB] (block slot), blk_kb[stored], blocks (512 elements each). */
 int srt_mfma_plan_copy_blocks(const srt_mfma_plan *p, int32_t *slot, int32_t *blk_kb, void *blocks);
 void srt_mfma_plan_destroy(srt_mfma_plan *p);
 

@@ -112,8 +112,9 @@ def dequant(codes, dtype: str) -> np.ndarray:
 
 
 def pack_blocks(A: np.ndarray, dtype: str):
-    """Reference packing: (slot [P, KB], blk_kb, blocks [n, 512]) with blocks
-    in MFMA operand order; a block is stored iff it holds a nonzero of A."""
+    """Reference packing: (slot [P, KB], blk_kb, blocks [P * KB, 512]) with
+    blocks in MFMA operand order, every block stored (dense 16-row panels),
+    block (p, kb) at slot p * KB + kb; blocks without a nonzero are zero."""
     A = np.asarray(A, dtype=np.float32)
     M, K = A.shape
     P = -(-M // PANEL)
@@ -128,8 +129,6 @@ def pack_blocks(A: np.ndarray, dtype: str):
     lane = np.arange(64)
     for p in range(P):
         for b in range(KB):
-            if not nzpad[p * PANEL:(p + 1) * PANEL, b * BLOCK_K:(b + 1) * BLOCK_K].any():
-                continue
             slot[p, b] = len(blocks)
             kb_list.append(b)
             tile = pad[p * PANEL:(p + 1) * PANEL, b * BLOCK_K:(b + 1) * BLOCK_K]

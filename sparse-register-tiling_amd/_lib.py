@@ -46,7 +46,8 @@ class TileConfigC(ct.Structure):
 
 class MfmaStatsC(ct.Structure):
     _fields_ = [("M", ct.c_int), ("K", ct.c_int), ("num_panels", ct.c_int), ("k_blocks", ct.c_int),
-                ("dtype", ct.c_int), ("nnz", ct.c_int64), ("stored_blocks", ct.c_int64)]
+                ("dtype", ct.c_int), ("nnz", ct.c_int64), ("stored_blocks", ct.c_int64),
+                ("nonzero_blocks", ct.c_int64)]
 
 
 SRT_DT_BF16 = 1

@@ -35,7 +35,7 @@ struct srt_mfma_plan {
     int device = -1;
     int dtype = 0;  // SRT_DT_BF16 or SRT_DT_FP8
     int M = 0, K = 0, P = 0, KB = 0;
-    int64_t nnz = 0, nblk = 0;
+    int64_t nnz = 0, nblk = 0, nz_blocks = 0;
     uint8_t *present = nullptr;
     int32_t *slot = nullptr;
     int32_t *blk_kb = nullptr;
@@ -119,10 +119,11 @@ int srt_mfma_plan_create(int m, int k, const float *Lx, const void *Lp, const vo
         SRT_HIP_TRY(launch_exclusive_scan(p->P, nblk, blk_ptr, s));
         int hbad = 0;
         SRT_HIP_TRY(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-        SRT_HIP_TRY(hipMemcpyAsync(&p->nblk, blk_ptr + p->P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipMemcpyAsync(&p->nz_blocks, blk_ptr + p->P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         SRT_HIP_TRY(hipStreamSynchronize(s));
         if (hbad) return fail(SRT_ERR_INVALID, "column index outside [0, k)");
     }
+    p->nblk = (int64_t)p->P * p->KB;  // every block stored (dense panels)
     MF_TRY(dalloc(&p->blk_kb, (size_t)p->nblk));
     const size_t blk_bytes = (size_t)p->nblk * 512 * elt_bytes(dtype);
     SRT_HIP_TRY(hipMalloc(&p->blocks, std::max<size_t>(blk_bytes, 16)));
@@ -153,7 +154,8 @@ int srt_mfma_plan_reserve(srt_mfma_plan *p, int n_max, int ksplit_max) {
 int srt_mfma_convert_b(int dtype, int64_t rows, int cols, const float *B, int64_t ldb, void *out,
                        int64_t ld_out, float *col_scales, srt_stream_t stream) {
     if (dtype != SRT_DT_BF16 && dtype != SRT_DT_FP8) return fail(SRT_ERR_INVALID, "dtype must be BF16 or FP8");
-    if (rows < 0 || cols < 0 || ldb < cols || ld_out < cols) return fail(SRT_ERR_INVALID, "bad shape");
+    if (rows < 0 || cols < 0 || ldb < cols) return fail(SRT_ERR_INVALID, "bad shape");
+    if (ld_out < (rows + 31) / 32 * 32) return fail(SRT_ERR_INVALID, "ld_out must be >= K rounded up to 32");
     if (rows * (int64_t)cols == 0) return SRT_OK;
     if (!B || !out || (dtype == SRT_DT_FP8 && !col_scales)) return fail(SRT_ERR_INVALID, "null pointer");
     SRT_HIP_TRY(launch_b_convert(rows, cols, B, ldb, dtype == SRT_DT_FP8, col_scales, out, ld_out,
@@ -167,13 +169,14 @@ int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const v
     if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
     if (n == 0 || p->M == 0) return SRT_OK;
     if (!C || !B) return fail(SRT_ERR_INVALID, "null B or C");
-    if (ldc < n || ldb < n) return fail(SRT_ERR_INVALID, "ldb and ldc must be >= n");
+    if (ldc < n) return fail(SRT_ERR_INVALID, "ldc must be >= n");
     if (p->dtype == SRT_DT_FP8 && !col_scales) return fail(SRT_ERR_INVALID, "FP8 needs the column scales of B");
     int dev = -1;
     SRT_HIP_TRY(hipGetDevice(&dev));
     if (dev != p->device) return fail(SRT_ERR_INVALID, "plan belongs to another device");
-    int nt = 0, wp = 0;
-    if (mfma_shape_tile(shape, &nt, &wp)) return fail(SRT_ERR_INVALID, "unknown launch shape");
+    int nt = 0, wp = 0, pw = 0;
+    if (mfma_shape_tile(shape, &nt, &wp, &pw)) return fail(SRT_ERR_INVALID, "unknown launch shape");
+    if (ldb < (int64_t)p->KB * 32) return fail(SRT_ERR_INVALID, "B (transposed, N x ldb) needs ldb >= K rounded up to 32");
     const int ks = std::max(1, std::min(ksplit, std::max(p->KB, 1)));
     if (ks > 1 && (size_t)ks * p->M * n > p->work_floats)
         return fail(SRT_ERR_INVALID, "K split needs a larger reservation (srt_mfma_plan_reserve)");
@@ -208,6 +211,7 @@ int srt_mfma_plan_stats_get(const srt_mfma_plan *p, srt_mfma_stats *o) {
     o->dtype = p->dtype;
     o->nnz = p->nnz;
     o->stored_blocks = p->nblk;
+    o->nonzero_blocks = p->nz_blocks;
     return SRT_OK;
 }
 

@@ -1,10 +1,12 @@
 // Precision form: BF16 and FP8-e4m3 SpMM on MFMA over 16-row panels.
 //
-// A is cut into 16-row panels and 32-column K blocks.  Every block holding a
-// nonzero is stored dense (zeros filled: the panel is padded to a dense 16-row
-// block, the extreme of the reference's ILP padding, sbench/ilp_pad/
-// nano_solver.py:76-116, where each column pattern is padded up to all rows);
-// all-zero blocks are skipped.  A stored block is laid out exactly as the
+// A is cut into 16-row panels and 32-column K blocks, each stored dense (zeros
+// filled: the panel is padded to a dense 16-row block, the extreme of the
+// reference's ILP padding, sbench/ilp_pad/nano_solver.py:76-116, where each
+// column pattern is padded up to all rows).  This form is meant for panels
+// dense enough for matrix cores (the 50-70 % sparse precision shapes hold a
+// nonzero in practically every block); sparser matrices take the register-
+// tiled executors.  Block (p, kb) sits at slot p * KB + kb, laid out exactly as the
 // v_mfma_f32_16x16x32_{bf16,fp8_fp8} A operand wants it: lane l holds row
 // l % 16, columns 8 * (l / 16) .. + 8, so one lane reads one 16-byte (BF16) or
 // 8-byte (FP8) word per block.
@@ -22,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "srt_internal.h"
 
@@ -102,22 +105,21 @@ __global__ void blk_count_kernel(int P, int KB, const uint8_t *__restrict__ pres
     nblk[p] = n;
 }
 
-// One thread per panel: slot of every stored block (K order), -1 elsewhere.
+// One thread per panel: every block of the panel is stored, at slot
+// p * KB + kb (dense, so the executor computes block addresses without a
+// lookup); blocks without a nonzero stay zero.
 __global__ void blk_slot_kernel(int P, int KB, const uint8_t *__restrict__ present,
                                 const int64_t *__restrict__ blk_ptr, int32_t *__restrict__ slot,
                                 int32_t *__restrict__ blk_kb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
-    int64_t s = blk_ptr[p];
     for (int b = 0; b < KB; ++b) {
-        if (present[(int64_t)p * KB + b]) {
-            slot[(int64_t)p * KB + b] = (int32_t)s;
-            blk_kb[s] = b;
-            ++s;
-        } else {
-            slot[(int64_t)p * KB + b] = -1;
-        }
+        const int64_t s = (int64_t)p * KB + b;
+        slot[s] = (int32_t)s;
+        blk_kb[s] = b;
     }
+    (void)present;
+    (void)blk_ptr;
 }
 
 // One wave per row: FP8 row scale s = max|A_i| / 448 computed in FP64.
@@ -163,15 +165,6 @@ __global__ void blk_fill_kernel(int m, int k, int KB, const I *__restrict__ Lp,
 }
 
 // ------------------------------------------------------------ B operand prep
-// One thread per element: fp32 B -> bf16 (row-major, ld_out).
-__global__ void b_to_bf16_kernel(int64_t rows, int cols, const float *__restrict__ src, int64_t ld_src,
-                                 uint16_t *__restrict__ dst, int64_t ld_dst) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows * cols) return;
-    const int64_t r = i / cols, c = i % cols;
-    dst[r * ld_dst + c] = to_bf16(src[r * ld_src + c]);
-}
-
 // One block of 256 threads per 256 columns: FP8 column scales in FP64.
 __global__ void col_scale_kernel(int64_t rows, int cols, const float *__restrict__ src, int64_t ld,
                                  float *__restrict__ scale) {
@@ -182,120 +175,185 @@ __global__ void col_scale_kernel(int64_t rows, int cols, const float *__restrict
     scale[c] = mx > 0.0 ? (float)(mx / 448.0) : 1.0f;
 }
 
-__global__ void b_to_e4m3_kernel(int64_t rows, int cols, const float *__restrict__ src, int64_t ld_src,
-                                 const float *__restrict__ scale, uint8_t *__restrict__ dst,
-                                 int64_t ld_dst) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows * cols) return;
-    const int64_t r = i / cols, c = i % cols;
-    dst[r * ld_dst + c] = to_e4m3((float)((double)src[r * ld_src + c] / (double)scale[c]));
+// fp32 B (K x N, row-major) -> Bt (N x ldk, k contiguous) in BF16 or e4m3,
+// k in [K, ldk) zero: the MFMA B operand wants eight consecutive k of one
+// column per lane.  32 x 32 tiles go through LDS so that both the reads of B
+// and the writes of Bt are coalesced.
+template <bool FP8>
+__global__ void b_transpose_kernel(int K, int N, const float *__restrict__ src, int64_t ld_src,
+                                   const float *__restrict__ cscale, typename std::conditional<FP8, uint8_t, uint16_t>::type *__restrict__ dst,
+                                   int64_t ldk) {
+    __shared__ float t[32][33];
+    const int k0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int k = k0 + i, n = n0 + tx;
+        t[i][tx] = (k < K && n < N) ? src[(int64_t)k * ld_src + n] : 0.0f;
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int n = n0 + i, k = k0 + tx;
+        if (n < N && k < ldk) {
+            const float v = t[tx][i];
+            if constexpr (FP8)
+                dst[(int64_t)n * ldk + k] = to_e4m3((float)((double)v / (double)cscale[n]));
+            else
+                dst[(int64_t)n * ldk + k] = to_bf16(v);
+        }
+    }
 }
 
 // ------------------------------------------------------------ executor
 template <bool FP8>
 struct Elt {
     typedef uint16_t T;
+    typedef uint4 AOp;                   // A operand of one lane: 8 BF16
     static constexpr int kPerChunk = 8;  // elements per 16-byte chunk
+    static constexpr int kLdsK = 40;     // LDS row: 32 k + 8 padding (80 bytes)
 };
 template <>
 struct Elt<true> {
     typedef uint8_t T;
+    typedef long AOp;  // 8 e4m3
     static constexpr int kPerChunk = 16;
+    static constexpr int kLdsK = 48;  // 32 k + 16 padding (48 bytes)
 };
 
-// LDS tile [NT][kLdsK] of B transposed; 8 elements of padding per column row
-// keep the 16-byte operand reads aligned and spread the transposed writes.
-constexpr int kLdsK = kBlockK + 8;
-
-template <bool FP8, int NT, int WP>
+// Workgroup: WP waves x PW panels each (16 * WP * PW rows) x NT columns.
+// B arrives transposed (Bt: N x ldb, ldb >= KB * 32, k past K zero), so a B
+// tile is NT rows of 32 contiguous k: a straight 16-byte copy into LDS, and
+// each MFMA B operand is one 16-byte (8-byte) LDS read.  Software pipeline,
+// one barrier per K block: the B chunks and A operands of block kb+1 are
+// loaded into registers while the MFMAs of block kb run from the other LDS
+// buffer (the ring below keeps several blocks in flight).
+template <bool FP8, int NT, int WP, int PW>
 __global__ void __launch_bounds__(64 * WP)
-    mfma_exec_kernel(int M, int N, int K, int KB, int P, const typename Elt<FP8>::T *__restrict__ B,
-                     int64_t ldb, const void *__restrict__ blocks, const int32_t *__restrict__ slot,
+    mfma_exec_kernel(int M, int N, int KB, int P, const typename Elt<FP8>::T *__restrict__ Bt, int64_t ldb,
+                     const void *__restrict__ blocks, const int32_t *__restrict__ slot,
                      const float *__restrict__ rscale, const float *__restrict__ cscale,
                      float *__restrict__ C, int64_t ldc, int ks, float *__restrict__ work, int vec_ok) {
     typedef typename Elt<FP8>::T T;
+    typedef typename Elt<FP8>::AOp AOp;
     constexpr int PER = Elt<FP8>::kPerChunk;
-    constexpr int CHUNKS = kBlockK * NT / PER;  // 16-byte chunks of one B tile
+    constexpr int LDK = Elt<FP8>::kLdsK;
+    constexpr int CPR = kBlockK / PER;  // 16-byte chunks per tile row
+    constexpr int CHUNKS = NT * CPR;
     constexpr int NTHR = 64 * WP;
     constexpr int CPT = (CHUNKS + NTHR - 1) / NTHR;
     constexpr int TN = NT / 16;
-    __shared__ __attribute__((aligned(16))) T tile[NT * kLdsK];
+    __shared__ __attribute__((aligned(16))) T tile[2][NT * LDK];
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int pg = blockIdx.x, nt = blockIdx.y, z = blockIdx.z;
-    const int p = pg * WP + wave;
+    const int p0 = (pg * WP + wave) * PW;
     const int n0 = nt * NT;
     const int kb0 = (int)((int64_t)z * KB / ks), kb1 = (int)((int64_t)(z + 1) * KB / ks);
+    const int col = lane & 15, kg = lane >> 4;
 
-    f32x4 acc[TN];
+    f32x4 acc[PW][TN];
 #pragma unroll
-    for (int t = 0; t < TN; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < PW; ++i)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) acc[i][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int kb = kb0; kb < kb1; ++kb) {
-        // B tile rows kb*32 .. +32, columns n0 .. n0+NT -> tile[col][k]
+    // Register ring of DEPTH K blocks in flight (B chunks and A operands):
+    // HBM latency spans several blocks' worth of MFMAs.  The loop is unrolled
+    // by DEPTH so every ring index is a compile-time constant.
+    constexpr int DEPTH = 4;
+    uint4 breg[DEPTH][CPT];
+    AOp areg[DEPTH][PW];
+    auto load_b = [&](uint4 (&dst)[CPT], int kb) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
             const int ch = threadIdx.x + c * NTHR;
-            if (ch < CHUNKS) {
-                const int kk = ch / (NT / PER), cn = (ch % (NT / PER)) * PER;
-                const int64_t kr = (int64_t)kb * kBlockK + kk;
-                T v[PER];
-                if (vec_ok && kr < K && n0 + cn + PER <= N) {
-                    const uint4 w = *reinterpret_cast<const uint4 *>(B + kr * ldb + n0 + cn);
-                    __builtin_memcpy(v, &w, 16);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < PER; ++e)
-                        v[e] = (kr < K && n0 + cn + e < N) ? B[kr * ldb + n0 + cn + e] : (T)0;
-                }
-#pragma unroll
-                for (int e = 0; e < PER; ++e) tile[(cn + e) * kLdsK + kk] = v[e];
-            }
-        }
-        __syncthreads();
-        if (p < P) {
-            const int32_t sl = slot[(int64_t)p * KB + kb];
-            if (sl >= 0) {
-                const int col = lane & 15, kg = lane >> 4;
-                if constexpr (FP8) {
-                    const long a = reinterpret_cast<const long *>(blocks)[(int64_t)sl * 64 + lane];
-#pragma unroll
-                    for (int t = 0; t < TN; ++t) {
-                        const long b = *reinterpret_cast<const long *>(&tile[(t * 16 + col) * kLdsK + kg * 8]);
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc[t], 0, 0, 0);
-                    }
-                } else {
-                    const uint4 aw = reinterpret_cast<const uint4 *>(blocks)[(int64_t)sl * 64 + lane];
-                    const bf16x8 a = __builtin_bit_cast(bf16x8, aw);
-#pragma unroll
-                    for (int t = 0; t < TN; ++t) {
-                        const uint4 bw = *reinterpret_cast<const uint4 *>(&tile[(t * 16 + col) * kLdsK + kg * 8]);
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, bw),
-                                                                         acc[t], 0, 0, 0);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (p >= P) return;
-    // D layout: lane l holds rows 4 * (l / 16) .. + 4 of column l % 16.
-    const int col = lane & 15, rg = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < TN; ++t) {
-        const int n = n0 + t * 16 + col;
-        if (n >= N) continue;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = p * kPanelRows + rg * 4 + i;
-            if (row >= M) continue;
-            float v = acc[t][i];
-            if (ks > 1) {
-                work[((int64_t)z * M + row) * N + n] = v;
+            const int n = ch / CPR, part = ch % CPR;
+            const bool in = ch < CHUNKS && n0 + n < N;
+            const T *src = Bt + (int64_t)(n0 + n) * ldb + (int64_t)kb * kBlockK + part * PER;
+            if (in && vec_ok) {
+                dst[c] = *reinterpret_cast<const uint4 *>(src);
             } else {
-                if constexpr (FP8) v = v * rscale[row] * cscale[n];
-                C[(int64_t)row * ldc + n] = v;
+                T v[PER];
+#pragma unroll
+                for (int e = 0; e < PER; ++e) v[e] = in ? src[e] : (T)0;
+                __builtin_memcpy(&dst[c], v, 16);
+            }
+        }
+    };
+    auto load_a = [&](AOp (&dst)[PW], int kb) {
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            const int p = p0 + i;
+            if (p < P) dst[i] = reinterpret_cast<const AOp *>(blocks)[((int64_t)p * KB + kb) * 64 + lane];
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < DEPTH; ++j) {
+        if (kb0 + j < kb1) {
+            load_b(breg[j], kb0 + j);
+            load_a(areg[j], kb0 + j);
+        }
+    }
+    int buf = 0;
+    for (int base = kb0; base < kb1; base += DEPTH) {
+#pragma unroll
+        for (int j = 0; j < DEPTH; ++j) {
+            const int kb = base + j;
+            if (kb >= kb1) break;
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+                const int ch = threadIdx.x + c * NTHR;
+                if (ch < CHUNKS)
+                    *reinterpret_cast<uint4 *>(&tile[buf][(ch / CPR) * LDK + (ch % CPR) * PER]) = breg[j][c];
+            }
+            __syncthreads();
+            AOp a[PW];
+#pragma unroll
+            for (int i = 0; i < PW; ++i) a[i] = areg[j][i];
+            if (kb + DEPTH < kb1) {
+                load_b(breg[j], kb + DEPTH);
+                load_a(areg[j], kb + DEPTH);
+            }
+#pragma unroll
+            for (int t = 0; t < TN; ++t) {
+                const T *bp = &tile[buf][(t * 16 + col) * LDK + kg * 8];
+                if constexpr (FP8) {
+                    const long bw = *reinterpret_cast<const long *>(bp);
+#pragma unroll
+                    for (int i = 0; i < PW; ++i)
+                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], bw, acc[i][t], 0, 0, 0);
+                } else {
+                    const bf16x8 bw = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(bp));
+#pragma unroll
+                    for (int i = 0; i < PW; ++i)
+                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[i]), bw,
+                                                                            acc[i][t], 0, 0, 0);
+                }
+            }
+            buf ^= 1;
+        }
+    }
+    // D layout: lane l holds rows 4 * (l / 16) .. + 4 of column l % 16.
+    const int rg = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {
+        const int p = p0 + i;
+        if (p >= P) continue;
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            const int n = n0 + t * 16 + col;
+            if (n >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = p * kPanelRows + rg * 4 + r;
+                if (row >= M) continue;
+                float v = acc[i][t][r];
+                if (ks > 1) {
+                    work[((int64_t)z * M + row) * N + n] = v;
+                } else {
+                    if constexpr (FP8) v = v * rscale[row] * cscale[n];
+                    C[(int64_t)row * ldc + n] = v;
+                }
             }
         }
     }
@@ -371,29 +429,29 @@ hipError_t launch_mfma_fill(int m, int k, int KB, const void *Lp, const void *Li
 }
 
 hipError_t launch_b_convert(int64_t rows, int cols, const float *src, int64_t ld_src, bool fp8,
-                            float *cscale, void *dst, int64_t ld_dst, hipStream_t s) {
-    const int64_t n = rows * cols;
-    if (n <= 0) return hipSuccess;
-    const unsigned grid = (unsigned)((n + 255) / 256);
+                            float *cscale, void *dst, int64_t ldk, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((ldk + 31) / 32), (unsigned)((cols + 31) / 32));
     if (fp8) {
         hipLaunchKernelGGL(col_scale_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, rows, cols, src, ld_src,
                            cscale);
-        hipLaunchKernelGGL(b_to_e4m3_kernel, dim3(grid), dim3(256), 0, s, rows, cols, src, ld_src, cscale,
-                           (uint8_t *)dst, ld_dst);
+        hipLaunchKernelGGL(b_transpose_kernel<true>, grid, dim3(256), 0, s, (int)rows, cols, src, ld_src, cscale,
+                           (uint8_t *)dst, ldk);
     } else {
-        hipLaunchKernelGGL(b_to_bf16_kernel, dim3(grid), dim3(256), 0, s, rows, cols, src, ld_src,
-                           (uint16_t *)dst, ld_dst);
+        hipLaunchKernelGGL(b_transpose_kernel<false>, grid, dim3(256), 0, s, (int)rows, cols, src, ld_src,
+                           (const float *)nullptr, (uint16_t *)dst, ldk);
     }
     return hipGetLastError();
 }
 
-template <bool FP8, int NT, int WP>
+template <bool FP8, int NT, int WP, int PW>
 static hipError_t launch_exec_t(const MfmaLaunch &a, hipStream_t s) {
-    const dim3 grid((a.P + WP - 1) / WP, (a.N + NT - 1) / NT, a.ks);
-    hipLaunchKernelGGL((mfma_exec_kernel<FP8, NT, WP>), grid, dim3(64 * WP), 0, s, a.M, a.N, a.K, a.KB, a.P,
+    const dim3 grid((a.P + WP * PW - 1) / (WP * PW), (a.N + NT - 1) / NT, a.ks);
+    const int vec_ok =
+        ((uintptr_t)a.B % 16 == 0 && (a.ldb * (int64_t)sizeof(typename Elt<FP8>::T)) % 16 == 0) ? 1 : 0;
+    hipLaunchKernelGGL((mfma_exec_kernel<FP8, NT, WP, PW>), grid, dim3(64 * WP), 0, s, a.M, a.N, a.KB, a.P,
                        static_cast<const typename Elt<FP8>::T *>(a.B), a.ldb, a.blocks, a.slot, a.rscale,
-                       a.cscale, a.C, a.ldc, a.ks, a.work,
-                       ((uintptr_t)a.B % 16 == 0 && (a.ldb * (int64_t)sizeof(typename Elt<FP8>::T)) % 16 == 0) ? 1 : 0);
+                       a.cscale, a.C, a.ldc, a.ks, a.work, vec_ok);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.ks <= 1) return e;
     const int64_t n = (int64_t)a.M * a.N;
@@ -402,14 +460,19 @@ static hipError_t launch_exec_t(const MfmaLaunch &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Launch shapes: (N tile, waves per workgroup, panels per wave).
+constexpr int kShapes[][3] = {{64, 4, 2}, {128, 4, 2}, {64, 4, 1}, {128, 2, 2}, {32, 4, 1}, {128, 4, 1}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
 template <bool FP8>
 static hipError_t launch_exec_dt(const MfmaLaunch &a, hipStream_t s) {
     switch (a.shape) {
-        case 0: return launch_exec_t<FP8, 64, 4>(a, s);
-        case 1: return launch_exec_t<FP8, 128, 4>(a, s);
-        case 2: return launch_exec_t<FP8, 64, 8>(a, s);
-        case 3: return launch_exec_t<FP8, 128, 2>(a, s);
-        case 4: return launch_exec_t<FP8, 32, 4>(a, s);
+        case 0: return launch_exec_t<FP8, 64, 4, 2>(a, s);
+        case 1: return launch_exec_t<FP8, 128, 4, 2>(a, s);
+        case 2: return launch_exec_t<FP8, 64, 4, 1>(a, s);
+        case 3: return launch_exec_t<FP8, 128, 2, 2>(a, s);
+        case 4: return launch_exec_t<FP8, 32, 4, 1>(a, s);
+        case 5: return launch_exec_t<FP8, 128, 4, 1>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -419,11 +482,11 @@ hipError_t launch_mfma_exec(const MfmaLaunch &a, hipStream_t s) {
     return a.fp8 ? launch_exec_dt<true>(a, s) : launch_exec_dt<false>(a, s);
 }
 
-int mfma_shape_tile(int shape, int *nt, int *wp) {
-    static const int t[5][2] = {{64, 4}, {128, 4}, {64, 8}, {128, 2}, {32, 4}};
-    if (shape < 0 || shape >= 5) return -1;
-    *nt = t[shape][0];
-    *wp = t[shape][1];
+int mfma_shape_tile(int shape, int *nt, int *wp, int *pw) {
+    if (shape < 0 || shape >= kNumShapes) return -1;
+    *nt = kShapes[shape][0];
+    *wp = kShapes[shape][1];
+    *pw = kShapes[shape][2];
     return 0;
 }
 

@@ -159,9 +159,9 @@ hipError_t launch_stable_order(int n, int NK, const int32_t *ids, const int32_t 
 // Precision form on MFMA (mfma_panel.hip): 16-row panels, 32-column blocks.
 struct MfmaLaunch {
     bool fp8;
-    int shape;  // (N tile, panels per workgroup) index, see mfma_shape_tile
+    int shape;  // (N tile, waves, panels per wave) index, see mfma_shape_tile
     int M, N, K, KB, P;
-    const void *B;  // bf16 or e4m3, row-major, ldb elements
+    const void *B;  // Bt: bf16 or e4m3, N x ldb (k contiguous), ldb >= KB * 32
     int64_t ldb;
     const void *blocks;     // stored blocks in MFMA A-operand layout
     const int32_t *slot;    // [P][KB] block slot or -1
@@ -181,9 +181,9 @@ hipError_t launch_mfma_fill(int m, int k, int KB, const void *Lp, const void *Li
                             const float *Lx, bool fp8, const int32_t *slot, float *rscale,
                             void *blocks, hipStream_t s);
 hipError_t launch_b_convert(int64_t rows, int cols, const float *src, int64_t ld_src, bool fp8,
-                            float *cscale, void *dst, int64_t ld_dst, hipStream_t s);
+                            float *cscale, void *dst, int64_t ldk, hipStream_t s);
 hipError_t launch_mfma_exec(const MfmaLaunch &a, hipStream_t s);
-int mfma_shape_tile(int shape, int *nt, int *wp);
+int mfma_shape_tile(int shape, int *nt, int *wp, int *pw);
 
 // CSR baseline (csr_spmm.hip).
 hipError_t launch_csr(int m, int k, int n, int64_t nnz, const void *rowptr, const void *colind,

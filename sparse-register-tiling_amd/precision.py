@@ -11,8 +11,8 @@ scales of B, both computed in FP64 on the device:
     C_ij = s_i * t_j * sum_k e4m3(A_ik / s_i) * e4m3(B_kj / t_j)
 
     plan = PanelPlan.from_csr(A_csr, dtype="bf16")      # or "fp8"
-    Bq, t = plan.prepare_b(B)                             # fp32 -> bf16 / e4m3 (+ scales)
-    plan(C, Bq, t)                                        # C = A . B  (fp32)
+    Bt, t = plan.prepare_b(B)                             # fp32 -> transposed bf16 / e4m3 (+ scales)
+    plan(C, Bt, t)                                        # C = A . B  (fp32)
 """
 from __future__ import annotations
 
@@ -26,8 +26,8 @@ from . import _lib
 from .native import ptr, stream_handle
 
 DTYPES = {"bf16": _lib.SRT_DT_BF16, "fp8": _lib.SRT_DT_FP8}
-# launch shapes: index -> (N tile, panels per workgroup)
-SHAPES = {0: (64, 4), 1: (128, 4), 2: (64, 8), 3: (128, 2), 4: (32, 4)}
+# launch shapes: index -> (N tile, waves per workgroup, panels per wave)
+SHAPES = {0: (64, 4, 2), 1: (128, 4, 2), 2: (64, 4, 1), 3: (128, 2, 2), 4: (32, 4, 1), 5: (128, 4, 1)}
 
 
 def _index_tensor(x, device) -> torch.Tensor:
@@ -39,10 +39,10 @@ def _index_tensor(x, device) -> torch.Tensor:
 
 def default_launch(num_panels: int, k_blocks: int, n: int, target_groups: int = 512) -> Tuple[int, int]:
     """(shape, ksplit) giving about `target_groups` workgroups: 64-column
-    tiles of 4 panels, the K range split when panels x tiles are too few."""
+    tiles of 8 panels, the K range split when panels x tiles are too few."""
     shape = 0
-    nt, wp = SHAPES[shape]
-    groups = -(-num_panels // wp) * -(-n // nt)
+    nt, wp, pw = SHAPES[shape]
+    groups = -(-num_panels // (wp * pw)) * -(-n // nt)
     ks = 1
     while groups * ks * 2 <= target_groups and ks * 2 <= max(1, k_blocks // 4):
         ks *= 2
@@ -85,17 +85,19 @@ class PanelPlan:
     # -- operands -----------------------------------------------------------
     def prepare_b(self, B: torch.Tensor, stream: Optional[torch.cuda.Stream] = None
                   ) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
-        """fp32 device B -> (B in the plan's dtype, FP8 column scales or None).
-        BF16 comes back as torch.bfloat16, FP8 as raw e4m3 bytes (uint8)."""
+        """fp32 device B (K x N) -> (Bt, FP8 column scales or None): Bt is the
+        B operand transposed to N x ldk (k contiguous, ldk = K rounded up to
+        32, zeros past K) in the plan's dtype -- torch.bfloat16, or raw e4m3
+        bytes (uint8)."""
         if not (B.is_cuda and B.dtype == torch.float32 and B.dim() == 2 and B.stride(1) == 1):
             raise ValueError("B must be a row-major 2-D float32 CUDA tensor")
         rows, cols = B.shape
-        ld = -(-cols // 16) * 16  # keep rows 16-byte aligned for the tile loads
+        ldk = -(-rows // 32) * 32
         if self.dtype == "bf16":
-            out = torch.empty((rows, ld), dtype=torch.bfloat16, device=B.device)[:, :cols]
+            out = torch.empty((cols, ldk), dtype=torch.bfloat16, device=B.device)
             scales = None
         else:
-            out = torch.empty((rows, ld), dtype=torch.uint8, device=B.device)[:, :cols]
+            out = torch.empty((cols, ldk), dtype=torch.uint8, device=B.device)
             scales = torch.empty(cols, dtype=torch.float32, device=B.device)
         _lib.call("srt_mfma_convert_b", DTYPES[self.dtype], rows, cols, ptr(B), B.stride(0), ptr(out),
                   out.stride(0), ptr(scales), stream_handle(stream))
@@ -111,11 +113,11 @@ class PanelPlan:
                  stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
         want = torch.bfloat16 if self.dtype == "bf16" else torch.uint8
         if Bq.dtype != want or not Bq.is_cuda or Bq.stride(1) != 1:
-            raise ValueError("B must be a row-major CUDA %s tensor (see prepare_b)" % want)
+            raise ValueError("B must be the transposed CUDA %s operand from prepare_b" % want)
         if not (C.is_cuda and C.dtype == torch.float32 and C.stride(1) == 1):
             raise ValueError("C must be a row-major float32 CUDA tensor")
-        n = Bq.shape[1] if n is None else int(n)
-        if Bq.shape[0] < self.k or C.shape[0] < self.m or C.shape[1] < n or Bq.shape[1] < n:
+        n = Bq.shape[0] if n is None else int(n)
+        if Bq.shape[1] < -(-self.k // 32) * 32 or C.shape[0] < self.m or C.shape[1] < n or Bq.shape[0] < n:
             raise ValueError("shape mismatch")
         st = self.stats()
         dshape, dks = default_launch(st["num_panels"], st["k_blocks"], n)

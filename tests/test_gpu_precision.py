@@ -48,7 +48,10 @@ def _run(srt, torch, A, B, dtype, shape=None, ksplit=None, check_pack=True):
     if check_pack:
         qb, tref = po.quantize_b(B, dtype)
         gotb = Bq.cpu().view(torch.int16).numpy().view(np.uint16) if dtype == "bf16" else Bq.cpu().numpy()
-        assert np.array_equal(gotb, qb)
+        K = B.shape[0]
+        assert gotb.shape == (B.shape[1], -(-K // 32) * 32)
+        assert np.array_equal(gotb[:, :K], qb.T)
+        assert not gotb[:, K:].any()
         if dtype == "fp8":
             assert np.array_equal(t.cpu().numpy(), tref)
     C = torch.full((A.shape[0], B.shape[1]), float("nan"), device="cuda")
@@ -75,7 +78,7 @@ def test_small_shapes(srt, torch_cuda, dtype, shape_mkn):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "fp8"])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("ksplit", [1, 2, 5])
 def test_launch_shapes_and_splits(srt, torch_cuda, dtype, shape, ksplit):
     _log("precision shape", dtype, shape, ksplit)
@@ -95,8 +98,8 @@ def test_unaligned_b_and_empty(srt, torch_cuda, dtype):
     B = np.random.default_rng(10).uniform(-1, 1, (96, 50)).astype(np.float32)
     plan = srt.precision.PanelPlan.from_dense(torch.from_numpy(A), dtype)
     Bq, t = plan.prepare_b(torch.from_numpy(B).cuda())
-    big = torch.zeros((96, 53), dtype=Bq.dtype, device="cuda")
-    view = big[:, 1:51]           # 2- or 1-byte offset: not 16-byte aligned, odd row stride
+    big = torch.zeros((50, 99), dtype=Bq.dtype, device="cuda")
+    view = big[:, 1:97]           # 2- or 1-byte offset: not 16-byte aligned, odd row stride
     view.copy_(Bq)
     C = torch.empty((64, 50), device="cuda")
     plan(C, view, t)
@@ -104,7 +107,7 @@ def test_unaligned_b_and_empty(srt, torch_cuda, dtype):
     _within(C.cpu().numpy(), A, B, dtype)
     Z = np.zeros((40, 30), dtype=np.float32)
     pz = srt.precision.PanelPlan.from_dense(torch.from_numpy(Z), dtype)
-    assert pz.stats()["stored_blocks"] == 0
+    assert pz.stats()["nonzero_blocks"] == 0
     Bz, tz = pz.prepare_b(torch.ones((30, 8), device="cuda"))
     Cz = torch.full((40, 8), 5.0, device="cuda")
     pz(Cz, Bz, tz)
