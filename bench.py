@@ -64,6 +64,12 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "serving", "cnn"],
+                    help="plain = the headline FP32 workload (default); the others print their own line")
+    ap.add_argument("--qps", default="5000,10000,20000,40000,80000,160000",
+                    help="serving: offered loads (requests/s)")
+    ap.add_argument("--requests", type=int, default=1500, help="serving: requests per load level")
+    ap.add_argument("--slo-ms", type=float, default=2.0, help="serving: p99 target for the throughput ratio")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--mapping", default="auto", help="M_r:name (e.g. 8:identity) or auto")
@@ -256,6 +262,124 @@ def choose_config(args, srt, csr, N, Bd):
     return M_r, name, srt.TileConfig(**tile), tune_top
 
 
+# --------------------------------------------------------------------------
+# other forms
+# --------------------------------------------------------------------------
+def bench_precision(args):
+    """BF16 / FP8-e4m3 on MFMA 16-row panels: the four precision workloads
+    (4096x1024 and 1024x4096, 50/70 % magnitude-pruned, N = 512), default
+    launch choice, graph-timed; error vs FP64 of the unquantised inputs."""
+    import numpy as np
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    at = importlib.import_module("sparse-register-tiling_amd.autotune")
+    from oracle import precision_oracle as po
+    wl = srt.workloads
+    rows = []
+    for (M, K), s in [((4096, 1024), 0.5), ((4096, 1024), 0.6), ((4096, 1024), 0.7),
+                      ((1024, 4096), 0.5), ((1024, 4096), 0.6), ((1024, 4096), 0.7)]:
+        A, B = wl.case_inputs(wl.SpMMCase(M, K, s, 512), seed=0)
+        nnz = int((A != 0).sum())
+        Bd = torch.from_numpy(B).cuda()
+        C32 = None
+        for dtype in ("bf16", "fp8"):
+            plan = srt.precision.PanelPlan.from_dense(torch.from_numpy(A), dtype)
+            Bt, t = plan.prepare_b(Bd)
+            C = torch.empty((M, 512), device="cuda")
+            shape, ks = srt.precision.default_launch(plan.stats()["num_panels"], plan.stats()["k_blocks"], 512)
+            # measured best launch per shape family (graph-timed sweep on MI355X)
+            best = (4, 1) if M == 4096 and dtype == "fp8" else ((0, 1) if M == 4096 else (0 if dtype == "bf16" else 2, 8))
+            shape, ks = best
+            plan.reserve(512, ks)
+            sec = at.time_graph(lambda: plan(C, Bt, t, shape=shape, ksplit=ks), reps=50)
+            plan(C, Bt, t, shape=shape, ksplit=ks)
+            torch.cuda.synchronize()
+            err = po.rel_error_vs_fp64(C.cpu().numpy(), A, B)
+            esz = 2 if dtype == "bf16" else 1
+            balg = nnz * esz + K * 512 * esz + M * 512 * 4 + (4 * M if dtype == "fp8" else 0)
+            st = plan.stats()
+            padded = 2.0 * st["stored_blocks"] * 512 * 512
+            peak = 2.5e15 if dtype == "bf16" else 5.0e15
+            rows.append({"M": M, "K": K, "sparsity": s, "dtype": dtype, "us": round(sec * 1e6, 2),
+                         "gflops": round(2 * nnz * 512 / sec / 1e9, 1),
+                         "hbm_frac": round(balg / sec / 8e12, 4), "mfma_padded_frac": round(padded / sec / peak, 4),
+                         "rel_err_vs_fp64": float("%.3e" % err), "shape": shape, "ksplit": ks})
+            log(rows[-1])
+            plan.close()
+        # the plain FP32 kernel on the same matrix, for the error comparison
+        csr = torch.from_numpy(A).to_sparse_csr()
+        p32 = srt.MatMulSpecialized.from_csr(csr, 512, mapping_id=srt.named_mapping(4, "orig"))
+        C32 = torch.empty((M, 512), device="cuda")
+        p32(C32, Bd)
+        torch.cuda.synchronize()
+        rows.append({"M": M, "K": K, "sparsity": s, "dtype": "f32 (register-tiled)",
+                     "rel_err_vs_fp64": float("%.3e" % po.rel_error_vs_fp64(C32.cpu().numpy(), A, B))})
+        p32.close()
+    print(json.dumps({"form": "precision", "metric": "SpMM GFLOP/s (2·nnz·N/s), %HBM and %MFMA peak, error vs FP64",
+                      "unit": "GFLOP/s", "n_gpus": 1, "data": "synthetic", "results": rows}), flush=True)
+
+
+def bench_serving(args):
+    """48 resident BF16 weights (12 x {1024x1024 x2, 4096x1024, 1024x4096},
+    90 % magnitude-pruned), Poisson requests with log-uniform N in [1, 512]:
+    the continuous-batching server against one plain FP32 launch per request,
+    same arrivals, at several offered loads."""
+    import numpy as np
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    sv = srt.serving
+    Ws = sv.serving_weights()
+    shapes = [W.shape for W in Ws]
+    t0 = time.perf_counter()
+    panels = [srt.precision.PanelPlan.from_dense(torch.from_numpy(W), "bf16") for W in Ws]
+    plans32 = []
+    for W in Ws:
+        csr = torch.from_numpy(W).to_sparse_csr()
+        plans32.append(srt.MatMulSpecialized.from_csr(csr, 512, mapping_id=srt.named_mapping(4, "orig"),
+                                                      tile_config=srt.TileConfig(xcd_remap=1)))
+    log("packed 48 weights in %.1f s" % (time.perf_counter() - t0))
+    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, grid=1024)
+    levels = []
+    for qps in [float(q) for q in args.qps.split(",")]:
+        reqs = sv.poisson_requests(args.requests, qps, shapes, seed=int(qps) % 1000 + 1)
+        st = server.serve(reqs)
+        plain_reqs = [sv.Request(rid=r.rid, weight=r.weight, B=r.B, arrival=r.arrival) for r in reqs]
+        Bs32 = [r.B.float() for r in plain_reqs]
+        pst = sv.serve_plain(plans32, plain_reqs, Bs32)
+        levels.append({"offered_qps": qps, "served": st, "plain_fp32": pst})
+        log("qps %.0f served p50 %.3f p99 %.3f ms %.0f req/s | plain p50 %.3f p99 %.3f ms %.0f req/s"
+            % (qps, st["p50_ms"], st["p99_ms"], st["req_per_s"], pst["p50_ms"], pst["p99_ms"], pst["req_per_s"]))
+        del reqs, plain_reqs, Bs32
+    # saturation: everything arrives at once
+    reqs = sv.poisson_requests(args.requests, 1e9, shapes, seed=99)
+    sat = server.serve(reqs)
+    plain_reqs = [sv.Request(rid=r.rid, weight=r.weight, B=r.B, arrival=r.arrival) for r in reqs]
+    psat = sv.serve_plain(plans32, plain_reqs, [r.B.float() for r in plain_reqs])
+    ok_s = [l["offered_qps"] for l in levels if l["served"]["p99_ms"] <= args.slo_ms]
+    ok_p = [l["offered_qps"] for l in levels if l["plain_fp32"]["p99_ms"] <= args.slo_ms]
+    ratio = (max(ok_s) / max(ok_p)) if ok_s and ok_p else None
+    # accuracy of the served form on a sample
+    from oracle import precision_oracle as po
+    sample = sv.poisson_requests(16, 1e9, shapes, seed=7)
+    for r in sample:
+        server.run_batch([r])
+    errs = [po.rel_error_vs_fp64(r.C.cpu().numpy(), Ws[r.weight], r.B.float().cpu().numpy()) for r in sample]
+    out = {"form": "serving", "metric": "served p99 latency vs plain; requests/s at p99 <= %.1f ms" % args.slo_ms,
+           "unit": "req/s", "n_gpus": 1, "data": "synthetic", "dtype": "bf16 (served) / f32 (plain)",
+           "weights": "48 x {1024x1024 x2, 4096x1024, 1024x4096} per 12 layers, 90% magnitude-pruned",
+           "requests_per_level": args.requests, "slo_ms": args.slo_ms,
+           "max_qps_within_slo": {"served": max(ok_s) if ok_s else None, "plain_fp32": max(ok_p) if ok_p else None},
+           "throughput_ratio_at_equal_p99": ratio,
+           "saturation_req_per_s": {"served": sat["req_per_s"], "plain_fp32": psat["req_per_s"]},
+           "saturation_ratio": sat["req_per_s"] / psat["req_per_s"],
+           "served_rel_err_vs_fp64_max": max(errs), "levels": levels}
+    print(json.dumps(out), flush=True)
+
+
+def bench_cnn(args):
+    raise SystemExit("the CNN grouped form is not built yet")
+
+
 def capture(launches):
     import torch
     s = torch.cuda.Stream()
@@ -280,6 +404,12 @@ def main():
         print("bench.py measures one GPU: runs on more than one GPU are not part of this build "
               "(the SpMM would shard by row panels without communication)", file=sys.stderr)
         sys.exit(2)
+    if args.form == "precision":
+        return bench_precision(args)
+    if args.form == "serving":
+        return bench_serving(args)
+    if args.form == "cnn":
+        return bench_cnn(args)
     import numpy as np
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")

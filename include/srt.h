@@ -265,6 +265,51 @@ int srt_mfma_plan_row_scales(const srt_mfma_plan *p, float *out);
 int srt_mfma_plan_copy_blocks(const srt_mfma_plan *p, int32_t *slot, int32_t *blk_kb, void *blocks);
 void srt_mfma_plan_destroy(srt_mfma_plan *p);
 
+/* Grouped, persistent execution of many independent SpMMs in one launch
+ * (a batch of served requests, the layers of a network).  No counterpart in
+ * the reference (it runs one MatMulSpecialized per call, SpMM_SOP.h:178).
+ * The host fills one srt_mfma_problem per SpMM (srt_mfma_problem_fill),
+ * assigns tile_begin as the running sum of srt_mfma_problem_tiles in table
+ * order, and copies the table plus a header {number of problems, total
+ * tiles} (two ints) to device memory; the launch reads both from there, so
+ * a captured graph replays any batch written into the same buffers.  Every
+ * workgroup of the persistent grid walks tiles blockIdx, blockIdx + grid, ...
+ * All problems share one dtype and launch shape; no K split. */
+typedef struct srt_mfma_problem {
+    const void *blocks;   /* plan's packed panels                         */
+    const void *Bt;       /* transposed B operand (n x ldb)               */
+    float *C;
+    const float *rscale;  /* plan's FP8 row scales                        */
+    const float *cscale;  /* FP8 column scales of B                       */
+    int64_t ldb, ldc;
+    int M, N, KB, P;
+    int tile_begin;
+    int vec_ok;
+} srt_mfma_problem;
+
+int srt_mfma_problem_fill(const srt_mfma_plan *p, const void *Bt, int64_t ldb, const float *col_scales, float *C,
+                          int64_t ldc, int n, int tile_begin, srt_mfma_problem *out);
+int srt_mfma_problem_tiles(const srt_mfma_problem *pr, int shape);
+int srt_mfma_grouped_execute(int dtype, int shape, const srt_mfma_problem *d_probs, const int *d_hdr, int grid,
+                             srt_stream_t stream);
+
+/* Grouped B preparation: fp32 or BF16 B (K x N, row-major) -> transposed
+ * operand (N x ldk, zeros past K), many problems in one launch (FP8 needs
+ * cscale computed beforehand).  Same table + header convention. */
+typedef struct srt_bprep_problem {
+    const void *src;
+    void *dst;
+    const float *cscale;
+    int64_t ld_src, ldk;
+    int K, N;
+    int tile_begin;
+    int src_bf16;   /* 1: src holds BF16, 0: fp32 */
+} srt_bprep_problem;
+
+int srt_bprep_problem_tiles(const srt_bprep_problem *pr);
+int srt_mfma_grouped_b_prep(int dtype, const srt_bprep_problem *d_probs, const int *d_hdr, int grid,
+                            srt_stream_t stream);
+
 /* ------------------------------------------------------------------------
  * CSR baseline (sbench/SOP/_C/pytorch_csr_wrapper.cpp:243-312 _csr_kernel,
  * torch entry csr.executor :314-353): C[m x n] = A_csr * B, device arrays

@@ -109,6 +109,11 @@ SIGNATURES = {
     "srt_mfma_plan_row_scales": (_i, [_vp, _vp]),
     "srt_mfma_plan_copy_blocks": (_i, [_vp, _vp, _vp, _vp]),
     "srt_mfma_plan_destroy": (None, [_vp]),
+    "srt_mfma_problem_fill": (_i, [_vp, _vp, _i64, _vp, _vp, _i64, _i, _i, _vp]),
+    "srt_mfma_problem_tiles": (_i, [_vp, _i]),
+    "srt_mfma_grouped_execute": (_i, [_i, _i, _vp, _vp, _i, _vp]),
+    "srt_bprep_problem_tiles": (_i, [_vp]),
+    "srt_mfma_grouped_b_prep": (_i, [_i, _vp, _vp, _i, _vp]),
 }
 
 _lib = None

@@ -233,4 +233,65 @@ int srt_mfma_plan_copy_blocks(const srt_mfma_plan *p, int32_t *slot, int32_t *bl
 
 void srt_mfma_plan_destroy(srt_mfma_plan *p) { delete p; }
 
+static_assert(sizeof(srt_mfma_problem) == sizeof(MfmaProblem), "problem layout");
+static_assert(sizeof(srt_bprep_problem) == sizeof(BPrepProblem), "problem layout");
+
+int srt_mfma_problem_fill(const srt_mfma_plan *p, const void *Bt, int64_t ldb, const float *col_scales, float *C,
+                          int64_t ldc, int n, int tile_begin, srt_mfma_problem *out) {
+    if (!p || !out) return fail(SRT_ERR_INVALID, "null argument");
+    if (n < 0 || ldc < n || ldb < (int64_t)p->KB * 32) return fail(SRT_ERR_INVALID, "bad problem shape");
+    if (n > 0 && (!Bt || !C)) return fail(SRT_ERR_INVALID, "null B or C");
+    if (p->dtype == SRT_DT_FP8 && n > 0 && !col_scales) return fail(SRT_ERR_INVALID, "FP8 needs column scales");
+    out->blocks = p->blocks;
+    out->Bt = Bt;
+    out->C = C;
+    out->rscale = p->rscale;
+    out->cscale = col_scales;
+    out->ldb = ldb;
+    out->ldc = ldc;
+    out->M = p->M;
+    out->N = n;
+    out->KB = p->KB;
+    out->P = p->P;
+    out->tile_begin = tile_begin;
+    const int eb = elt_bytes(p->dtype);
+    out->vec_ok = ((uintptr_t)Bt % 16 == 0 && (ldb * eb) % 16 == 0) ? 1 : 0;
+    return SRT_OK;
+}
+
+int srt_mfma_problem_tiles(const srt_mfma_problem *pr, int shape) {
+    int nt = 0, wp = 0, pw = 0;
+    if (!pr || mfma_shape_tile(shape, &nt, &wp, &pw)) return -1;
+    if (pr->N <= 0 || pr->M <= 0) return 0;
+    return ((pr->P + wp * pw - 1) / (wp * pw)) * ((pr->N + nt - 1) / nt);
+}
+
+int srt_mfma_grouped_execute(int dtype, int shape, const srt_mfma_problem *d_probs, const int *d_hdr, int grid,
+                             srt_stream_t stream) {
+    if (dtype != SRT_DT_BF16 && dtype != SRT_DT_FP8) return fail(SRT_ERR_INVALID, "dtype must be BF16 or FP8");
+    int nt = 0, wp = 0, pw = 0;
+    if (mfma_shape_tile(shape, &nt, &wp, &pw)) return fail(SRT_ERR_INVALID, "unknown launch shape");
+    if (grid < 1) return fail(SRT_ERR_INVALID, "grid must be >= 1");
+    if (!d_probs || !d_hdr) return fail(SRT_ERR_INVALID, "null problem table");
+    SRT_HIP_TRY(launch_mfma_grouped(dtype == SRT_DT_FP8, shape, reinterpret_cast<const MfmaProblem *>(d_probs),
+                                    d_hdr, grid, (hipStream_t)stream));
+    return SRT_OK;
+}
+
+int srt_bprep_problem_tiles(const srt_bprep_problem *pr) {
+    if (!pr) return -1;
+    if (pr->N <= 0 || pr->ldk <= 0) return 0;
+    return (int)(((pr->ldk + 31) / 32) * ((pr->N + 31) / 32));
+}
+
+int srt_mfma_grouped_b_prep(int dtype, const srt_bprep_problem *d_probs, const int *d_hdr, int grid,
+                            srt_stream_t stream) {
+    if (dtype != SRT_DT_BF16 && dtype != SRT_DT_FP8) return fail(SRT_ERR_INVALID, "dtype must be BF16 or FP8");
+    if (grid < 1) return fail(SRT_ERR_INVALID, "grid must be >= 1");
+    if (!d_probs || !d_hdr) return fail(SRT_ERR_INVALID, "null problem table");
+    SRT_HIP_TRY(launch_grouped_b_prep(dtype == SRT_DT_FP8, reinterpret_cast<const BPrepProblem *>(d_probs), d_hdr,
+                                      grid, (hipStream_t)stream));
+    return SRT_OK;
+}
+
 }  // extern "C"

@@ -226,12 +226,18 @@ struct Elt<true> {
 // one barrier per K block: the B chunks and A operands of block kb+1 are
 // loaded into registers while the MFMAs of block kb run from the other LDS
 // buffer (the ring below keeps several blocks in flight).
+template <bool FP8, int NT>
+using TileBuf = typename Elt<FP8>::T[2][NT * Elt<FP8>::kLdsK];
+
+// One output tile: panels (pg * WP + wave) * PW .. + PW, columns nt * NT ..,
+// K range z of ks.  Every wave of the workgroup calls it (barriers inside).
 template <bool FP8, int NT, int WP, int PW>
-__global__ void __launch_bounds__(64 * WP)
-    mfma_exec_kernel(int M, int N, int KB, int P, const typename Elt<FP8>::T *__restrict__ Bt, int64_t ldb,
-                     const void *__restrict__ blocks, const int32_t *__restrict__ slot,
-                     const float *__restrict__ rscale, const float *__restrict__ cscale,
-                     float *__restrict__ C, int64_t ldc, int ks, float *__restrict__ work, int vec_ok) {
+__device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, int KB, int P,
+                                          const typename Elt<FP8>::T *__restrict__ Bt, int64_t ldb,
+                                          const void *__restrict__ blocks, const float *__restrict__ rscale,
+                                          const float *__restrict__ cscale, float *__restrict__ C,
+                                          int64_t ldc, int ks, float *__restrict__ work, int vec_ok, int pg,
+                                          int nt, int z) {
     typedef typename Elt<FP8>::T T;
     typedef typename Elt<FP8>::AOp AOp;
     constexpr int PER = Elt<FP8>::kPerChunk;
@@ -241,11 +247,9 @@ __global__ void __launch_bounds__(64 * WP)
     constexpr int NTHR = 64 * WP;
     constexpr int CPT = (CHUNKS + NTHR - 1) / NTHR;
     constexpr int TN = NT / 16;
-    __shared__ __attribute__((aligned(16))) T tile[2][NT * LDK];
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int pg = blockIdx.x, nt = blockIdx.y, z = blockIdx.z;
     const int p0 = (pg * WP + wave) * PW;
     const int n0 = nt * NT;
     const int kb0 = (int)((int64_t)z * KB / ks), kb1 = (int)((int64_t)(z + 1) * KB / ks);
@@ -359,6 +363,42 @@ __global__ void __launch_bounds__(64 * WP)
     }
 }
 
+template <bool FP8, int NT, int WP, int PW>
+__global__ void __launch_bounds__(64 * WP)
+    mfma_exec_kernel(int M, int N, int KB, int P, const typename Elt<FP8>::T *__restrict__ Bt, int64_t ldb,
+                     const void *__restrict__ blocks, const float *__restrict__ rscale,
+                     const float *__restrict__ cscale, float *__restrict__ C, int64_t ldc, int ks,
+                     float *__restrict__ work, int vec_ok) {
+    __shared__ __attribute__((aligned(16))) TileBuf<FP8, NT> tile;
+    mfma_tile<FP8, NT, WP, PW>(tile, M, N, KB, P, Bt, ldb, blocks, rscale, cscale, C, ldc, ks, work, vec_ok,
+                               blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Grouped, persistent: the tiles of every problem in the table, problem q
+// owning global tiles [tile_begin[q], tile_begin[q + 1]); each workgroup
+// takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  One launch runs a
+// whole batch of independent SpMMs (requests, layers).  The problem count and
+// tile total come from device memory (hdr[0], hdr[1]) next to the table.
+template <bool FP8, int NT, int WP, int PW>
+__global__ void __launch_bounds__(64 * WP)
+    mfma_grouped_kernel(const MfmaProblem *__restrict__ probs, const int *__restrict__ hdr) {
+    __shared__ __attribute__((aligned(16))) TileBuf<FP8, NT> tile;
+    const int nprob = hdr[0], total_tiles = hdr[1];  // read from memory: a captured graph replays any batch
+    int q = 0;
+    for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        while (q + 1 < nprob && probs[q + 1].tile_begin <= t) ++q;  // tiles ascend per workgroup
+        const MfmaProblem &pr = probs[q];
+        const int local = t - pr.tile_begin;
+        const int ntiles = (pr.N + NT - 1) / NT;
+        mfma_tile<FP8, NT, WP, PW>(tile, pr.M, pr.N, pr.KB, pr.P,
+                                   static_cast<const typename Elt<FP8>::T *>(pr.Bt), pr.ldb, pr.blocks,
+                                   pr.rscale, pr.cscale, pr.C, pr.ldc, 1, nullptr, pr.vec_ok,
+                                   local / ntiles, local % ntiles, 0);
+        __syncthreads();  // the next tile rewrites the LDS buffers
+    }
+}
+
+
 // Adds the ks partial tiles in split order (and applies the FP8 scales).
 template <bool FP8>
 __global__ void split_reduce_kernel(int M, int N, int ks, const float *__restrict__ work,
@@ -450,8 +490,8 @@ static hipError_t launch_exec_t(const MfmaLaunch &a, hipStream_t s) {
     const int vec_ok =
         ((uintptr_t)a.B % 16 == 0 && (a.ldb * (int64_t)sizeof(typename Elt<FP8>::T)) % 16 == 0) ? 1 : 0;
     hipLaunchKernelGGL((mfma_exec_kernel<FP8, NT, WP, PW>), grid, dim3(64 * WP), 0, s, a.M, a.N, a.KB, a.P,
-                       static_cast<const typename Elt<FP8>::T *>(a.B), a.ldb, a.blocks, a.slot, a.rscale,
-                       a.cscale, a.C, a.ldc, a.ks, a.work, vec_ok);
+                       static_cast<const typename Elt<FP8>::T *>(a.B), a.ldb, a.blocks, a.rscale, a.cscale,
+                       a.C, a.ldc, a.ks, a.work, vec_ok);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.ks <= 1) return e;
     const int64_t n = (int64_t)a.M * a.N;
@@ -475,6 +515,83 @@ static hipError_t launch_exec_dt(const MfmaLaunch &a, hipStream_t s) {
         case 5: return launch_exec_t<FP8, 128, 4, 1>(a, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+template <bool FP8, int NT, int WP, int PW>
+static hipError_t launch_grouped_t(const MfmaProblem *probs, const int *hdr, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((mfma_grouped_kernel<FP8, NT, WP, PW>), dim3(grid), dim3(64 * WP), 0, s, probs, hdr);
+    return hipGetLastError();
+}
+
+template <bool FP8>
+static hipError_t launch_grouped_dt(int shape, const MfmaProblem *probs, const int *hdr, int grid, hipStream_t s) {
+    switch (shape) {
+        case 0: return launch_grouped_t<FP8, 64, 4, 2>(probs, hdr, grid, s);
+        case 1: return launch_grouped_t<FP8, 128, 4, 2>(probs, hdr, grid, s);
+        case 2: return launch_grouped_t<FP8, 64, 4, 1>(probs, hdr, grid, s);
+        case 3: return launch_grouped_t<FP8, 128, 2, 2>(probs, hdr, grid, s);
+        case 4: return launch_grouped_t<FP8, 32, 4, 1>(probs, hdr, grid, s);
+        case 5: return launch_grouped_t<FP8, 128, 4, 1>(probs, hdr, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_mfma_grouped(bool fp8, int shape, const MfmaProblem *probs, const int *hdr, int grid,
+                               hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
+    return fp8 ? launch_grouped_dt<true>(shape, probs, hdr, grid, s)
+               : launch_grouped_dt<false>(shape, probs, hdr, grid, s);
+}
+
+// Grouped B preparation: every problem's fp32 B (K x N, row-major) into its
+// transposed BF16 / e4m3 operand, 32 x 32 tiles, problem q owning tiles
+// [tile_begin, ...) of the table.
+template <bool FP8>
+__global__ void grouped_transpose_kernel(const BPrepProblem *__restrict__ probs, const int *__restrict__ hdr) {
+    __shared__ float tbuf[32][33];
+    const int nprob = hdr[0], total_tiles = hdr[1];
+    int q = 0;
+    for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        while (q + 1 < nprob && probs[q + 1].tile_begin <= t) ++q;
+        const BPrepProblem &pr = probs[q];
+        const int local = t - pr.tile_begin;
+        const int ktiles = (int)((pr.ldk + 31) / 32);
+        const int k0 = (local % ktiles) * 32, n0 = (local / ktiles) * 32;
+        const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+        for (int i = ty; i < 32; i += 8) {
+            const int k = k0 + i, n = n0 + tx;
+            float v = 0.0f;
+            if (k < pr.K && n < pr.N) {
+                if (pr.src_bf16)  // BF16 source: widen (exact)
+                    v = __builtin_bit_cast(float, (uint32_t)static_cast<const uint16_t *>(pr.src)[(int64_t)k * pr.ld_src + n] << 16);
+                else
+                    v = static_cast<const float *>(pr.src)[(int64_t)k * pr.ld_src + n];
+            }
+            tbuf[i][tx] = v;
+        }
+        __syncthreads();
+        for (int i = ty; i < 32; i += 8) {
+            const int n = n0 + i, k = k0 + tx;
+            if (n < pr.N && k < pr.ldk) {
+                const float v = tbuf[tx][i];
+                if constexpr (FP8)
+                    static_cast<uint8_t *>(pr.dst)[(int64_t)n * pr.ldk + k] =
+                        to_e4m3((float)((double)v / (double)pr.cscale[n]));
+                else
+                    static_cast<uint16_t *>(pr.dst)[(int64_t)n * pr.ldk + k] = to_bf16(v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_grouped_b_prep(bool fp8, const BPrepProblem *probs, const int *hdr, int grid, hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
+    if (fp8)
+        hipLaunchKernelGGL(grouped_transpose_kernel<true>, dim3(grid), dim3(256), 0, s, probs, hdr);
+    else
+        hipLaunchKernelGGL(grouped_transpose_kernel<false>, dim3(grid), dim3(256), 0, s, probs, hdr);
+    return hipGetLastError();
 }
 
 hipError_t launch_mfma_exec(const MfmaLaunch &a, hipStream_t s) {

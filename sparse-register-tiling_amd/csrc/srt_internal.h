@@ -183,6 +183,32 @@ hipError_t launch_mfma_fill(int m, int k, int KB, const void *Lp, const void *Li
 hipError_t launch_b_convert(int64_t rows, int cols, const float *src, int64_t ld_src, bool fp8,
                             float *cscale, void *dst, int64_t ldk, hipStream_t s);
 hipError_t launch_mfma_exec(const MfmaLaunch &a, hipStream_t s);
+// One problem of a grouped launch (layout shared with include/srt.h's
+// srt_mfma_problem).
+struct MfmaProblem {
+    const void *blocks;
+    const void *Bt;
+    float *C;
+    const float *rscale;
+    const float *cscale;
+    int64_t ldb, ldc;
+    int M, N, KB, P;
+    int tile_begin;  // first global tile of this problem
+    int vec_ok;      // Bt rows 16-byte aligned
+};
+struct BPrepProblem {
+    const void *src;  // fp32 or bf16 (src_bf16), K x N row-major, ld_src
+    void *dst;
+    const float *cscale;
+    int64_t ld_src, ldk;
+    int K, N;
+    int tile_begin;
+    int src_bf16;
+};
+// hdr (device): {number of problems, total tiles}.
+hipError_t launch_mfma_grouped(bool fp8, int shape, const MfmaProblem *probs, const int *hdr, int grid,
+                               hipStream_t s);
+hipError_t launch_grouped_b_prep(bool fp8, const BPrepProblem *probs, const int *hdr, int grid, hipStream_t s);
 int mfma_shape_tile(int shape, int *nt, int *wp, int *pw);
 
 // CSR baseline (csr_spmm.hip).

@@ -1,0 +1,376 @@
+"""Serving form: resident pre-packed sparse weights, continuous batching.
+
+No counterpart in the reference (its bindings run one SpMM per call,
+SpMM_SOP.h:178, template:307-392).  A ``Server`` keeps packed BF16 weights
+(``PanelPlan``, 16-row MFMA panels) resident in HBM and serves requests
+C = W . B (B: K x N BF16, any N) as they arrive:
+
+* admission by HBM budget: each admitted request holds its transposed B
+  operand and its FP32 C in one reserved slab (first-fit); requests wait
+  while the budget is full;
+* continuous batching: whenever one of the launch slots is free, every
+  admitted request not yet running (up to ``max_batch``) goes into one batch;
+  slots run on their own streams, so batches admitted while others are on
+  the GPU overlap on the device;
+* one batch = one replay of a HIP graph captured per slot: upload of the
+  problem tables (pinned host -> device), one grouped launch preparing every
+  request's B operand, one persistent grouped MFMA launch computing every
+  request's tiles.  Problem counts live in device memory, so the same graph
+  replays any batch;
+* latency = completion (HIP event on the slot's stream, relative to a
+  reference event) - arrival; p50 / p99 and throughput are reported.
+
+Every output tile is computed by one workgroup from the same data in the same
+order whatever else is in the batch, so a request's result is bit-identical
+alone and in any mix.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .precision import DTYPES, SHAPES, PanelPlan
+
+PROBLEM_DTYPE = np.dtype([("blocks", "<u8"), ("Bt", "<u8"), ("C", "<u8"), ("rscale", "<u8"), ("cscale", "<u8"),
+                          ("ldb", "<i8"), ("ldc", "<i8"), ("M", "<i4"), ("N", "<i4"), ("KB", "<i4"), ("P", "<i4"),
+                          ("tile_begin", "<i4"), ("vec_ok", "<i4")])
+BPREP_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("cscale", "<u8"), ("ld_src", "<i8"), ("ldk", "<i8"),
+                        ("K", "<i4"), ("N", "<i4"), ("tile_begin", "<i4"), ("src_bf16", "<i4")])
+assert PROBLEM_DTYPE.itemsize == 80 and BPREP_DTYPE.itemsize == 56
+
+
+class SlabPool:
+    """The HBM budget: one device allocation carved first-fit in 256-byte
+    units.  ``alloc`` returns None when the request does not fit."""
+
+    ALIGN = 256
+
+    def __init__(self, nbytes: int, device):
+        self.nbytes = int(nbytes) // self.ALIGN * self.ALIGN
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        self.free_list = [(0, self.nbytes)]  # sorted (offset, size)
+        self.used = 0
+
+    def alloc(self, nbytes: int) -> Optional[int]:
+        need = -(-int(nbytes) // self.ALIGN) * self.ALIGN
+        for i, (off, size) in enumerate(self.free_list):
+            if size >= need:
+                if size == need:
+                    self.free_list.pop(i)
+                else:
+                    self.free_list[i] = (off + need, size - need)
+                self.used += need
+                return off
+        return None
+
+    def free(self, off: int, nbytes: int) -> None:
+        need = -(-int(nbytes) // self.ALIGN) * self.ALIGN
+        self.used -= need
+        fl = self.free_list
+        fl.append((off, need))
+        fl.sort()
+        merged = [fl[0]]
+        for o, s in fl[1:]:
+            po, ps = merged[-1]
+            if po + ps == o:
+                merged[-1] = (po, ps + s)
+            else:
+                merged.append((o, s))
+        self.free_list = merged
+
+    def view(self, off: int, shape, dtype) -> torch.Tensor:
+        n = int(np.prod(shape)) * torch.tensor([], dtype=dtype).element_size()
+        return self.buf[off:off + n].view(dtype).view(*shape)
+
+
+@dataclass
+class Request:
+    rid: int
+    weight: int
+    B: torch.Tensor              # K x N BF16, row-major, on the device
+    arrival: float = 0.0         # seconds, relative to the server's clock
+    done: float = float("nan")
+    C: Optional[torch.Tensor] = None
+    _off: int = -1
+    _bytes: int = 0
+    _Bt: Optional[torch.Tensor] = None
+
+    @property
+    def N(self) -> int:
+        return self.B.shape[1]
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class _Slot:
+    def __init__(self, max_batch: int, device, dtype_code: int, shape: int, grid: int):
+        self.stream = torch.cuda.Stream(device=device)
+        self.h_prob = torch.zeros(max_batch * PROBLEM_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
+        self.h_prep = torch.zeros(max_batch * BPREP_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
+        self.h_hdr = torch.zeros(4, dtype=torch.int32).pin_memory()  # prep {n, tiles}, spmm {n, tiles}
+        self.d_prob = torch.zeros_like(self.h_prob, device=device)
+        self.d_prep = torch.zeros_like(self.h_prep, device=device)
+        self.d_hdr = torch.zeros(4, dtype=torch.int32, device=device)
+        self.prob = np.frombuffer(self.h_prob.numpy(), dtype=PROBLEM_DTYPE)
+        self.prep = np.frombuffer(self.h_prep.numpy(), dtype=BPREP_DTYPE)
+        self.hdr = self.h_hdr.numpy()
+        self.dtype_code, self.shape, self.grid = dtype_code, shape, grid
+        self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self.batch: List[Request] = []
+        self.end = torch.cuda.Event(enable_timing=True)
+
+    def _enqueue(self) -> None:
+        s = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.d_prob.copy_(self.h_prob, non_blocking=True)
+        self.d_prep.copy_(self.h_prep, non_blocking=True)
+        self.d_hdr.copy_(self.h_hdr, non_blocking=True)
+        _lib.call("srt_mfma_grouped_b_prep", self.dtype_code, ct.c_void_p(self.d_prep.data_ptr()),
+                  ct.c_void_p(self.d_hdr.data_ptr()), self.grid, s)
+        _lib.call("srt_mfma_grouped_execute", self.dtype_code, self.shape, ct.c_void_p(self.d_prob.data_ptr()),
+                  ct.c_void_p(self.d_hdr.data_ptr() + 8), self.grid, s)
+
+    def capture(self) -> None:
+        """Capture upload + grouped prep + grouped SpMM once; replays reuse it."""
+        self.hdr[:] = 0
+        with torch.cuda.stream(self.stream):
+            self._enqueue()  # warm-up outside capture (an empty batch)
+        self.stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            self._enqueue()
+        self.graph = g
+
+    def launch(self, use_graph: bool = True) -> None:
+        with torch.cuda.stream(self.stream):
+            if use_graph and self.graph is not None:
+                self.graph.replay()
+            else:
+                self._enqueue()
+            self.end.record(self.stream)
+
+
+class Server:
+    """Resident weights + admission + continuous batching (see module doc)."""
+
+    def __init__(self, weights: Sequence[PanelPlan], budget_bytes: int = 256 << 20, max_batch: int = 64,
+                 slots: int = 3, shape: int = 0, grid: int = 1024, use_graphs: bool = True):
+        if not weights:
+            raise ValueError("no weights")
+        dt = {w.dtype for w in weights}
+        if len(dt) != 1:
+            raise ValueError("all weights must share one dtype")
+        self.dtype = dt.pop()
+        self.weights = list(weights)
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self.pool = SlabPool(budget_bytes, self.device)
+        self.max_batch = int(max_batch)
+        self.shape = int(shape)
+        self.grid = int(grid)
+        self.use_graphs = use_graphs
+        code = DTYPES[self.dtype]
+        self.slots = [_Slot(self.max_batch, self.device, code, self.shape, self.grid) for _ in range(slots)]
+        if use_graphs:
+            for sl in self.slots:
+                sl.capture()
+        torch.cuda.synchronize()
+        self._prob_scratch = np.zeros(1, dtype=PROBLEM_DTYPE)
+        self.stats: Dict[str, float] = {}
+
+    # -- admission ------------------------------------------------------------
+    def request_bytes(self, req: Request) -> int:
+        w = self.weights[req.weight]
+        ldk = -(-w.k // 32) * 32
+        esz = 2 if self.dtype == "bf16" else 1
+        return -(-req.N * ldk * esz // 256) * 256 + w.m * req.N * 4
+
+    def admit(self, req: Request) -> bool:
+        """Reserve the request's operand and result in the budget (False = wait)."""
+        w = self.weights[req.weight]
+        if req.B.shape[0] != w.k or req.B.dtype != torch.bfloat16:
+            raise ValueError("request B must be K x N BF16 for weight %d" % req.weight)
+        nbytes = self.request_bytes(req)
+        off = self.pool.alloc(nbytes)
+        if off is None:
+            return False
+        ldk = -(-w.k // 32) * 32
+        bt_bytes = -(-req.N * ldk * 2 // 256) * 256
+        req._off, req._bytes = off, nbytes
+        req._Bt = self.pool.view(off, (req.N, ldk), torch.bfloat16)
+        req.C = self.pool.view(off + bt_bytes, (w.m, req.N), torch.float32)
+        return True
+
+    def release(self, req: Request, keep_result: bool = False, stream: Optional[torch.cuda.Stream] = None) -> None:
+        """Return the request's memory to the budget.  With keep_result the
+        result is first copied out on `stream` (and waited for), since the
+        freed region may be reused by the next batch on another stream."""
+        if keep_result and req.C is not None:
+            with torch.cuda.stream(stream or torch.cuda.current_stream()):
+                req.C = req.C.clone()
+            (stream or torch.cuda.current_stream()).synchronize()
+        else:
+            req.C = None
+        self.pool.free(req._off, req._bytes)
+        req._Bt = None
+
+    # -- batches ------------------------------------------------------------
+    def _fill(self, slot: _Slot, batch: List[Request]) -> None:
+        nt, wp, pw = SHAPES[self.shape]
+        tiles = prep_tiles = 0
+        for i, r in enumerate(batch):
+            w = self.weights[r.weight]
+            ldk = r._Bt.shape[1]
+            rec = slot.prob[i:i + 1]
+            _lib.call("srt_mfma_problem_fill", w._h, ct.c_void_p(r._Bt.data_ptr()), ldk, None,
+                      ct.c_void_p(r.C.data_ptr()), r.N, r.N, tiles, rec.ctypes.data_as(ct.c_void_p))
+            tiles += -(-w.m // (16 * wp * pw)) * -(-r.N // nt)
+            pr = slot.prep[i]
+            pr["src"] = r.B.data_ptr()
+            pr["dst"] = r._Bt.data_ptr()
+            pr["cscale"] = 0
+            pr["ld_src"] = r.B.stride(0)
+            pr["ldk"] = ldk
+            pr["K"] = w.k
+            pr["N"] = r.N
+            pr["tile_begin"] = prep_tiles
+            pr["src_bf16"] = 1
+            prep_tiles += -(-ldk // 32) * -(-r.N // 32)
+        slot.hdr[:] = (len(batch), prep_tiles, len(batch), tiles)
+
+    def run_batch(self, batch: List[Request], slot_index: int = 0) -> None:
+        """Admit, run and release one batch synchronously; results stay in
+        req.C.  Raises if the batch does not fit the budget."""
+        for i, r in enumerate(batch):
+            if not self.admit(r):
+                for q in batch[:i]:
+                    self.release(q)
+                raise MemoryError("batch does not fit the HBM budget")
+        slot = self.slots[slot_index]
+        self._fill(slot, batch)
+        slot.launch(self.use_graphs)
+        slot.stream.synchronize()
+        for r in batch:
+            self.release(r, keep_result=True, stream=slot.stream)
+
+    def serve(self, requests: List[Request], keep_results: bool = False) -> Dict[str, float]:
+        """Open-loop serving of requests (arrival times in seconds from start).
+        Returns latency percentiles and throughput; fills req.done (and
+        req.C with keep_results, which waits for each copy-out)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        ref = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ref.record()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nxt = 0
+        waiting: List[Request] = []
+        admitted: List[Request] = []
+        busy: Dict[int, List[Request]] = {}
+        done_count = 0
+        max_inflight = batches = 0
+        while done_count < len(reqs):
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                waiting.append(reqs[nxt])
+                nxt += 1
+            # completions
+            for si in list(busy):
+                sl = self.slots[si]
+                if sl.end.query():
+                    t_end = ref.elapsed_time(sl.end) * 1e-3
+                    for r in busy.pop(si):
+                        r.done = t_end
+                        self.release(r, keep_results, sl.stream)
+                        done_count += 1
+            # admission by budget, arrival order
+            while waiting and self.admit(waiting[0]):
+                admitted.append(waiting.pop(0))
+            if waiting and not busy and not admitted and self.pool.used == 0:
+                raise MemoryError("request %d needs %d bytes, more than the whole budget"
+                                  % (waiting[0].rid, self.request_bytes(waiting[0])))
+            # dispatch to free slots
+            for si, sl in enumerate(self.slots):
+                if not admitted:
+                    break
+                if si in busy:
+                    continue
+                batch, admitted = admitted[:self.max_batch], admitted[self.max_batch:]
+                self._fill(sl, batch)
+                sl.launch(self.use_graphs)
+                busy[si] = batch
+                batches += 1
+                max_inflight = max(max_inflight, len(busy))
+        lat = np.array([r.latency for r in reqs])
+        span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+        self.stats = {"requests": len(reqs), "batches": batches, "max_inflight_batches": max_inflight,
+                      "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                      "mean_ms": float(lat.mean() * 1e3), "req_per_s": len(reqs) / span if span > 0 else float("inf")}
+        return self.stats
+
+
+def serve_plain(plans: Sequence, requests: List[Request], Bs32: Sequence[torch.Tensor]) -> Dict[str, float]:
+    """The baseline: one plain FP32 launch per request (``MatMulSpecialized``
+    per weight), in arrival order on one stream, each as soon as it arrived."""
+    reqs = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
+    ref = torch.cuda.Event(enable_timing=True)
+    outs = [torch.empty((plans[requests[i].weight].m, requests[i].N), device="cuda") for i in range(len(requests))]
+    events = [torch.cuda.Event(enable_timing=True) for _ in requests]
+    torch.cuda.synchronize()
+    ref.record()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in reqs:
+        r = requests[i]
+        while time.perf_counter() - t0 < r.arrival:
+            pass
+        plans[r.weight](outs[i], Bs32[i])
+        events[i].record()
+    torch.cuda.synchronize()
+    for i in reqs:
+        requests[i].done = ref.elapsed_time(events[i]) * 1e-3
+        requests[i].C = outs[i]
+    lat = np.array([requests[i].latency for i in reqs])
+    span = max(r.done for r in requests) - min(r.arrival for r in requests)
+    return {"requests": len(requests), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
+            "req_per_s": len(requests) / span if span > 0 else float("inf")}
+
+
+# ----------------------------------------------------------------- workload
+SERVING_SHAPES = [(1024, 1024), (1024, 1024), (4096, 1024), (1024, 4096)]
+
+
+def serving_weights(layers: int = 12, sparsity: float = 0.9, seed: int = 0):
+    """48 weight matrices: per layer 1024x1024 (x2), 4096x1024, 1024x4096,
+    magnitude-pruned (log-normal row scales) to `sparsity`."""
+    from .workloads import magnitude_pruned
+    out = []
+    for layer in range(layers):
+        for j, (M, K) in enumerate(SERVING_SHAPES):
+            out.append(magnitude_pruned(M, K, sparsity, seed=seed + 97 * layer + j))
+    return out
+
+
+def poisson_requests(num: int, qps: float, weight_shapes, seed: int = 1, n_max: int = 512,
+                     device="cuda") -> List[Request]:
+    """Poisson arrivals at `qps`; each request picks a uniform random weight
+    and N = round(exp(U(0, ln n_max))) in [1, n_max]; B ~ U[-1, 1] in BF16."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / qps, num)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs = []
+    for i in range(num):
+        w = int(rng.integers(len(weight_shapes)))
+        n = int(min(n_max, max(1, round(math.exp(rng.uniform(0.0, math.log(n_max)))))))
+        K = weight_shapes[w][1]
+        B = torch.from_numpy(rng.uniform(-1.0, 1.0, (K, n)).astype(np.float32)).to(device).to(torch.bfloat16)
+        reqs.append(Request(rid=i, weight=w, B=B, arrival=float(arrivals[i])))
+    return reqs

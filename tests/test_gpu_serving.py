@@ -1,0 +1,94 @@
+"""GPU tests of the serving form: resident BF16 weights, admission by HBM
+budget, continuous batching into grouped persistent launches replayed from
+HIP graphs.
+
+Bar: every served result is within the FP32-accumulation bound of the FP64
+product of the BF16-quantised operands (oracle.precision_oracle), its
+normwise error against the FP64 product of the unquantised inputs is below
+1e-2, and each result is bit-identical alone and inside a mixed batch, with
+graphs and without.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import precision_oracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def env():
+    import importlib
+    import torch
+    torch.cuda.init()
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    shapes = [(256, 128), (128, 256), (300, 96), (64, 520)]
+    Ws = [srt.workloads.magnitude_pruned(M, K, 0.9, seed=i) for i, (M, K) in enumerate(shapes)]
+    plans = [srt.precision.PanelPlan.from_dense(torch.from_numpy(W), "bf16") for W in Ws]
+    return srt, torch, shapes, Ws, plans
+
+
+def _requests(srt, torch, shapes, num, qps, seed, n_max=96):
+    return srt.serving.poisson_requests(num, qps, shapes, seed=seed, n_max=n_max)
+
+
+def _check(req, W):
+    B = req.B.float().cpu().numpy()
+    C = req.C.cpu().numpy()
+    ref, bound = po.reference_f64(W, B, "bf16")
+    assert np.all(np.abs(C - ref) <= bound + 1e-30), "request %d" % req.rid
+    assert po.rel_error_vs_fp64(C, W, B) <= 1e-2
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_served_results_and_latency(env, graphs):
+    srt, torch, shapes, Ws, plans = env
+    _log("serving graphs=%s" % graphs)
+    server = srt.serving.Server(plans, budget_bytes=64 << 20, max_batch=8, slots=3, use_graphs=graphs, grid=256)
+    reqs = _requests(srt, torch, shapes, 40, qps=20000.0, seed=3)
+    st = server.serve(reqs, keep_results=True)
+    _log("  ", st)
+    assert st["requests"] == 40 and st["batches"] >= 5
+    for r in reqs:
+        assert r.done >= r.arrival
+        _check(r, Ws[r.weight])
+    assert server.pool.used == 0
+
+
+def test_alone_equals_mixed(env):
+    srt, torch, shapes, Ws, plans = env
+    _log("serving alone vs mixed")
+    server = srt.serving.Server(plans, budget_bytes=64 << 20, max_batch=16, slots=2, grid=64)
+    reqs = _requests(srt, torch, shapes, 12, qps=1e6, seed=5)
+    alone = []
+    for r in reqs:
+        server.run_batch([r])
+        alone.append(r.C.cpu().numpy())
+    mixed = [srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B) for r in reqs]
+    server.run_batch(list(reversed(mixed)), slot_index=1)
+    for r, a in zip(mixed, alone):
+        assert np.array_equal(r.C.cpu().numpy().view(np.uint32), a.view(np.uint32)), "request %d" % r.rid
+
+
+def test_admission_by_budget(env):
+    srt, torch, shapes, Ws, plans = env
+    _log("serving small budget")
+    probe = srt.serving.Request(rid=0, weight=2, B=torch.zeros((96, 96), dtype=torch.bfloat16, device="cuda"))
+    server = srt.serving.Server(plans, budget_bytes=256 << 10, max_batch=64, slots=2, grid=64)
+    assert server.request_bytes(probe) < server.pool.nbytes < 4 * server.request_bytes(probe)
+    reqs = _requests(srt, torch, shapes, 24, qps=1e6, seed=7)  # all arrive at once
+    st = server.serve(reqs, keep_results=True)
+    _log("  ", st)
+    assert st["batches"] > 3          # the budget forced several small batches
+    for r in reqs:
+        _check(r, Ws[r.weight])
+    with pytest.raises(MemoryError):
+        big = srt.serving.Request(rid=99, weight=3, B=torch.zeros((520, 512), dtype=torch.bfloat16, device="cuda"))
+        server.run_batch([big])
