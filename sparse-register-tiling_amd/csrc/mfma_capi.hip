@@ -177,6 +177,8 @@ int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const v
     int nt = 0, wp = 0, pw = 0;
     if (mfma_shape_tile(shape, &nt, &wp, &pw)) return fail(SRT_ERR_INVALID, "unknown launch shape");
     if (ldb < (int64_t)p->KB * 32) return fail(SRT_ERR_INVALID, "B (transposed, N x ldb) needs ldb >= K rounded up to 32");
+    if ((uintptr_t)B % 16 != 0 || (ldb * elt_bytes(p->dtype)) % 16 != 0)
+        return fail(SRT_ERR_INVALID, "B (transposed) rows must be 16-byte aligned (as srt_mfma_convert_b writes them)");
     const int ks = std::max(1, std::min(ksplit, std::max(p->KB, 1)));
     if (ks > 1 && (size_t)ks * p->M * n > p->work_floats)
         return fail(SRT_ERR_INVALID, "K split needs a larger reservation (srt_mfma_plan_reserve)");
@@ -242,6 +244,8 @@ int srt_mfma_problem_fill(const srt_mfma_plan *p, const void *Bt, int64_t ldb, c
     if (n < 0 || ldc < n || ldb < (int64_t)p->KB * 32) return fail(SRT_ERR_INVALID, "bad problem shape");
     if (n > 0 && (!Bt || !C)) return fail(SRT_ERR_INVALID, "null B or C");
     if (p->dtype == SRT_DT_FP8 && n > 0 && !col_scales) return fail(SRT_ERR_INVALID, "FP8 needs column scales");
+    if (n > 0 && ((uintptr_t)Bt % 16 != 0 || (ldb * elt_bytes(p->dtype)) % 16 != 0))
+        return fail(SRT_ERR_INVALID, "B (transposed) rows must be 16-byte aligned");
     out->blocks = p->blocks;
     out->Bt = Bt;
     out->C = C;

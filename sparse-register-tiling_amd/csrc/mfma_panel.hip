@@ -267,59 +267,64 @@ __device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, 
     constexpr int DEPTH = 4;
     uint4 breg[DEPTH][CPT];
     AOp areg[DEPTH][PW];
+    // Branch-free (the host guarantees 16-byte aligned rows): a lane past
+    // column N loads row 0 (valid) and its chunk is zeroed when stored, so
+    // no load sits in a branch or feeds a select (either would force a wait
+    // right after it and defeat the ring).
+    (void)vec_ok;
     auto load_b = [&](uint4 (&dst)[CPT], int kb) {
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
             const int ch = threadIdx.x + c * NTHR;
             const int n = ch / CPR, part = ch % CPR;
             const bool in = ch < CHUNKS && n0 + n < N;
-            const T *src = Bt + (int64_t)(n0 + n) * ldb + (int64_t)kb * kBlockK + part * PER;
-            if (in && vec_ok) {
-                dst[c] = *reinterpret_cast<const uint4 *>(src);
-            } else {
-                T v[PER];
-#pragma unroll
-                for (int e = 0; e < PER; ++e) v[e] = in ? src[e] : (T)0;
-                __builtin_memcpy(&dst[c], v, 16);
-            }
+            const T *src = Bt + (int64_t)(in ? n0 + n : 0) * ldb + (int64_t)kb * kBlockK + part * PER;
+            dst[c] = *reinterpret_cast<const uint4 *>(src);
         }
     };
+    // Loads are unconditional (panel and block indices clamped to valid
+    // ones; results past the panel count or the K range are never stored),
+    // so every path issues the same loads and the waits stay partial.
     auto load_a = [&](AOp (&dst)[PW], int kb) {
 #pragma unroll
         for (int i = 0; i < PW; ++i) {
-            const int p = p0 + i;
-            if (p < P) dst[i] = reinterpret_cast<const AOp *>(blocks)[((int64_t)p * KB + kb) * 64 + lane];
+            const int p = p0 + i < P ? p0 + i : P - 1;
+            dst[i] = reinterpret_cast<const AOp *>(blocks)[((int64_t)p * KB + kb) * 64 + lane];
         }
     };
+    const bool any = kb0 < kb1;  // uniform; an empty K range still stores zeros
 #pragma unroll
-    for (int j = 0; j < DEPTH; ++j) {
-        if (kb0 + j < kb1) {
-            load_b(breg[j], kb0 + j);
-            load_a(areg[j], kb0 + j);
-        }
+    for (int j = 0; j < DEPTH && any; ++j) {
+        const int kb = kb0 + j < kb1 ? kb0 + j : kb1 - 1;
+        load_b(breg[j], kb);
+        load_a(areg[j], kb);
     }
     int buf = 0;
     for (int base = kb0; base < kb1; base += DEPTH) {
 #pragma unroll
         for (int j = 0; j < DEPTH; ++j) {
             const int kb = base + j;
-            if (kb >= kb1) break;
+            const bool run = kb < kb1;  // uniform; every wave takes every barrier
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
                 const int ch = threadIdx.x + c * NTHR;
-                if (ch < CHUNKS)
-                    *reinterpret_cast<uint4 *>(&tile[buf][(ch / CPR) * LDK + (ch % CPR) * PER]) = breg[j][c];
+                if (ch < CHUNKS) {
+                    const bool in = n0 + ch / CPR < N;
+                    *reinterpret_cast<uint4 *>(&tile[buf][(ch / CPR) * LDK + (ch % CPR) * PER]) =
+                        in ? breg[j][c] : uint4{0u, 0u, 0u, 0u};
+                }
             }
             __syncthreads();
             AOp a[PW];
 #pragma unroll
             for (int i = 0; i < PW; ++i) a[i] = areg[j][i];
-            if (kb + DEPTH < kb1) {
-                load_b(breg[j], kb + DEPTH);
-                load_a(areg[j], kb + DEPTH);
+            {
+                const int kn = kb + DEPTH < kb1 ? kb + DEPTH : kb1 - 1;
+                load_b(breg[j], kn);
+                load_a(areg[j], kn);
             }
 #pragma unroll
-            for (int t = 0; t < TN; ++t) {
+            for (int t = 0; t < TN && run; ++t) {
                 const T *bp = &tile[buf][(t * 16 + col) * LDK + kg * 8];
                 if constexpr (FP8) {
                     const long bw = *reinterpret_cast<const long *>(bp);

@@ -6,8 +6,8 @@ SpMM_SOP.h:178, template:307-392).  A ``Server`` keeps packed BF16 weights
 C = W . B (B: K x N BF16, any N) as they arrive:
 
 * admission by HBM budget: each admitted request holds its transposed B
-  operand and its FP32 C in one reserved slab (first-fit); requests wait
-  while the budget is full;
+  operand and its FP32 C in one block of a reserved slab (power-of-two size
+  classes); requests wait while the budget is full;
 * continuous batching: whenever one of the launch slots is free, every
   admitted request not yet running (up to ``max_batch``) goes into one batch;
   slots run on their own streams, so batches admitted while others are on
@@ -29,8 +29,9 @@ from __future__ import annotations
 import ctypes as ct
 import math
 import time
-from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Sequence
+from dataclasses import dataclass
+from collections import deque
+from typing import Deque, Dict, List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -47,43 +48,53 @@ assert PROBLEM_DTYPE.itemsize == 80 and BPREP_DTYPE.itemsize == 56
 
 
 class SlabPool:
-    """The HBM budget: one device allocation carved first-fit in 256-byte
-    units.  ``alloc`` returns None when the request does not fit."""
+    """The HBM budget: one device allocation handed out in power-of-two
+    size classes (16 KiB .. the whole slab) with O(1) allocate and free:
+    fresh space is carved from the top, freed blocks go to their class's
+    list.  ``used`` counts class bytes of live blocks; ``alloc`` returns None
+    when the request does not fit.  With nothing live the slab is reset, so
+    free lists never strand the budget."""
 
-    ALIGN = 256
+    MIN_CLASS = 14
 
     def __init__(self, nbytes: int, device):
-        self.nbytes = int(nbytes) // self.ALIGN * self.ALIGN
+        self.nbytes = int(nbytes) >> self.MIN_CLASS << self.MIN_CLASS
         self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
-        self.free_list = [(0, self.nbytes)]  # sorted (offset, size)
+        self.base = self.buf.data_ptr()
+        self.free_lists: Dict[int, List[int]] = {}
+        self.top = 0
         self.used = 0
+        self.live = 0
+
+    @classmethod
+    def size_class(cls, nbytes: int) -> int:
+        return max(cls.MIN_CLASS, (int(nbytes) - 1).bit_length())
 
     def alloc(self, nbytes: int) -> Optional[int]:
-        need = -(-int(nbytes) // self.ALIGN) * self.ALIGN
-        for i, (off, size) in enumerate(self.free_list):
-            if size >= need:
-                if size == need:
-                    self.free_list.pop(i)
-                else:
-                    self.free_list[i] = (off + need, size - need)
-                self.used += need
-                return off
-        return None
+        c = self.size_class(nbytes)
+        size = 1 << c
+        if self.used + size > self.nbytes:
+            return None
+        fl = self.free_lists.get(c)
+        if fl:
+            off = fl.pop()
+        elif self.top + size <= self.nbytes:
+            off = self.top
+            self.top += size
+        else:
+            return None
+        self.used += size
+        self.live += 1
+        return off
 
     def free(self, off: int, nbytes: int) -> None:
-        need = -(-int(nbytes) // self.ALIGN) * self.ALIGN
-        self.used -= need
-        fl = self.free_list
-        fl.append((off, need))
-        fl.sort()
-        merged = [fl[0]]
-        for o, s in fl[1:]:
-            po, ps = merged[-1]
-            if po + ps == o:
-                merged[-1] = (po, ps + s)
-            else:
-                merged.append((o, s))
-        self.free_list = merged
+        c = self.size_class(nbytes)
+        self.used -= 1 << c
+        self.live -= 1
+        self.free_lists.setdefault(c, []).append(off)
+        if self.live == 0:
+            self.free_lists.clear()
+            self.top = 0
 
     def view(self, off: int, shape, dtype) -> torch.Tensor:
         n = int(np.prod(shape)) * torch.tensor([], dtype=dtype).element_size()
@@ -100,7 +111,8 @@ class Request:
     C: Optional[torch.Tensor] = None
     _off: int = -1
     _bytes: int = 0
-    _Bt: Optional[torch.Tensor] = None
+    _bt_off: int = 0   # slab offsets of the transposed operand and the result
+    _c_off: int = 0
 
     @property
     def N(self) -> int:
@@ -182,10 +194,23 @@ class Server:
             for sl in self.slots:
                 sl.capture()
         torch.cuda.synchronize()
-        self._prob_scratch = np.zeros(1, dtype=PROBLEM_DTYPE)
+        self._weight_tables()
         self.stats: Dict[str, float] = {}
 
     # -- admission ------------------------------------------------------------
+    def _weight_tables(self) -> None:
+        """Per-weight constants of the problem records (one C call per weight)."""
+        nt, wp, pw = SHAPES[self.shape]
+        n = len(self.weights)
+        self._wrec = np.zeros(n, dtype=PROBLEM_DTYPE)
+        self._wM = np.array([w.m for w in self.weights], dtype=np.int64)
+        self._wK = np.array([w.k for w in self.weights], dtype=np.int64)
+        self._wldk = -(-self._wK // 32) * 32
+        self._wrows = -(-self._wM // (16 * wp * pw))  # row tiles per weight
+        for i, w in enumerate(self.weights):
+            _lib.call("srt_mfma_problem_fill", w._h, ct.c_void_p(self.pool.base), int(self._wldk[i]), None,
+                      ct.c_void_p(self.pool.base), 1, 1, 0, self._wrec[i:i + 1].ctypes.data_as(ct.c_void_p))
+
     def request_bytes(self, req: Request) -> int:
         w = self.weights[req.weight]
         ldk = -(-w.k // 32) * 32
@@ -195,55 +220,65 @@ class Server:
     def admit(self, req: Request) -> bool:
         """Reserve the request's operand and result in the budget (False = wait)."""
         w = self.weights[req.weight]
-        if req.B.shape[0] != w.k or req.B.dtype != torch.bfloat16:
-            raise ValueError("request B must be K x N BF16 for weight %d" % req.weight)
+        if req.B.shape[0] != w.k or req.B.dtype != torch.bfloat16 or not req.B.is_contiguous():
+            raise ValueError("request B must be a contiguous K x N BF16 tensor for weight %d" % req.weight)
         nbytes = self.request_bytes(req)
         off = self.pool.alloc(nbytes)
         if off is None:
             return False
         ldk = -(-w.k // 32) * 32
-        bt_bytes = -(-req.N * ldk * 2 // 256) * 256
         req._off, req._bytes = off, nbytes
-        req._Bt = self.pool.view(off, (req.N, ldk), torch.bfloat16)
-        req.C = self.pool.view(off + bt_bytes, (w.m, req.N), torch.float32)
+        req._bt_off = off
+        req._c_off = off + -(-req.N * ldk * 2 // 256) * 256
         return True
+
+    def result_view(self, req: Request) -> torch.Tensor:
+        w = self.weights[req.weight]
+        return self.pool.view(req._c_off, (w.m, req.N), torch.float32)
 
     def release(self, req: Request, keep_result: bool = False, stream: Optional[torch.cuda.Stream] = None) -> None:
         """Return the request's memory to the budget.  With keep_result the
         result is first copied out on `stream` (and waited for), since the
         freed region may be reused by the next batch on another stream."""
-        if keep_result and req.C is not None:
+        if keep_result:
             with torch.cuda.stream(stream or torch.cuda.current_stream()):
-                req.C = req.C.clone()
+                req.C = self.result_view(req).clone()
             (stream or torch.cuda.current_stream()).synchronize()
-        else:
-            req.C = None
         self.pool.free(req._off, req._bytes)
-        req._Bt = None
 
     # -- batches ------------------------------------------------------------
     def _fill(self, slot: _Slot, batch: List[Request]) -> None:
+        """Problem tables of one batch, vectorised over its requests."""
         nt, wp, pw = SHAPES[self.shape]
-        tiles = prep_tiles = 0
-        for i, r in enumerate(batch):
-            w = self.weights[r.weight]
-            ldk = r._Bt.shape[1]
-            rec = slot.prob[i:i + 1]
-            _lib.call("srt_mfma_problem_fill", w._h, ct.c_void_p(r._Bt.data_ptr()), ldk, None,
-                      ct.c_void_p(r.C.data_ptr()), r.N, r.N, tiles, rec.ctypes.data_as(ct.c_void_p))
-            tiles += -(-w.m // (16 * wp * pw)) * -(-r.N // nt)
-            pr = slot.prep[i]
-            pr["src"] = r.B.data_ptr()
-            pr["dst"] = r._Bt.data_ptr()
-            pr["cscale"] = 0
-            pr["ld_src"] = r.B.stride(0)
-            pr["ldk"] = ldk
-            pr["K"] = w.k
-            pr["N"] = r.N
-            pr["tile_begin"] = prep_tiles
-            pr["src_bf16"] = 1
-            prep_tiles += -(-ldk // 32) * -(-r.N // 32)
-        slot.hdr[:] = (len(batch), prep_tiles, len(batch), tiles)
+        n = len(batch)
+        wi = np.fromiter((r.weight for r in batch), dtype=np.int64, count=n)
+        N = np.fromiter((r.N for r in batch), dtype=np.int64, count=n)
+        bt = self.pool.base + np.fromiter((r._bt_off for r in batch), dtype=np.int64, count=n)
+        cp = self.pool.base + np.fromiter((r._c_off for r in batch), dtype=np.int64, count=n)
+        bp = np.fromiter((r.B.data_ptr() for r in batch), dtype=np.int64, count=n)
+        ldk = self._wldk[wi]
+        tiles = self._wrows[wi] * -(-N // nt)
+        ptiles = (ldk // 32) * -(-N // 32)
+        pr = slot.prob[:n]
+        pr[:] = self._wrec[wi]
+        pr["Bt"] = bt
+        pr["C"] = cp
+        pr["ldb"] = ldk
+        pr["ldc"] = N
+        pr["N"] = N
+        pr["tile_begin"] = np.cumsum(tiles) - tiles
+        pr["vec_ok"] = 1
+        pp = slot.prep[:n]
+        pp["src"] = bp
+        pp["dst"] = bt
+        pp["cscale"] = 0
+        pp["ld_src"] = N
+        pp["ldk"] = ldk
+        pp["K"] = self._wK[wi]
+        pp["N"] = N
+        pp["tile_begin"] = np.cumsum(ptiles) - ptiles
+        pp["src_bf16"] = 1
+        slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()))
 
     def run_batch(self, batch: List[Request], slot_index: int = 0) -> None:
         """Admit, run and release one batch synchronously; results stay in
@@ -271,7 +306,7 @@ class Server:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         nxt = 0
-        waiting: List[Request] = []
+        waiting: Deque[Request] = deque()
         admitted: List[Request] = []
         busy: Dict[int, List[Request]] = {}
         done_count = 0
@@ -292,7 +327,7 @@ class Server:
                         done_count += 1
             # admission by budget, arrival order
             while waiting and self.admit(waiting[0]):
-                admitted.append(waiting.pop(0))
+                admitted.append(waiting.popleft())
             if waiting and not busy and not admitted and self.pool.used == 0:
                 raise MemoryError("request %d needs %d bytes, more than the whole budget"
                                   % (waiting[0].rid, self.request_bytes(waiting[0])))

@@ -102,7 +102,11 @@ def test_unaligned_b_and_empty(srt, torch_cuda, dtype):
     view = big[:, 1:97]           # 2- or 1-byte offset: not 16-byte aligned, odd row stride
     view.copy_(Bq)
     C = torch.empty((64, 50), device="cuda")
-    plan(C, view, t)
+    with pytest.raises(srt.SrtError):
+        plan(C, view, t)          # the kernel reads 16-byte rows: refused, not misread
+    wide = torch.zeros((50, 128), dtype=Bq.dtype, device="cuda")
+    wide[:, :96].copy_(Bq)        # a wider, aligned row stride works
+    plan(C, wide, t)
     torch.cuda.synchronize()
     _within(C.cpu().numpy(), A, B, dtype)
     Z = np.zeros((40, 30), dtype=np.float32)
