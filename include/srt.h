@@ -179,6 +179,31 @@ int srt_plan_copy_packed(const srt_plan *p, int32_t *counts, int64_t *col_ptr, i
 void srt_plan_destroy(srt_plan *p);
 
 /* ------------------------------------------------------------------------
+ * Grouped execution: several plans (the 1x1-conv layers of a network, each
+ * with its own B and C) as one kernel launch.  No counterpart in the
+ * reference, whose benchmark runs one MatMulSpecialized per matrix
+ * (cpp_testbed/src/spmm/nano/SpMM_SOP.h:178); every problem keeps the
+ * per-plan semantics of srt_plan_execute, bit for bit.
+ *
+ * The plans must share one executor (mapping) and waves_per_block and live
+ * on the current device.  Operands are bound at create / rebind (a device
+ * problem table is written and the stream synchronised); srt_group_execute
+ * only launches, so it can be captured in a graph and replayed.
+ * ---------------------------------------------------------------------- */
+typedef struct srt_group srt_group;
+
+int srt_group_create(const srt_plan *const *plans, int count, const float *const *B, const int64_t *ldb,
+                     float *const *C, const int64_t *ldc, const int *n, srt_stream_t stream, srt_group **out);
+/* Re-bind operands (same plans; shapes may change).  Not while a launch of
+ * the group is pending or a graph holding it may replay. */
+int srt_group_rebind(srt_group *g, const float *const *B, const int64_t *ldb, float *const *C,
+                     const int64_t *ldc, const int *n, srt_stream_t stream);
+int srt_group_execute(const srt_group *g, srt_stream_t stream);
+/* Lane width (floats per lane) of the launch, workgroups, LDS bytes. */
+int srt_group_info(const srt_group *g, int *vec_width, int *workgroups, int *lds_bytes);
+void srt_group_destroy(srt_group *g);
+
+/* ------------------------------------------------------------------------
  * SOPTile binding primitives (sbench/SOP/_C/pytorch_wrapper_intrin.cpp.template).
  * ---------------------------------------------------------------------- */
 

@@ -13,6 +13,8 @@ Public surface (each mirrors a reference interface):
   csr.executor, csr.spmm                                      (csr)
   precision.PanelPlan, precision.spmm  (BF16 / FP8 on MFMA 16-row panels)
   serving.Server, serving.Request      (resident weights, continuous batching)
+  cnn.CnnFp32, cnn.CnnPanels, cnn.GroupedPlans, cnn.GroupedPanels
+                                       (many layers in one grouped launch)
   SOPOrig, save_mapping, load_mapping, named_mapping, Mapping  (mapping)
 """
 from . import mapping, workloads
@@ -27,6 +29,6 @@ from .native import NativeMapping, executors, resolve_mapping
 from .matmul import MatMulSpecialized, SpMM_SOP, TileConfig, get_executor_id
 from .sop import (Acc, SOPModule, SOPTile, execute_tile, make_sop_module, panel_pattern_codes,
                   pattern_code, sort_and_encode_patterns)
-from . import csr, precision, serving
+from . import cnn, csr, precision, serving
 
 __version__ = "0.1.0"

@@ -90,6 +90,11 @@ SIGNATURES = {
     "srt_plan_stats_get": (_i, [_vp, ct.POINTER(PlanStatsC)]),
     "srt_plan_copy_packed": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "srt_plan_destroy": (None, [_vp]),
+    "srt_group_create": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "srt_group_rebind": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "srt_group_execute": (_i, [_vp, _vp]),
+    "srt_group_info": (_i, [_vp, _vp, _vp, _vp]),
+    "srt_group_destroy": (None, [_vp]),
     "srt_plan_split_counts": (_i, [_vp, _vp]),
     "srt_tile_split_counts": (_i, [_vp, _vp]),
     "srt_sort_and_encode_patterns": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),

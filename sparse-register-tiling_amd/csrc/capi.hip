@@ -843,6 +843,157 @@ void srt_plan_destroy(srt_plan *p) {
     delete p;
 }
 
+// ------------------------------------------------------------ grouped plans
+}  // extern "C"
+
+struct srt_group {
+    std::vector<const srt_plan *> plans;
+    const ExecutorDesc *exec = nullptr;
+    int waves = 0, V = 1, total_blocks = 0, xcd_remap = 1, device = 0;
+    size_t lds = 0;
+    ExecProblem *d_probs = nullptr;
+    std::vector<ExecProblem> h_probs;
+};
+
+// Problem table of a group for the given operands: the lane width is the
+// widest every problem allows (alignment, V | n, LDS of its reduction), each
+// problem gets n_wblocks * ceil(n / 64V) consecutive workgroups.
+static int group_bind(srt_group &g, const float *const *B, const int64_t *ldb, float *const *C, const int64_t *ldc,
+                      const int *n, hipStream_t s) {
+    if (!B || !ldb || !C || !ldc || !n) return fail(SRT_ERR_INVALID, "srt_group: null operand array");
+    const int cnt = (int)g.plans.size();
+    int V = 4;
+    for (int i = 0; i < cnt; ++i) V = std::min(V, g.plans[i]->cfg.vec_width);
+    for (int i = 0; i < cnt; ++i) {
+        const srt_plan &p = *g.plans[i];
+        if (n[i] < 0) return fail(SRT_ERR_INVALID, "srt_group: n must be >= 0");
+        if (n[i] > 0 && p.pk.M > 0) {
+            if (!B[i] || !C[i]) return fail(SRT_ERR_INVALID, "srt_group: null B or C");
+            if (ldb[i] < n[i] || ldc[i] < n[i]) return fail(SRT_ERR_INVALID, "srt_group: ldb and ldc must be >= n");
+            if ((int64_t)p.pk.K * ldb[i] * (int64_t)sizeof(float) > (int64_t)0xffffffffLL)
+                return fail(SRT_ERR_UNSUPPORTED, "srt_group: B larger than 4 GiB (K * ldb * 4 bytes)");
+            while (V > 1 && (n[i] % V || ldb[i] % V || ldc[i] % V || !aligned(B[i], 4 * V) || !aligned(C[i], 4 * V)))
+                V >>= 1;
+        }
+        while (V > 1 && !lds_fits(false, p.pk.K, g.waves, p.pk.M_r, V, p.sch.reduce)) V >>= 1;
+        if (!lds_fits(false, p.pk.K, g.waves, p.pk.M_r, V, p.sch.reduce))
+            return fail(SRT_ERR_UNSUPPORTED, "srt_group: chunk reduction needs more LDS than a workgroup has");
+    }
+    std::vector<ExecProblem> t;
+    int64_t total = 0;
+    size_t lds = 0;
+    for (int i = 0; i < cnt; ++i) {
+        const srt_plan &p = *g.plans[i];
+        if (n[i] == 0 || p.pk.M == 0 || p.sch.n_wblocks == 0) continue;
+        ExecProblem q{};
+        q.groups = p.pk.groups;
+        q.grp_ptr = p.pk.grp_ptr;
+        q.desc = p.sch.desc;
+        q.col_ptr = p.pk.col_ptr;
+        q.val_ptr = p.pk.val_ptr;
+        q.cols = p.pk.cols;
+        q.vals = p.pk.vals;
+        q.B = B[i];
+        q.C = C[i];
+        q.ldc = ldc[i];
+        q.b_bytes = (uint32_t)std::max<int64_t>((int64_t)p.pk.K * ldb[i] * (int64_t)sizeof(float), 4);
+        q.ldb = (uint32_t)ldb[i];
+        q.M = p.pk.M;
+        q.N = n[i];
+        q.K = p.pk.K;
+        q.n_wblocks = p.sch.n_wblocks;
+        q.reduce = p.sch.reduce;
+        q.load_c = 0;
+        q.block_begin = (int)total;
+        total += (int64_t)p.sch.n_wblocks * ((n[i] + 64 * V - 1) / (64 * V));
+        if (total > 0x7fffffffLL) return fail(SRT_ERR_UNSUPPORTED, "srt_group: more than 2^31 workgroups");
+        if (p.sch.reduce) lds = std::max(lds, reduce_lds(g.waves, p.pk.M_r, V));
+        t.push_back(q);
+    }
+    if (t.size() > g.h_probs.size() || !g.d_probs) {
+        ExecProblem *d = nullptr;
+        SRT_TRY(dmalloc(&d, std::max<size_t>(t.size(), 1)));
+        hipFree(g.d_probs);
+        g.d_probs = d;
+    }
+    if (!t.empty()) {
+        SRT_HIP_TRY(hipMemcpyAsync(g.d_probs, t.data(), sizeof(ExecProblem) * t.size(), hipMemcpyHostToDevice, s));
+        SRT_HIP_TRY(hipStreamSynchronize(s));
+    }
+    g.h_probs = std::move(t);
+    g.V = V;
+    g.total_blocks = (int)total;
+    g.lds = lds;
+    return SRT_OK;
+}
+
+extern "C" {
+
+int srt_group_create(const srt_plan *const *plans, int count, const float *const *B, const int64_t *ldb,
+                     float *const *C, const int64_t *ldc, const int *n, srt_stream_t stream, srt_group **out) {
+    if (!out || !plans) return fail(SRT_ERR_INVALID, "srt_group_create: null argument");
+    *out = nullptr;
+    if (count < 1) return fail(SRT_ERR_INVALID, "srt_group_create: count must be >= 1");
+    auto g = std::make_unique<srt_group>();
+    SRT_HIP_TRY(hipGetDevice(&g->device));
+    for (int i = 0; i < count; ++i) {
+        const srt_plan *p = plans[i];
+        if (!p) return fail(SRT_ERR_INVALID, "srt_group_create: null plan");
+        if (p->device != g->device) return fail(SRT_ERR_INVALID, "srt_group_create: plan belongs to another device");
+        if (i == 0) {
+            g->exec = p->exec;
+            g->waves = p->sch.waves_per_block;
+            g->xcd_remap = p->cfg.xcd_remap;
+        } else if (p->exec != g->exec) {
+            return fail(SRT_ERR_INVALID, "srt_group_create: plans use different executors (mappings)");
+        } else if (p->sch.waves_per_block != g->waves) {
+            return fail(SRT_ERR_INVALID, "srt_group_create: plans use different waves_per_block");
+        }
+        g->plans.push_back(p);
+    }
+    if (!g->exec->grouped[0]) return fail(SRT_ERR_UNSUPPORTED, "srt_group_create: executor has no grouped variant");
+    int rc = group_bind(*g, B, ldb, C, ldc, n, (hipStream_t)stream);
+    if (rc) {
+        hipFree(g->d_probs);
+        return rc;
+    }
+    *out = g.release();
+    return SRT_OK;
+}
+
+int srt_group_rebind(srt_group *g, const float *const *B, const int64_t *ldb, float *const *C, const int64_t *ldc,
+                     const int *n, srt_stream_t stream) {
+    if (!g) return fail(SRT_ERR_INVALID, "null group");
+    return group_bind(*g, B, ldb, C, ldc, n, (hipStream_t)stream);
+}
+
+int srt_group_execute(const srt_group *g, srt_stream_t stream) {
+    if (!g) return fail(SRT_ERR_INVALID, "null group");
+    int dev = -1;
+    SRT_HIP_TRY(hipGetDevice(&dev));
+    if (dev != g->device) return fail(SRT_ERR_INVALID, "group belongs to another device");
+    if (g->total_blocks == 0) return SRT_OK;
+    ExecGroupedFn fn = g->exec->grouped[log2_small(g->V)];
+    if (!fn) return fail(SRT_ERR_UNSUPPORTED, "grouped executor variant not generated");
+    SRT_HIP_TRY(fn(g->d_probs, (int)g->h_probs.size(), g->total_blocks, g->waves, g->lds, g->xcd_remap,
+                   (hipStream_t)stream));
+    return SRT_OK;
+}
+
+int srt_group_info(const srt_group *g, int *vec_width, int *workgroups, int *lds_bytes) {
+    if (!g) return fail(SRT_ERR_INVALID, "null group");
+    if (vec_width) *vec_width = g->V;
+    if (workgroups) *workgroups = g->total_blocks;
+    if (lds_bytes) *lds_bytes = (int)g->lds;
+    return SRT_OK;
+}
+
+void srt_group_destroy(srt_group *g) {
+    if (!g) return;
+    hipFree(g->d_probs);
+    delete g;
+}
+
 // ------------------------------------------------------------ SOPTile path
 int srt_sort_and_encode_patterns(const srt_mapping *m, const int32_t *pat_codes, int n,
                                  int32_t *encoded, int32_t *order, srt_stream_t stream) {

@@ -285,21 +285,20 @@ __device__ __forceinline__ void pipe_step(float (&acc)[MR][V], const Stage<V, U>
 // panel; all packed-column reads then hit LDS.  The partial tiles of the chunk
 // reduction reuse the same LDS after a barrier.  Every wave, idle slots
 // included, takes part in the copy and reaches every barrier.
+// The work of one (logical) workgroup b of one executor launch; shared by
+// the single-problem kernel and the grouped one.
 template <int MR, int V, int U, bool STAGE, class D>
-__global__ void __launch_bounds__(1024)
-    exec_kernel(const int2 *__restrict__ groups, const int64_t *__restrict__ grp_ptr,
-                const int4 *__restrict__ desc, const int64_t *__restrict__ col_ptr,
-                const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
-                const float *__restrict__ vals, const float *__restrict__ B, uint32_t b_bytes,
-                uint32_t ldb, float *__restrict__ C, int64_t ldc, int M, int N, int K, int n_wblocks,
-                int reduce, int load_c, int xcd_remap) {
+__device__ __forceinline__ void exec_body(float *lds, const int2 *__restrict__ groups,
+                                          const int64_t *__restrict__ grp_ptr, const int4 *__restrict__ desc,
+                                          const int64_t *__restrict__ col_ptr,
+                                          const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
+                                          const float *__restrict__ vals, const float *__restrict__ B,
+                                          uint32_t b_bytes, uint32_t ldb, float *__restrict__ C, int64_t ldc,
+                                          int M, int N, int K, int n_wblocks, int reduce, int load_c, int b) {
     typedef typename VecOf<V>::type vec;
-    extern __shared__ float lds[];  // STAGE: B tile [K][64*V]; reduction: [waves][MR][V][64]
     const int waves = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    int b = blockIdx.x;
-    if (xcd_remap) b = xcd_swizzle(b, gridDim.x);
     const int ntile = b / n_wblocks;
     const int sb = b - ntile * n_wblocks;
     const int4 *__restrict__ dw = desc + ((int64_t)sb * waves + wave) * 2;
@@ -453,6 +452,43 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
+template <int MR, int V, int U, bool STAGE, class D>
+__global__ void __launch_bounds__(1024)
+    exec_kernel(const int2 *__restrict__ groups, const int64_t *__restrict__ grp_ptr,
+                const int4 *__restrict__ desc, const int64_t *__restrict__ col_ptr,
+                const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
+                const float *__restrict__ vals, const float *__restrict__ B, uint32_t b_bytes,
+                uint32_t ldb, float *__restrict__ C, int64_t ldc, int M, int N, int K, int n_wblocks,
+                int reduce, int load_c, int xcd_remap) {
+    extern __shared__ float lds[];  // STAGE: B tile [K][64*V]; reduction: [waves][MR][V][64]
+    int b = blockIdx.x;
+    if (xcd_remap) b = xcd_swizzle(b, gridDim.x);
+    exec_body<MR, V, U, STAGE, D>(lds, groups, grp_ptr, desc, col_ptr, val_ptr, cols, vals, B, b_bytes, ldb, C,
+                                  ldc, M, N, K, n_wblocks, reduce, load_c, b);
+}
+
+// Grouped: several independent packed matrices (the layers of a network,
+// a batch of requests) in one launch.  Problem q owns logical workgroups
+// [block_begin[q], block_begin[q + 1]); all problems share the executor, V,
+// U and the workgroup size.
+template <int MR, int V, int U, bool STAGE, class D>
+__global__ void __launch_bounds__(1024)
+    exec_grouped_kernel(const ExecProblem *__restrict__ probs, int nprob, int xcd_remap) {
+    extern __shared__ float lds[];
+    int b = blockIdx.x;
+    if (xcd_remap) b = xcd_swizzle(b, gridDim.x);
+    int lo = 0, hi = nprob - 1;  // last problem with block_begin <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].block_begin <= b) lo = mid; else hi = mid - 1;
+    }
+    const ExecProblem &q = probs[lo];
+    exec_body<MR, V, U, STAGE, D>(lds, reinterpret_cast<const int2 *>(q.groups), q.grp_ptr,
+                                  reinterpret_cast<const int4 *>(q.desc), q.col_ptr, q.val_ptr, q.cols, q.vals,
+                                  q.B, q.b_bytes, q.ldb, q.C, q.ldc, q.M, q.N, q.K, q.n_wblocks, q.reduce,
+                                  q.load_c, b - q.block_begin);
+}
+
 // LDS bytes of the chunk reduction: one partial tile per wave slot.
 inline size_t reduce_lds_bytes(int MR, int V, int waves, int reduce) {
     return reduce ? (size_t)waves * MR * V * 64 * sizeof(float) : 0;
@@ -486,6 +522,25 @@ hipError_t launch_exec(const ExecLaunch &a, hipStream_t s) {
                        a.grp_ptr, reinterpret_cast<const int4 *>(a.desc), a.col_ptr, a.val_ptr, a.cols,
                        a.vals, a.B, a.b_bytes, (uint32_t)a.ldb, a.C, a.ldc, a.M, a.N, a.K, a.n_wblocks,
                        a.reduce, a.load_c, a.xcd_remap);
+    return hipGetLastError();
+}
+
+template <int MR, int V, int U, bool STAGE, class D>
+hipError_t launch_exec_grouped(const ExecProblem *d_probs, int nprob, int total_blocks, int waves, size_t lds,
+                               int xcd_remap, hipStream_t s) {
+    if (total_blocks <= 0 || nprob <= 0) return hipSuccess;
+    if (waves < 1 || waves > 16) return hipErrorInvalidConfiguration;
+    if (lds > 65536) {
+        static bool raised = false;
+        if (!raised) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&exec_grouped_kernel<MR, V, U, STAGE, D>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
+            if (e != hipSuccess) return e;
+            raised = true;
+        }
+    }
+    hipLaunchKernelGGL((exec_grouped_kernel<MR, V, U, STAGE, D>), dim3((unsigned)total_blocks), dim3(64 * waves),
+                       lds, s, d_probs, nprob, xcd_remap);
     return hipGetLastError();
 }
 

@@ -49,6 +49,26 @@ struct ExecLaunch {
 
 typedef hipError_t (*ExecLaunchFn)(const ExecLaunch &, hipStream_t);
 
+// One problem of a grouped executor launch (device table entry).
+struct ExecProblem {
+    const int32_t *groups;
+    const int64_t *grp_ptr;
+    const int32_t *desc;
+    const int64_t *col_ptr;
+    const int64_t *val_ptr;
+    const int32_t *cols;
+    const float *vals;
+    const float *B;
+    float *C;
+    int64_t ldc;
+    uint32_t b_bytes, ldb;
+    int M, N, K, n_wblocks, reduce, load_c;
+    int block_begin;  // first logical workgroup of this problem
+    int pad;
+};
+typedef hipError_t (*ExecGroupedFn)(const ExecProblem *d_probs, int nprob, int total_blocks, int waves,
+                                    size_t lds, int xcd_remap, hipStream_t s);
+
 // One generated executor (one pattern set).  launch[s][v][u]: s = 0 reads B
 // rows from global memory, s = 1 stages the B tile in LDS; V = 1,2,4 and
 // unroll = 1,2,4,8 (index log2); a null entry is a variant not generated.
@@ -58,6 +78,7 @@ struct ExecutorDesc {
     int num_patterns;
     const int32_t *patterns;  // id order
     ExecLaunchFn launch[2][3][4];
+    ExecGroupedFn grouped[3];  // global-memory B path, unroll 4, V = 1,2,4 (null: not generated)
 };
 
 // Largest dynamic LDS allocation of one workgroup on the current device.
