@@ -16,8 +16,9 @@ Public surface (each mirrors a reference interface):
   cnn.CnnFp32, cnn.CnnPanels, cnn.GroupedPlans, cnn.GroupedPanels
                                        (many layers in one grouped launch)
   SOPOrig, save_mapping, load_mapping, named_mapping, Mapping  (mapping)
+  matrix_io.read_smtx / read_mtx / read_filelist               (file formats)
 """
-from . import mapping, workloads
+from . import mapping, matrix_io, workloads
 from .mapping import (Mapping, SOPOrig, gen_all_combos, load_mapping, mapping_id, named_mapping,
                       read_mapping_file, save_mapping)
 

@@ -471,9 +471,15 @@ __global__ void __launch_bounds__(1024)
 // a batch of requests) in one launch.  Problem q owns logical workgroups
 // [block_begin[q], block_begin[q + 1]); all problems share the executor, V,
 // U and the workgroup size.
+// The table is read through the constant address space: pointers loaded
+// from there are known to address global memory, so the uniform metadata
+// reads of exec_body stay scalar loads (from a generic table they would
+// become vector loads with per-lane results).
+typedef const __attribute__((address_space(4))) ExecProblem *ConstProblems;
+
 template <int MR, int V, int U, bool STAGE, class D>
 __global__ void __launch_bounds__(1024)
-    exec_grouped_kernel(const ExecProblem *__restrict__ probs, int nprob, int xcd_remap) {
+    exec_grouped_kernel(ConstProblems probs, int nprob, int xcd_remap) {
     extern __shared__ float lds[];
     int b = blockIdx.x;
     if (xcd_remap) b = xcd_swizzle(b, gridDim.x);
@@ -482,11 +488,12 @@ __global__ void __launch_bounds__(1024)
         const int mid = (lo + hi + 1) >> 1;
         if (probs[mid].block_begin <= b) lo = mid; else hi = mid - 1;
     }
-    const ExecProblem &q = probs[lo];
-    exec_body<MR, V, U, STAGE, D>(lds, reinterpret_cast<const int2 *>(q.groups), q.grp_ptr,
-                                  reinterpret_cast<const int4 *>(q.desc), q.col_ptr, q.val_ptr, q.cols, q.vals,
-                                  q.B, q.b_bytes, q.ldb, q.C, q.ldc, q.M, q.N, q.K, q.n_wblocks, q.reduce,
-                                  q.load_c, b - q.block_begin);
+    exec_body<MR, V, U, STAGE, D>(lds, reinterpret_cast<const int2 *>(probs[lo].groups), probs[lo].grp_ptr,
+                                  reinterpret_cast<const int4 *>(probs[lo].desc), probs[lo].col_ptr,
+                                  probs[lo].val_ptr, probs[lo].cols, probs[lo].vals, probs[lo].B,
+                                  probs[lo].b_bytes, probs[lo].ldb, probs[lo].C, probs[lo].ldc, probs[lo].M,
+                                  probs[lo].N, probs[lo].K, probs[lo].n_wblocks, probs[lo].reduce,
+                                  probs[lo].load_c, b - probs[lo].block_begin);
 }
 
 // LDS bytes of the chunk reduction: one partial tile per wave slot.
@@ -540,7 +547,7 @@ hipError_t launch_exec_grouped(const ExecProblem *d_probs, int nprob, int total_
         }
     }
     hipLaunchKernelGGL((exec_grouped_kernel<MR, V, U, STAGE, D>), dim3((unsigned)total_blocks), dim3(64 * waves),
-                       lds, s, d_probs, nprob, xcd_remap);
+                       lds, s, (ConstProblems)d_probs, nprob, xcd_remap);
     return hipGetLastError();
 }
 

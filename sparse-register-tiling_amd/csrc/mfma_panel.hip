@@ -384,21 +384,28 @@ __global__ void __launch_bounds__(64 * WP)
 // takes tiles blockIdx.x, blockIdx.x + gridDim.x, ...  One launch runs a
 // whole batch of independent SpMMs (requests, layers).  The problem count and
 // tile total come from device memory (hdr[0], hdr[1]) next to the table.
+//
+// The tables are read through the constant address space: pointers loaded
+// from there are known to address global memory (from a generic table every
+// access through them would be a flat access).
+typedef const __attribute__((address_space(4))) MfmaProblem *ConstMfmaProblems;
+typedef const __attribute__((address_space(4))) BPrepProblem *ConstBPrepProblems;
+
 template <bool FP8, int NT, int WP, int PW>
 __global__ void __launch_bounds__(64 * WP)
-    mfma_grouped_kernel(const MfmaProblem *__restrict__ probs, const int *__restrict__ hdr) {
+    mfma_grouped_kernel(ConstMfmaProblems probs, const int *__restrict__ hdr) {
     __shared__ __attribute__((aligned(16))) TileBuf<FP8, NT> tile;
     const int nprob = hdr[0], total_tiles = hdr[1];  // read from memory: a captured graph replays any batch
     int q = 0;
     for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
         while (q + 1 < nprob && probs[q + 1].tile_begin <= t) ++q;  // tiles ascend per workgroup
-        const MfmaProblem &pr = probs[q];
-        const int local = t - pr.tile_begin;
-        const int ntiles = (pr.N + NT - 1) / NT;
-        mfma_tile<FP8, NT, WP, PW>(tile, pr.M, pr.N, pr.KB, pr.P,
-                                   static_cast<const typename Elt<FP8>::T *>(pr.Bt), pr.ldb, pr.blocks,
-                                   pr.rscale, pr.cscale, pr.C, pr.ldc, 1, nullptr, pr.vec_ok,
-                                   local / ntiles, local % ntiles, 0);
+        const int local = t - probs[q].tile_begin;
+        const int N = probs[q].N;
+        const int ntiles = (N + NT - 1) / NT;
+        mfma_tile<FP8, NT, WP, PW>(tile, probs[q].M, N, probs[q].KB, probs[q].P,
+                                   static_cast<const typename Elt<FP8>::T *>(probs[q].Bt), probs[q].ldb,
+                                   probs[q].blocks, probs[q].rscale, probs[q].cscale, probs[q].C, probs[q].ldc, 1,
+                                   nullptr, probs[q].vec_ok, local / ntiles, local % ntiles, 0);
         __syncthreads();  // the next tile rewrites the LDS buffers
     }
 }
@@ -524,7 +531,8 @@ static hipError_t launch_exec_dt(const MfmaLaunch &a, hipStream_t s) {
 
 template <bool FP8, int NT, int WP, int PW>
 static hipError_t launch_grouped_t(const MfmaProblem *probs, const int *hdr, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((mfma_grouped_kernel<FP8, NT, WP, PW>), dim3(grid), dim3(64 * WP), 0, s, probs, hdr);
+    hipLaunchKernelGGL((mfma_grouped_kernel<FP8, NT, WP, PW>), dim3(grid), dim3(64 * WP), 0, s,
+                       (ConstMfmaProblems)probs, hdr);
     return hipGetLastError();
 }
 
@@ -552,38 +560,42 @@ hipError_t launch_mfma_grouped(bool fp8, int shape, const MfmaProblem *probs, co
 // transposed BF16 / e4m3 operand, 32 x 32 tiles, problem q owning tiles
 // [tile_begin, ...) of the table.
 template <bool FP8>
-__global__ void grouped_transpose_kernel(const BPrepProblem *__restrict__ probs, const int *__restrict__ hdr) {
+__global__ void grouped_transpose_kernel(ConstBPrepProblems probs, const int *__restrict__ hdr) {
     __shared__ float tbuf[32][33];
     const int nprob = hdr[0], total_tiles = hdr[1];
     int q = 0;
     for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
         while (q + 1 < nprob && probs[q + 1].tile_begin <= t) ++q;
-        const BPrepProblem &pr = probs[q];
-        const int local = t - pr.tile_begin;
-        const int ktiles = (int)((pr.ldk + 31) / 32);
+        const void *src = probs[q].src;
+        void *dst = probs[q].dst;
+        const float *cscale = probs[q].cscale;
+        const int64_t ld_src = probs[q].ld_src, ldk = probs[q].ldk;
+        const int K = probs[q].K, N = probs[q].N, src_bf16 = probs[q].src_bf16;
+        const int local = t - probs[q].tile_begin;
+        const int ktiles = (int)((ldk + 31) / 32);
         const int k0 = (local % ktiles) * 32, n0 = (local / ktiles) * 32;
         const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
         for (int i = ty; i < 32; i += 8) {
             const int k = k0 + i, n = n0 + tx;
             float v = 0.0f;
-            if (k < pr.K && n < pr.N) {
-                if (pr.src_bf16)  // BF16 source: widen (exact)
-                    v = __builtin_bit_cast(float, (uint32_t)static_cast<const uint16_t *>(pr.src)[(int64_t)k * pr.ld_src + n] << 16);
+            if (k < K && n < N) {
+                if (src_bf16)  // BF16 source: widen (exact)
+                    v = __builtin_bit_cast(float, (uint32_t)static_cast<const uint16_t *>(src)[(int64_t)k * ld_src + n] << 16);
                 else
-                    v = static_cast<const float *>(pr.src)[(int64_t)k * pr.ld_src + n];
+                    v = static_cast<const float *>(src)[(int64_t)k * ld_src + n];
             }
             tbuf[i][tx] = v;
         }
         __syncthreads();
         for (int i = ty; i < 32; i += 8) {
             const int n = n0 + i, k = k0 + tx;
-            if (n < pr.N && k < pr.ldk) {
+            if (n < N && k < ldk) {
                 const float v = tbuf[tx][i];
                 if constexpr (FP8)
-                    static_cast<uint8_t *>(pr.dst)[(int64_t)n * pr.ldk + k] =
-                        to_e4m3((float)((double)v / (double)pr.cscale[n]));
+                    static_cast<uint8_t *>(dst)[(int64_t)n * ldk + k] =
+                        to_e4m3((float)((double)v / (double)cscale[n]));
                 else
-                    static_cast<uint16_t *>(pr.dst)[(int64_t)n * pr.ldk + k] = to_bf16(v);
+                    static_cast<uint16_t *>(dst)[(int64_t)n * ldk + k] = to_bf16(v);
             }
         }
         __syncthreads();
@@ -593,9 +605,10 @@ __global__ void grouped_transpose_kernel(const BPrepProblem *__restrict__ probs,
 hipError_t launch_grouped_b_prep(bool fp8, const BPrepProblem *probs, const int *hdr, int grid, hipStream_t s) {
     if (grid <= 0) return hipSuccess;
     if (fp8)
-        hipLaunchKernelGGL(grouped_transpose_kernel<true>, dim3(grid), dim3(256), 0, s, probs, hdr);
+        hipLaunchKernelGGL(grouped_transpose_kernel<true>, dim3(grid), dim3(256), 0, s, (ConstBPrepProblems)probs, hdr);
     else
-        hipLaunchKernelGGL(grouped_transpose_kernel<false>, dim3(grid), dim3(256), 0, s, probs, hdr);
+        hipLaunchKernelGGL(grouped_transpose_kernel<false>, dim3(grid), dim3(256), 0, s, (ConstBPrepProblems)probs,
+                           hdr);
     return hipGetLastError();
 }
 
