@@ -17,8 +17,9 @@ Public surface (each mirrors a reference interface):
                                        (many layers in one grouped launch)
   SOPOrig, save_mapping, load_mapping, named_mapping, Mapping  (mapping)
   matrix_io.read_smtx / read_mtx / read_filelist               (file formats)
+  pattern_select.select_patterns / specialized_mapping         (set-cover pattern sets)
 """
-from . import mapping, matrix_io, workloads
+from . import mapping, matrix_io, pattern_select, workloads
 from .mapping import (Mapping, SOPOrig, gen_all_combos, load_mapping, mapping_id, named_mapping,
                       read_mapping_file, save_mapping)
 

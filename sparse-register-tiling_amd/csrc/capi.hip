@@ -454,6 +454,12 @@ static int run_executor(const Packed &pk, const Schedule &sch, const ExecutorDes
     while (V > 1 && (n % V || ldb % V || ldc % V || !aligned(B, 4 * V) || !aligned(C, 4 * V)))
         V >>= 1;
     bool stage = cfg.b_staging == 1 && tile_lds(pk.K, 1) <= lds_limit();
+    if (stage) {  // executors built at run time may hold the global-memory path only
+        bool any = false;
+        for (int v = 0; v < 3; ++v)
+            for (int u = 0; u < 4; ++u) any = any || exec->launch[1][v][u] != nullptr;
+        stage = any;
+    }
     while (V > 1 && !lds_fits(stage, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce)) V >>= 1;
     if (stage && !lds_fits(true, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce)) stage = false;
     if (!lds_fits(stage, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce))

@@ -21,29 +21,35 @@ _loaded = {}
 
 
 def _compile(eid: str, sources) -> str:
+    """Compile the translation units in parallel, link against libsrt."""
+    import concurrent.futures as cf
     from importlib import util
     spec = util.spec_from_file_location("_srt_build_lib", os.path.join(PKG_DIR, "build_lib.py"))
     bl = util.module_from_spec(spec)
     spec.loader.exec_module(bl)
     os.makedirs(JIT_DIR, exist_ok=True)
-    srcs = []
+    objs = []
     for name, text in sorted(sources.items()):
         path = os.path.join(JIT_DIR, name)
         with open(path, "w") as f:
             f.write(text)
-        srcs.append(path)
+        objs.append((path, path + ".o"))
+    jobs = min(len(objs), os.cpu_count() or 4, 16)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda a: bl._compile(*a), objs))
     out = os.path.join(JIT_DIR, "libsrt_exec_%s.so" % eid)
-    cmd = [bl.HIPCC] + bl.CFLAGS + ["-shared"] + srcs + ["-o", out + ".tmp", "-L" + PKG_DIR, "-lsrt",
-                                                        "-Wl,-rpath," + PKG_DIR]
+    cmd = [bl.HIPCC, "-shared", "-fPIC", "--offload-arch=" + bl.ARCH] + [o for _, o in objs] + \
+        ["-o", out + ".tmp", "-L" + PKG_DIR, "-lsrt", "-Wl,-rpath," + PKG_DIR]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("executor build failed:\n%s\n%s" % (r.stdout, r.stderr))
+        raise RuntimeError("executor link failed:\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(out + ".tmp", out)
     return out
 
 
-def ensure_executor(native, regen: bool = False) -> str:
-    """Executor id for ``native``'s pattern set, building it if needed."""
+def ensure_executor(native, regen: bool = False, lean: bool = False) -> str:
+    """Executor id for ``native``'s pattern set, building it if needed
+    (`lean`: only the default launch variants, see codegen)."""
     try:
         if not regen:
             return native.executor_id()
@@ -53,7 +59,7 @@ def ensure_executor(native, regen: bool = False) -> str:
     eid = executor_id(native.M_r, native.patterns)
     path = os.path.join(JIT_DIR, "libsrt_exec_%s.so" % eid)
     if regen or not os.path.exists(path):
-        path = _compile(eid, gen_executor_sources(native.M_r, native.patterns))
+        path = _compile(eid, gen_executor_sources(native.M_r, native.patterns, lean=lean))
     if path not in _loaded:
         _lib.lib()
         _loaded[path] = ct.CDLL(path)
