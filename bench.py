@@ -64,13 +64,14 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--form", default="plain", choices=["plain", "precision", "serving", "cnn"],
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "serving", "cnn", "sweep"],
                     help="plain = the headline FP32 workload (default); the others print their own line")
     ap.add_argument("--qps", default="5000,10000,20000,40000,80000,160000",
                     help="serving: offered loads (requests/s)")
     ap.add_argument("--requests", type=int, default=1500, help="serving: requests per load level")
     ap.add_argument("--slo-ms", type=float, default=2.0, help="serving: p99 target for the throughput ratio")
     ap.add_argument("--cnn-batches", default="1,4,8", help="cnn: images per batch (N = H*W*batch)")
+    ap.add_argument("--sweep-cpu-seconds", type=float, default=0.3, help="sweep: host time per case")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--mapping", default="auto", help="M_r:name (e.g. 8:identity) or auto")
@@ -377,6 +378,69 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sweep(args):
+    """The 48 pruned-FFN workloads (A in {512x512, 2048x512, 512x2048,
+    1024x4096} magnitude-pruned to 70/80/90/95 %, N in {128, 256, 512}):
+    the register-tiled executor (best of the M_r = 4 and 8 "orig" mappings
+    at their default launch), the HIP CSR kernel, and the oracle's C
+    executor on all host threads, each graph- or wall-timed on one resident
+    operand set.  Every GPU result is checked against the CPU result."""
+    import numpy as np
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    at = importlib.import_module("sparse-register-tiling_amd.autotune")
+    from oracle import cbind
+    wl = srt.workloads
+    nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    maps = {(4, "orig"): srt.named_mapping(4, "orig"), (8, "orig"): srt.named_mapping(8, "orig")}
+    rows = []
+    for case in wl.gpu_cases():
+        A, B = wl.case_inputs(case, seed=0)
+        nnz = int(np.count_nonzero(A))
+        flop = 2.0 * nnz * case.N
+        bytes_alg = wl.bytes_alg(nnz, case.M, case.K, case.N)
+        csr_t = torch.from_numpy(A).to_sparse_csr()
+        Bd = torch.from_numpy(B).cuda()
+        C = torch.empty((case.M, case.N), device="cuda")
+        best = None
+        for key, mp in maps.items():
+            plan = srt.MatMulSpecialized.from_csr(csr_t, case.N, mapping_id=mp)
+            t = at.time_graph(lambda: plan(C, Bd), reps=50)
+            if best is None or t < best[0]:
+                best = (t, key, plan.get_config())
+            plan.close()
+        plan = srt.MatMulSpecialized.from_csr(csr_t, case.N, mapping_id=maps[best[1]])
+        plan(C, Bd)
+        parts = srt.csr._parts(csr_t)
+        Ccsr = torch.empty_like(C)
+        t_csr = at.time_graph(lambda: srt.csr.spmm(csr_t, Bd, Ccsr, _parts_cache=parts), reps=50)
+        torch.cuda.synchronize()
+        pk = cbind.pack_dense(A, best[1][0], maps[best[1]].table)
+        ref = cbind.exec_packed(pk, B, nsplit=plan.split_counts())
+        exact = bool(np.array_equal(C.cpu().numpy().view(np.uint32), ref.view(np.uint32)))
+        csr_err = float(np.abs(Ccsr.cpu().numpy() - ref).max())
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 3 or time.perf_counter() - t0 < args.sweep_cpu_seconds:
+            cbind.exec_packed(pk, B, nthreads=nthreads)
+            reps += 1
+        t_cpu = (time.perf_counter() - t0) / reps
+        plan.close()
+        r = {"M": case.M, "K": case.K, "sparsity": case.sparsity, "N": case.N, "nnz": nnz,
+             "mapping": "%d:%s" % best[1], "vec_width": best[2].vec_width,
+             "exec_us": round(best[0] * 1e6, 2), "exec_gflops": round(flop / best[0] / 1e9, 1),
+             "exec_hbm_frac": round(bytes_alg / best[0] / 8e12, 4),
+             "csr_us": round(t_csr * 1e6, 2), "csr_gflops": round(flop / t_csr / 1e9, 1),
+             "cpu_gflops": round(flop / t_cpu / 1e9, 1), "bit_exact_vs_oracle": exact,
+             "csr_max_abs_diff": csr_err}
+        rows.append(r)
+        log(r)
+    print(json.dumps({"form": "sweep", "metric": "SpMM GFLOP/s (2·nnz·N/s) and %HBM roofline, executor vs HIP "
+                      "CSR vs host C executor", "unit": "GFLOP/s", "n_gpus": 1, "data": "synthetic",
+                      "timing": "graph-replayed, one resident operand set (warm caches)",
+                      "cpu_threads": nthreads, "results": rows}), flush=True)
+
+
 def _rotated_time(make_launch, set_bytes, reps=20):
     """Graph-timed seconds per launch, cycling through enough operand sets
     that the replayed working set exceeds twice the 256 MB last-level cache
@@ -526,6 +590,8 @@ def main():
         return bench_serving(args)
     if args.form == "cnn":
         return bench_cnn(args)
+    if args.form == "sweep":
+        return bench_sweep(args)
     import numpy as np
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")
