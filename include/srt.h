@@ -58,7 +58,10 @@ int srt_version(void);
  * ---------------------------------------------------------------------- */
 typedef struct srt_tile_config {
     int M_c;               /* rows per workgroup (M_r * panels per workgroup)        */
-    int K_c;               /* K tile; realised as K (no K split on the GPU path)      */
+    int K_c;               /* K tile: 0 < K_c < K packs every panel as K tiles of K_c
+                              columns, one after another (count_nkerns.cpp:52-126), and
+                              the chain runs tile by tile as the reference's K-tiled
+                              execution with load_c does; otherwise one tile of K      */
     int N_c;               /* columns per wavefront (64 * vec_width)                  */
     int tiling_strategy;   /* reference enum: 0 manual, 1 CAKE, 2 CAKE+TLB (recorded) */
     float beta;            /* CAKE tiling beta (recorded; not a BLAS beta)            */
@@ -84,7 +87,8 @@ typedef struct srt_tile_config {
                               packed column; 0 = choose (global memory)          */
 } srt_tile_config;
 
-/* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54). */
+/* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54,
+ * except K_c = 0: one K tile; a caller passing the reference's 256 gets K tiles). */
 void srt_tile_config_default(srt_tile_config *cfg);
 
 /* ------------------------------------------------------------------------
@@ -167,11 +171,13 @@ typedef struct srt_plan_stats {
     int64_t packed_cols;    /* packed columns over all panels                */
     int64_t packed_values;  /* stored values incl. padding                   */
     int64_t padding;        /* stored values equal to 0.0f (template:197-201) */
+    int k_tile;             /* columns per K tile (K_c; K when untiled)       */
+    int k_tiles;            /* K tiles per panel                              */
 } srt_plan_stats;
 int srt_plan_stats_get(const srt_plan *p, srt_plan_stats *out);
 
 /* Copy the packed arrays to host memory (for inspection / parity checks):
- * counts[num_panels*num_patterns], col_ptr[num_panels+1], val_ptr[num_panels+1],
+ * counts[num_panels*k_tiles*num_patterns] (per panel, per K tile), col_ptr[num_panels+1], val_ptr[num_panels+1],
  * cols[packed_cols], vals[packed_values].  Any pointer may be NULL. */
 int srt_plan_copy_packed(const srt_plan *p, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
                          int32_t *cols, float *vals);

@@ -45,6 +45,10 @@ def lib():
     L.srt_oracle_exec.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, _i32p, _i32p, _i64p, _i64p,
                                   _i32p, _f32p, _f32p, ct.c_longlong, _f32p, ct.c_longlong,
                                   ct.c_int, _i32p, ct.c_int]
+    L.srt_oracle_exec_kt.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _i32p, _i32p, _i64p,
+                                     _i64p, _i32p, _f32p, _f32p, ct.c_longlong, _f32p, ct.c_longlong,
+                                     ct.c_int, _i32p, ct.c_int]
+    L.srt_oracle_exec_kt.restype = ct.c_int
     L.srt_oracle_isa.restype = ct.c_int
     L.srt_oracle_exec.restype = ct.c_int
     L.srt_oracle_csr.argtypes = [ct.c_int, ct.c_int, _i64p, _i32p, _f32p, _f32p, ct.c_longlong,
@@ -70,6 +74,37 @@ def build_lut(M_r: int, table: Dict[int, List[int]], patterns: List[int]):
                     ids.append(enc(int(p)))
     ptr[2 ** M_r] = len(ids)
     return ptr, np.array(ids if ids else [0], dtype=np.int32)
+
+
+def pack_dense_ktiled(A: np.ndarray, M_r: int, table: Dict[int, List[int]], k_tile: int):
+    """K-tiled packing (count_nkerns.cpp:52-126: A cut into K_c-column tiles,
+    each panel packed per tile): every K slice is packed on its own with its
+    column offset and the slices' packed columns are laid one after another
+    per panel; counts become [P, KT, NK]."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    M, K = A.shape
+    if not (0 < k_tile < K):
+        return pack_dense(A, M_r, table)
+    slices = [pack_dense(A[:, k0:min(K, k0 + k_tile)], M_r, table, col_offset=k0) for k0 in range(0, K, k_tile)]
+    P, NK = slices[0]["P"], slices[0]["NK"]
+    cols, vals = [], []
+    col_ptr = np.zeros(P + 1, dtype=np.int64)
+    val_ptr = np.zeros(P + 1, dtype=np.int64)
+    for p in range(P):
+        for sl in slices:
+            cols.append(sl["cols"][sl["col_ptr"][p]:sl["col_ptr"][p + 1]])
+            vals.append(sl["vals"][sl["val_ptr"][p]:sl["val_ptr"][p + 1]])
+        col_ptr[p + 1] = col_ptr[p] + sum(sl["col_ptr"][p + 1] - sl["col_ptr"][p] for sl in slices)
+        val_ptr[p + 1] = val_ptr[p] + sum(sl["val_ptr"][p + 1] - sl["val_ptr"][p] for sl in slices)
+    counts = np.stack([sl["counts"] for sl in slices], axis=1) if NK else np.zeros((P, len(slices), 0), np.int32)
+    return {
+        "M": M, "K": K, "M_r": M_r, "P": P, "NK": NK, "KT": len(slices), "patterns": slices[0]["patterns"],
+        "counts": np.ascontiguousarray(counts, dtype=np.int32),
+        "col_ptr": col_ptr, "val_ptr": val_ptr,
+        "cols": np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32),
+        "vals": np.concatenate(vals).astype(np.float32) if vals else np.zeros(0, np.float32),
+        "padding": sum(sl["padding"] for sl in slices),
+    }
 
 
 def pack_dense(A: np.ndarray, M_r: int, table: Dict[int, List[int]], col_offset: int = 0):
@@ -119,10 +154,10 @@ def exec_packed(packed, B: np.ndarray, nthreads: int = 0, C: np.ndarray | None =
     cols = packed["cols"] if packed["cols"].size else np.zeros(1, dtype=np.int32)
     vals = packed["vals"] if packed["vals"].size else np.zeros(1, dtype=np.float32)
     id_pat = np.array(packed["patterns"] or [0], dtype=np.int32)
-    rc = lib().srt_oracle_exec(M, N, M_r, NK, id_pat, counts, packed["col_ptr"], packed["val_ptr"],
-                               np.ascontiguousarray(cols, dtype=np.int32),
-                               np.ascontiguousarray(vals, dtype=np.float32), B, B.shape[1], C, N,
-                               1 if load_c else 0, kp, nthreads)
+    rc = lib().srt_oracle_exec_kt(M, N, M_r, NK, packed.get("KT", 1), id_pat, counts, packed["col_ptr"],
+                                  packed["val_ptr"], np.ascontiguousarray(cols, dtype=np.int32),
+                                  np.ascontiguousarray(vals, dtype=np.float32), B, B.shape[1], C, N,
+                                  1 if load_c else 0, kp, nthreads)
     if rc:
         raise MemoryError("oracle executor failed (%d)" % rc)
     return C

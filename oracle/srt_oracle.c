@@ -152,10 +152,10 @@ long long srt_oracle_pack_fill(int M, int K, const float *A, long long lda, int 
  * chosen by CPU feature at first use; fmaf is exact in every variant.
  */
 #define SRT_EXEC_PANEL(NAME, ATTR)                                                               \
-    ATTR static void NAME(int M, int N, int M_r, int NK, const int *id_pattern, const int *cnt,  \
-                          const int *cols, const float *vals, const float *B, long long ldb,     \
-                          float *C, long long ldc, int load_c, int p, int nsplit, long long ncols, \
-                          float *acc) {                                                          \
+    ATTR static void NAME(int M, int N, int M_r, int NK, int KT, const int *id_pattern,          \
+                          const int *cnt, const int *cols, const float *vals, const float *B,    \
+                          long long ldb, float *C, long long ldc, int load_c, int p, int nsplit, \
+                          long long ncols, float *acc) {                                         \
         for (int w = 0; w < nsplit; ++w)                                                         \
             for (int r = 0; r < M_r; ++r) {                                                      \
                 int row = p * M_r + r;                                                           \
@@ -168,9 +168,10 @@ long long srt_oracle_pack_fill(int M, int K, const float *A, long long lda, int 
         long long c = 0, v = 0;                                                                  \
         int w = 0;                                                                               \
         long long w_end = ncols / nsplit;                                                        \
-        for (int id = 0; id < NK; ++id) {                                                        \
+        for (int t = 0; t < KT * NK; ++t) {                                                      \
+            const int id = t % NK;                                                               \
             unsigned pattern = (unsigned)id_pattern[id];                                         \
-            for (int i = 0; i < cnt[id]; ++i, ++c) {                                             \
+            for (int i = 0; i < cnt[t]; ++i, ++c) {                                              \
                 while (c >= w_end && w < nsplit - 1) {                                           \
                     ++w;                                                                         \
                     w_end = (long long)(w + 1) * ncols / nsplit;                                 \
@@ -214,7 +215,7 @@ SRT_CSR_ROW(csr_row_avx512, __attribute__((target("avx512f,avx512vl,fma"))))
 SRT_CSR_ROW(csr_row_avx2, __attribute__((target("avx2,fma"))))
 SRT_CSR_ROW(csr_row_base, )
 
-typedef void (*exec_panel_fn)(int, int, int, int, const int *, const int *, const int *,
+typedef void (*exec_panel_fn)(int, int, int, int, int, const int *, const int *, const int *,
                               const float *, const float *, long long, float *, long long, int, int,
                               int, long long, float *);
 typedef void (*csr_row_fn)(int, const long long *, const int *, const float *, const float *,
@@ -232,12 +233,15 @@ static int cpu_level(void) {
 /* Which inner-loop variant runs on this host: 2 AVX-512, 1 AVX2+FMA, 0 base. */
 SRT_ORACLE_API int srt_oracle_isa(void) { return cpu_level(); }
 
-/* kp[p] = number of chunks panel p is split into (NULL: one chunk each). */
+/* kp[p] = number of chunks panel p is split into (NULL: one chunk each).
+ * KT = K tiles per panel: counts[(p*KT + t)*NK + id] holds tile t's counts and
+ * the panel's packed columns are its tiles' columns one tile after another
+ * (the K-tiled packing of count_nkerns.cpp:52-126 with load_c chaining). */
 SRT_ORACLE_API
-int srt_oracle_exec(int M, int N, int M_r, int NK, const int *id_pattern,
-                    const int *counts, const long long *col_ptr, const long long *val_ptr,
-                    const int *cols, const float *vals, const float *B, long long ldb,
-                    float *C, long long ldc, int load_c, const int *kp, int nthreads) {
+int srt_oracle_exec_kt(int M, int N, int M_r, int NK, int KT, const int *id_pattern,
+                       const int *counts, const long long *col_ptr, const long long *val_ptr,
+                       const int *cols, const float *vals, const float *B, long long ldb,
+                       float *C, long long ldc, int load_c, const int *kp, int nthreads) {
     int P = (M + M_r - 1) / M_r;
     if (nthreads <= 0) nthreads = omp_get_max_threads();
     int kmax = 1;
@@ -258,13 +262,22 @@ int srt_oracle_exec(int M, int N, int M_r, int NK, const int *id_pattern,
         } else {
 #pragma omp for schedule(static)
             for (int p = 0; p < P; ++p)
-                fn(M, N, M_r, NK, id_pattern, counts + (long long)p * NK, cols + col_ptr[p],
+                fn(M, N, M_r, NK, KT, id_pattern, counts + (long long)p * KT * NK, cols + col_ptr[p],
                    vals + val_ptr[p], B, ldb, C, ldc, load_c, p, kp ? kp[p] : 1,
                    col_ptr[p + 1] - col_ptr[p], acc);
             free(acc);
         }
     }
     return err ? -1 : 0;
+}
+
+SRT_ORACLE_API
+int srt_oracle_exec(int M, int N, int M_r, int NK, const int *id_pattern,
+                    const int *counts, const long long *col_ptr, const long long *val_ptr,
+                    const int *cols, const float *vals, const float *B, long long ldb,
+                    float *C, long long ldc, int load_c, const int *kp, int nthreads) {
+    return srt_oracle_exec_kt(M, N, M_r, NK, 1, id_pattern, counts, col_ptr, val_ptr, cols, vals, B, ldb,
+                              C, ldc, load_c, kp, nthreads);
 }
 
 SRT_ORACLE_API

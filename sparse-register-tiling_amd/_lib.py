@@ -58,7 +58,7 @@ class PlanStatsC(ct.Structure):
     _fields_ = [
         ("M", ct.c_int), ("K", ct.c_int), ("M_r", ct.c_int), ("num_panels", ct.c_int),
         ("num_patterns", ct.c_int), ("nnz", ct.c_int64), ("packed_cols", ct.c_int64),
-        ("packed_values", ct.c_int64), ("padding", ct.c_int64),
+        ("packed_values", ct.c_int64), ("padding", ct.c_int64), ("k_tile", ct.c_int), ("k_tiles", ct.c_int),
     ]
 
 

@@ -273,7 +273,7 @@ void srt_tile_config_default(srt_tile_config *c) {
     if (!c) return;
     // SOPConfig defaults (cpp_testbed/src/spmm/nano/SpMM_SOP.h:33-54).
     c->M_c = 16;
-    c->K_c = 256;
+    c->K_c = 0;  /* 0: one K tile (the reference's SOPConfig default is 256) */
     c->N_c = 64;
     c->tiling_strategy = 0;
     c->beta = 1.0f;
@@ -406,7 +406,7 @@ static bool lds_fits(bool stage, int K, int waves, int M_r, int V, bool reduce) 
 // `total_cols` packed columns and B width n (the choices the reference's
 // heuristic JSON made per sparsity bucket are made here from the launch
 // geometry; the autotuner overrides them with measured choices).
-static void realise_config(srt_tile_config &c, int M_r, int K, int64_t total_cols, int n) {
+static void realise_config(srt_tile_config &c, int M_r, int K, int K_tile, int64_t total_cols, int n) {
     if (c.vec_width != 1 && c.vec_width != 2 && c.vec_width != 4)
         c.vec_width = (n % 4 == 0 && n >= 256) ? 4 : (n % 2 == 0 && n >= 128) ? 2 : 1;
     if (c.waves_per_block < 1 || c.waves_per_block > kMaxWaves) c.waves_per_block = 8;
@@ -435,7 +435,7 @@ static void realise_config(srt_tile_config &c, int M_r, int K, int64_t total_col
     if (!pow2_upto8(c.unroll)) c.unroll = 4;
     c.M_c = M_r * c.waves_per_block;
     c.N_c = 64 * c.vec_width;
-    c.K_c = K;
+    c.K_c = K_tile;
 }
 
 static bool aligned(const void *p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
@@ -534,19 +534,31 @@ struct Scratch {
 // Per-panel lists of non-empty groups (id, count) from pk.counts, the id ->
 // pattern table, and a host copy of the column offsets.
 static int build_groups(Packed &pk, const int32_t *d_id_pattern, hipStream_t s) {
+    // Group lists per (panel, K tile) segment, laid out in segment order, so
+    // the list of panel p is the concatenation of its K tiles' lists; the
+    // panel-level offsets are every KT-th segment offset.
+    const int Q = pk.P * pk.KT;
     Scratch sc;
-    int64_t *ng = nullptr;
-    SRT_TRY(sc.get(&ng, pk.P));
+    int64_t *ng = nullptr, *seg_ptr = nullptr;
+    SRT_TRY(sc.get(&ng, Q));
     SRT_TRY(dmalloc(&pk.grp_ptr, (size_t)pk.P + 1));
-    SRT_HIP_TRY(launch_group_count(pk.P, pk.NK, pk.counts, ng, s));
-    SRT_HIP_TRY(launch_exclusive_scan(pk.P, ng, pk.grp_ptr, s));
+    int64_t *gp = pk.grp_ptr;
+    if (pk.KT > 1) {
+        SRT_TRY(sc.get(&seg_ptr, (size_t)Q + 1));
+        gp = seg_ptr;
+    }
+    SRT_HIP_TRY(launch_group_count(Q, pk.NK, pk.counts, ng, s));
+    SRT_HIP_TRY(launch_exclusive_scan(Q, ng, gp, s));
     int64_t tot = 0;
-    SRT_HIP_TRY(hipMemcpyAsync(&tot, pk.grp_ptr + pk.P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&tot, gp + Q, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     SRT_HIP_TRY(hipStreamSynchronize(s));
     pk.total_groups = tot;
     SRT_TRY(dmalloc(&pk.groups, 2 * ((size_t)tot + SRT_GROUP_PAD)));
     SRT_HIP_TRY(hipMemsetAsync(pk.groups + 2 * tot, 0, sizeof(int32_t) * 2 * SRT_GROUP_PAD, s));
-    SRT_HIP_TRY(launch_group_fill(pk.P, pk.NK, pk.counts, pk.grp_ptr, d_id_pattern, pk.groups, s));
+    SRT_HIP_TRY(launch_group_fill(Q, pk.NK, pk.counts, gp, d_id_pattern, pk.groups, s));
+    if (pk.KT > 1)
+        SRT_HIP_TRY(hipMemcpy2DAsync(pk.grp_ptr, sizeof(int64_t), seg_ptr, sizeof(int64_t) * pk.KT,
+                                     sizeof(int64_t), (size_t)pk.P + 1, hipMemcpyDeviceToDevice, s));
     SRT_TRY(dmalloc(&pk.id_pattern, (size_t)std::max(pk.NK, 1)));
     if (pk.NK)
         SRT_HIP_TRY(hipMemcpyAsync(pk.id_pattern, d_id_pattern, sizeof(int32_t) * pk.NK,
@@ -632,7 +644,7 @@ static int alloc_packed_arrays(Packed &pk, hipStream_t s) {
 
 // Packs an m x k CSR matrix (device arrays) into pk.  Mirrors the packing
 // done by the MatMulSpecialized constructor (SpMM_SOP.h:117-124).
-static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, const float *Lx, const void *Lp,
+static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, int K_c, const float *Lx, const void *Lp,
                     const void *Li, int index_bytes, hipStream_t s) {
     DeviceLut *lut = nullptr;
     SRT_TRY(map.device_lut(&lut));
@@ -657,38 +669,54 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, const float *Lx,
     const int M_r = map.M_r;
     const int NK = (int)map.patterns.size();
     const int P = (m + M_r - 1) / M_r;
+    // K tiling (the reference's K_c tiles, count_nkerns.cpp:52-126): panel p
+    // is packed as KT segments of K_c columns, segment after segment, so its
+    // chain runs tile by tile exactly as K-tiled execution with load_c would.
+    const int Kc = (K_c >= 1 && K_c < k) ? K_c : std::max(k, 1);
+    const int KT = k > 0 ? (k + Kc - 1) / Kc : 1;
+    const int Q = P * KT;
+    const int64_t ld = (int64_t)KT * Kc;  // padded row of the staging arrays
     pk.M = m;
     pk.K = k;
     pk.M_r = M_r;
     pk.P = P;
     pk.NK = NK;
+    pk.KT = KT;
+    pk.K_c = k > 0 ? Kc : 0;
     Scratch sc;
     uint32_t *codes = nullptr;
     float *dense = nullptr;
     int *bad = nullptr;
-    int64_t *ncols = nullptr, *nvals = nullptr, *pad = nullptr;
-    const size_t pk_cells = (size_t)P * (size_t)k;
+    int64_t *ncols = nullptr, *nvals = nullptr, *pad = nullptr, *seg_col = nullptr, *seg_val = nullptr;
+    const size_t pk_cells = (size_t)P * (size_t)ld;
     SRT_TRY(sc.get(&codes, pk_cells));
     SRT_TRY(sc.get(&dense, pk_cells * M_r));
     SRT_TRY(sc.get(&bad, 1));
-    SRT_TRY(sc.get(&ncols, P));
-    SRT_TRY(sc.get(&nvals, P));
-    SRT_TRY(sc.get(&pad, P));
+    SRT_TRY(sc.get(&ncols, Q));
+    SRT_TRY(sc.get(&nvals, Q));
+    SRT_TRY(sc.get(&pad, Q));
     SRT_HIP_TRY(hipMemsetAsync(codes, 0, sizeof(uint32_t) * pk_cells, s));
     SRT_HIP_TRY(hipMemsetAsync(dense, 0, sizeof(float) * pk_cells * M_r, s));
     SRT_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), s));
-    SRT_TRY(dmalloc(&pk.counts, (size_t)P * NK));
+    SRT_TRY(dmalloc(&pk.counts, (size_t)Q * NK));
     SRT_TRY(dmalloc(&pk.col_ptr, (size_t)P + 1));
     SRT_TRY(dmalloc(&pk.val_ptr, (size_t)P + 1));
-    SRT_HIP_TRY(launch_scatter_csr(m, k, M_r, Lx, Lp, Li, index_bytes, codes, dense, bad, s));
-    SRT_HIP_TRY(launch_count_codes(P, k, *lut, codes, pk.counts, ncols, nvals, s));
-    SRT_HIP_TRY(launch_exclusive_scan(P, ncols, pk.col_ptr, s));
-    SRT_HIP_TRY(launch_exclusive_scan(P, nvals, pk.val_ptr, s));
+    if (KT > 1) {
+        SRT_TRY(sc.get(&seg_col, (size_t)Q + 1));
+        SRT_TRY(sc.get(&seg_val, (size_t)Q + 1));
+    } else {
+        seg_col = pk.col_ptr;
+        seg_val = pk.val_ptr;
+    }
+    SRT_HIP_TRY(launch_scatter_csr(m, k, ld, M_r, Lx, Lp, Li, index_bytes, codes, dense, bad, s));
+    SRT_HIP_TRY(launch_count_codes(Q, Kc, *lut, codes, pk.counts, ncols, nvals, s));
+    SRT_HIP_TRY(launch_exclusive_scan(Q, ncols, seg_col, s));
+    SRT_HIP_TRY(launch_exclusive_scan(Q, nvals, seg_val, s));
     int h_bad = 0;
     int64_t tot_c = 0, tot_v = 0;
     SRT_HIP_TRY(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-    SRT_HIP_TRY(hipMemcpyAsync(&tot_c, pk.col_ptr + P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    SRT_HIP_TRY(hipMemcpyAsync(&tot_v, pk.val_ptr + P, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&tot_c, seg_col + Q, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&tot_v, seg_val + Q, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     // nnz from the row pointer
     int64_t lp0 = 0, lpm = 0;
     if (index_bytes == 8) {
@@ -708,10 +736,16 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, const float *Lx,
     pk.total_cols = tot_c;
     pk.total_vals = tot_v;
     SRT_TRY(alloc_packed_arrays(pk, s));
-    SRT_HIP_TRY(launch_fill_codes(P, k, 0, *lut, codes, dense, pk.col_ptr, pk.val_ptr, pk.counts,
+    SRT_HIP_TRY(launch_fill_codes(Q, Kc, 0, KT, *lut, codes, dense, seg_col, seg_val, pk.counts,
                                   pk.cols, pk.vals, pad, s));
-    std::vector<int64_t> h_pad(P);
-    if (P) SRT_HIP_TRY(hipMemcpyAsync(h_pad.data(), pad, sizeof(int64_t) * P, hipMemcpyDeviceToHost, s));
+    if (KT > 1) {  // panel offsets = every KT-th segment offset
+        SRT_HIP_TRY(hipMemcpy2DAsync(pk.col_ptr, sizeof(int64_t), seg_col, sizeof(int64_t) * KT,
+                                     sizeof(int64_t), (size_t)P + 1, hipMemcpyDeviceToDevice, s));
+        SRT_HIP_TRY(hipMemcpy2DAsync(pk.val_ptr, sizeof(int64_t), seg_val, sizeof(int64_t) * KT,
+                                     sizeof(int64_t), (size_t)P + 1, hipMemcpyDeviceToDevice, s));
+    }
+    std::vector<int64_t> h_pad(Q);
+    if (Q) SRT_HIP_TRY(hipMemcpyAsync(h_pad.data(), pad, sizeof(int64_t) * Q, hipMemcpyDeviceToHost, s));
     SRT_HIP_TRY(hipStreamSynchronize(s));
     pk.padding = 0;
     for (int64_t v : h_pad) pk.padding += v;
@@ -744,7 +778,8 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
         return fail(SRT_ERR_INVALID, "executor and mapping pattern sets differ");
     auto plan = std::make_unique<srt_plan>();
     SRT_HIP_TRY(hipGetDevice(&plan->device));
-    int rc = pack_csr(plan->pk, map, m, k, Lx, Lp, Li, index_bytes, (hipStream_t)stream);
+    const int K_c = cfg ? cfg->K_c : 0;
+    int rc = pack_csr(plan->pk, map, m, k, K_c, Lx, Lp, Li, index_bytes, (hipStream_t)stream);
     if (rc) {
         plan->pk.release();
         return rc;
@@ -754,7 +789,7 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
     else srt_tile_config_default(&plan->cfg_user);
     plan->n_max = n;
     plan->cfg = plan->cfg_user;
-    realise_config(plan->cfg, map.M_r, k, plan->pk.total_cols, std::max(n, 1));
+    realise_config(plan->cfg, map.M_r, k, plan->pk.K_c, plan->pk.total_cols, std::max(n, 1));
     rc = build_schedule(plan->pk, plan->cfg, plan->sch, (hipStream_t)stream);
     if (rc) {
         plan->pk.release();
@@ -788,7 +823,7 @@ int srt_plan_allocate_executor(srt_plan *p, int n) {
     // Re-realise the choices that depend on the B width and rebuild the
     // schedule if they changed.  Not to be called while the plan executes.
     srt_tile_config c = p->cfg_user;
-    realise_config(c, p->pk.M_r, p->pk.K, p->pk.total_cols, std::max(n, 1));
+    realise_config(c, p->pk.M_r, p->pk.K, p->pk.K_c, p->pk.total_cols, std::max(n, 1));
     if (!same_schedule(c, p->cfg)) SRT_TRY(build_schedule(p->pk, c, p->sch, nullptr));
     p->cfg = c;
     return SRT_OK;
@@ -820,13 +855,15 @@ int srt_plan_stats_get(const srt_plan *p, srt_plan_stats *o) {
     o->packed_cols = p->pk.total_cols;
     o->packed_values = p->pk.total_vals;
     o->padding = p->pk.padding;
+    o->k_tile = p->pk.K_c;
+    o->k_tiles = p->pk.KT;
     return SRT_OK;
 }
 
 static int copy_packed(const Packed &pk, int32_t *counts, int64_t *col_ptr, int64_t *val_ptr,
                        int32_t *cols, float *vals) {
     if (counts && pk.P && pk.NK)
-        SRT_HIP_TRY(hipMemcpy(counts, pk.counts, sizeof(int32_t) * pk.P * pk.NK, hipMemcpyDeviceToHost));
+        SRT_HIP_TRY(hipMemcpy(counts, pk.counts, sizeof(int32_t) * pk.P * pk.KT * pk.NK, hipMemcpyDeviceToHost));
     if (col_ptr) SRT_HIP_TRY(hipMemcpy(col_ptr, pk.col_ptr, sizeof(int64_t) * (pk.P + 1), hipMemcpyDeviceToHost));
     if (val_ptr) SRT_HIP_TRY(hipMemcpy(val_ptr, pk.val_ptr, sizeof(int64_t) * (pk.P + 1), hipMemcpyDeviceToHost));
     if (cols && pk.total_cols)
@@ -1193,7 +1230,7 @@ int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, fl
     srt_tile_config c;
     if (cfg) c = *cfg;
     else srt_tile_config_default(&c);
-    realise_config(c, t->pk.M_r, t->pk.K, t->pk.total_cols, std::max(N_c, 1));
+    realise_config(c, t->pk.M_r, t->pk.K, t->pk.K_c ? t->pk.K_c : t->pk.K, t->pk.total_cols, std::max(N_c, 1));
     std::lock_guard<std::mutex> g(t->mu);
     if (!t->sch.desc || !same_schedule(c, t->sch_cfg)) {
         SRT_TRY(build_schedule(t->pk, c, t->sch, (hipStream_t)stream));

@@ -31,7 +31,7 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 template <typename I>
-__global__ void scatter_csr_kernel(int m, int k, int M_r, const float *__restrict__ Lx,
+__global__ void scatter_csr_kernel(int m, int k, int64_t ld, int M_r, const float *__restrict__ Lx,
                                    const I *__restrict__ Lp, const I *__restrict__ Li,
                                    uint32_t *__restrict__ codes, float *__restrict__ dense,
                                    int *__restrict__ bad) {
@@ -46,7 +46,7 @@ __global__ void scatter_csr_kernel(int m, int k, int M_r, const float *__restric
             atomicOr(bad, 1);
             continue;
         }
-        const int64_t pc = (int64_t)p * k + (int64_t)col;
+        const int64_t pc = (int64_t)p * ld + (int64_t)col;
         atomicOr(&codes[pc], 1u << r);
         dense[pc * M_r + r] = Lx[q];
     }
@@ -191,7 +191,7 @@ __device__ __forceinline__ int64_t wave_sum(int64_t v) {
 
 // One 64-lane workgroup per panel.
 __global__ void __launch_bounds__(64)
-    fill_codes_kernel(int K, int col_offset, int NK, int M_r, const int32_t *__restrict__ lut_ptr,
+    fill_codes_kernel(int K, int col_offset, int seg, int NK, int M_r, const int32_t *__restrict__ lut_ptr,
                       const int32_t *__restrict__ lut_ids, const int32_t *__restrict__ id_pattern,
                       const uint32_t *__restrict__ codes, const float *__restrict__ dense,
                       const int64_t *__restrict__ col_ptr, const int64_t *__restrict__ val_ptr,
@@ -205,6 +205,9 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     const int64_t cbase = col_ptr[p], vbase = val_ptr[p];
     const uint32_t *pcodes = codes + (int64_t)p * K;
+    // Segment p covers columns [(p % seg) * K, (p % seg + 1) * K) of its row
+    // panel (seg K tiles per panel; seg = 1: the whole row).
+    col_offset += (p % seg) * K;
     int64_t zeros = 0;
     for (int chunk = 0; chunk < K; chunk += 64) {
         const int c = chunk + lane;
@@ -451,17 +454,17 @@ __global__ void __launch_bounds__(64)
 
 }  // namespace
 
-hipError_t launch_scatter_csr(int m, int k, int M_r, const float *Lx, const void *Lp,
+hipError_t launch_scatter_csr(int m, int k, int64_t ld, int M_r, const float *Lx, const void *Lp,
                               const void *Li, int index_bytes, uint32_t *codes, float *dense,
                               int *bad, hipStream_t s) {
     if (m <= 0) return hipSuccess;
     const int waves = 4;
     const dim3 grid((m + waves - 1) / waves), block(64 * waves);
     if (index_bytes == 8)
-        hipLaunchKernelGGL(scatter_csr_kernel<int64_t>, grid, block, 0, s, m, k, M_r, Lx,
+        hipLaunchKernelGGL(scatter_csr_kernel<int64_t>, grid, block, 0, s, m, k, ld, M_r, Lx,
                            (const int64_t *)Lp, (const int64_t *)Li, codes, dense, bad);
     else
-        hipLaunchKernelGGL(scatter_csr_kernel<int32_t>, grid, block, 0, s, m, k, M_r, Lx,
+        hipLaunchKernelGGL(scatter_csr_kernel<int32_t>, grid, block, 0, s, m, k, ld, M_r, Lx,
                            (const int32_t *)Lp, (const int32_t *)Li, codes, dense, bad);
     return hipGetLastError();
 }
@@ -475,13 +478,13 @@ hipError_t launch_count_codes(int P, int K, const DeviceLut &lut, const uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_fill_codes(int P, int K, int col_offset, const DeviceLut &lut,
+hipError_t launch_fill_codes(int P, int K, int col_offset, int seg, const DeviceLut &lut,
                              const uint32_t *codes, const float *dense, const int64_t *col_ptr,
                              const int64_t *val_ptr, const int32_t *counts, int32_t *cols,
                              float *vals, int64_t *padding, hipStream_t s) {
     if (P <= 0) return hipSuccess;
     hipLaunchKernelGGL(fill_codes_kernel, dim3(P), dim3(64), 2 * sizeof(int) * lut.num_patterns,
-                       s, K, col_offset, lut.num_patterns, lut.M_r, lut.lut_ptr, lut.lut_ids,
+                       s, K, col_offset, seg, lut.num_patterns, lut.M_r, lut.lut_ptr, lut.lut_ids,
                        lut.id_pattern, codes, dense, col_ptr, val_ptr, counts, cols, vals,
                        padding);
     return hipGetLastError();

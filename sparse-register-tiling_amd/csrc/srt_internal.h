@@ -112,7 +112,9 @@ constexpr int SRT_GROUP_PAD = 4;
 // Packed matrix in device memory (flat arrays over all panels).
 struct Packed {
     int M = 0, K = 0, M_r = 0, P = 0, NK = 0;
-    int32_t *counts = nullptr;   // [P * NK]  nkern_counts of every panel
+    int KT = 1;                  // K tiles per panel (K_c tiling), each packed on its own
+    int K_c = 0;                 // columns per K tile (K when untiled)
+    int32_t *counts = nullptr;   // [P * KT * NK]  nkern_counts of every (panel, K tile)
     int64_t *col_ptr = nullptr;  // [P + 1]
     int64_t *val_ptr = nullptr;  // [P + 1]
     int32_t *cols = nullptr;
@@ -141,12 +143,12 @@ struct Schedule {
 };
 
 // Kernel launchers (packer.hip).
-hipError_t launch_scatter_csr(int m, int k, int M_r, const float *Lx, const void *Lp,
+hipError_t launch_scatter_csr(int m, int k, int64_t ld, int M_r, const float *Lx, const void *Lp,
                               const void *Li, int index_bytes, uint32_t *codes, float *dense,
                               int *bad, hipStream_t s);
 hipError_t launch_count_codes(int P, int K, const DeviceLut &lut, const uint32_t *codes,
                               int32_t *counts, int64_t *ncols, int64_t *nvals, hipStream_t s);
-hipError_t launch_fill_codes(int P, int K, int col_offset, const DeviceLut &lut,
+hipError_t launch_fill_codes(int P, int K, int col_offset, int seg, const DeviceLut &lut,
                              const uint32_t *codes, const float *dense, const int64_t *col_ptr,
                              const int64_t *val_ptr, const int32_t *counts, int32_t *cols,
                              float *vals, int64_t *padding, hipStream_t s);

@@ -43,7 +43,7 @@ class TileConfig:
     chunk_cols (target packed columns per wave) and b_staging (1: B tile
     staged in LDS, 2: B rows read from global memory)."""
     M_c: int = 16
-    K_c: int = 256
+    K_c: int = 0
     N_c: int = 64
     tiling_strategy: int = 0
     beta: float = 1.0
@@ -167,11 +167,12 @@ class MatMulSpecialized:
         return kp[:self.stats()["num_panels"]]
 
     def packed(self) -> Dict[str, np.ndarray]:
-        """Packed arrays copied to the host: counts [P, NK] (nkern_counts),
-        col_ptr / val_ptr [P+1], cols, vals."""
+        """Packed arrays copied to the host: counts [P, NK] (nkern_counts;
+        [P, k_tiles, NK] for a K-tiled plan), col_ptr / val_ptr [P+1], cols,
+        vals."""
         st = self.stats()
-        P, NK = st["num_panels"], st["num_patterns"]
-        counts = np.zeros(max(1, P * NK), dtype=np.int32)
+        P, NK, KT = st["num_panels"], st["num_patterns"], st["k_tiles"]
+        counts = np.zeros(max(1, P * KT * NK), dtype=np.int32)
         col_ptr = np.zeros(P + 1, dtype=np.int64)
         val_ptr = np.zeros(P + 1, dtype=np.int64)
         cols = np.zeros(max(1, st["packed_cols"]), dtype=np.int32)
@@ -179,7 +180,8 @@ class MatMulSpecialized:
         _lib.call("srt_plan_copy_packed", self._h, counts.ctypes.data_as(ct.c_void_p),
                   col_ptr.ctypes.data_as(ct.c_void_p), val_ptr.ctypes.data_as(ct.c_void_p),
                   cols.ctypes.data_as(ct.c_void_p), vals.ctypes.data_as(ct.c_void_p))
-        return {"counts": counts[:P * NK].reshape(P, NK), "col_ptr": col_ptr, "val_ptr": val_ptr,
+        counts = counts[:P * KT * NK].reshape(P, KT, NK) if KT > 1 else counts[:P * NK].reshape(P, NK)
+        return {"counts": counts, "col_ptr": col_ptr, "val_ptr": val_ptr,
                 "cols": cols[:st["packed_cols"]], "vals": vals[:st["packed_values"]],
                 "patterns": list(self.mapping.patterns)}
 

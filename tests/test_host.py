@@ -151,5 +151,6 @@ def test_default_tile_config(srt):
     c = srt._lib.TileConfigC()
     srt._lib.lib().srt_tile_config_default(ct.byref(c))
     t = srt.TileConfig.from_c(c)
-    assert (t.M_c, t.K_c, t.N_c, t.beta, t.max_tlb_entries, t.tlb_page_size) == (16, 256, 64, 1.0, 64, 4096)
+    # the reference's SOPConfig defaults, except K_c: 0 = one K tile (256 would tile K like the reference)
+    assert (t.M_c, t.K_c, t.N_c, t.beta, t.max_tlb_entries, t.tlb_page_size) == (16, 0, 64, 1.0, 64, 4096)
     assert srt.TileConfig.from_c(t.to_c()) == t
