@@ -81,6 +81,7 @@ def parse():
     ap.add_argument("--nsplit", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--staging", type=int, default=0, help="B path: 1 LDS tile, 2 global rows, 0 choose")
+    ap.add_argument("--k-tile", type=int, default=0, help="K_c: K tile of the packing (0 = one tile)")
     ap.add_argument("--config", help="JSON file written by --save-config")
     ap.add_argument("--save-config", help="write the chosen configuration here")
     ap.add_argument("--replicas", type=int, default=0, help="0 = enough to exceed 2x the 256 MB cache")
@@ -240,13 +241,13 @@ def choose_config(args, srt, csr, N, Bd):
             csr.shape[0], csr.shape[1], 1.0 - csr.values().numel() / float(csr.shape[0] * csr.shape[1]), N)
         (M_r, name), cfg = autotune.best_config(recs)
         tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
-                                             "chunk_cols", "xcd_remap", "b_staging")}
+                                             "chunk_cols", "xcd_remap", "b_staging", "K_c")}
         log("shipped tuning record:", recs[0])
     elif args.mapping == "auto":
         recs = autotune.tune(csr, N, reps=args.tune_reps, B=Bd)
         (M_r, name), cfg = autotune.best_config(recs)
         tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
-                                             "chunk_cols", "xcd_remap", "b_staging")}
+                                             "chunk_cols", "xcd_remap", "b_staging", "K_c")}
         tune_top = [{k: (round(v * 1e6, 3) if k == "seconds" else v) for k, v in r.items()}
                     for r in recs[:5]]
         log("autotune best (us):", tune_top[0])
@@ -254,11 +255,12 @@ def choose_config(args, srt, csr, N, Bd):
         mr, name = args.mapping.split(":")
         M_r = int(mr)
         tile = dict(vec_width=args.vec_width, unroll=args.unroll, waves_per_block=args.waves,
-                    nsplit=args.nsplit, chunk_cols=args.chunk, xcd_remap=1, b_staging=args.staging)
+                    nsplit=args.nsplit, chunk_cols=args.chunk, xcd_remap=1, b_staging=args.staging,
+                    K_c=args.k_tile)
     if args.save_config:
-        flags = ("--mapping %d:%s --vec-width %d --unroll %d --waves %d --nsplit %d --chunk %d --staging %d"
-                 % (M_r, name, tile["vec_width"], tile["unroll"], tile["waves_per_block"], tile["nsplit"],
-                    tile["chunk_cols"], tile.get("b_staging", 0)))
+        flags = ("--mapping %d:%s --vec-width %d --unroll %d --waves %d --nsplit %d --chunk %d --staging %d "
+                 "--k-tile %d" % (M_r, name, tile["vec_width"], tile["unroll"], tile["waves_per_block"],
+                                  tile["nsplit"], tile["chunk_cols"], tile.get("b_staging", 0), tile.get("K_c", 0)))
         with open(args.save_config, "w") as f:
             json.dump({"M_r": M_r, "mapping": name, "tile": tile, "flags": flags}, f)
     return M_r, name, srt.TileConfig(**tile), tune_top
@@ -381,10 +383,10 @@ def bench_serving(args):
 def bench_sweep(args):
     """The 48 pruned-FFN workloads (A in {512x512, 2048x512, 512x2048,
     1024x4096} magnitude-pruned to 70/80/90/95 %, N in {128, 256, 512}):
-    the register-tiled executor (best of the M_r = 4 and 8 "orig" mappings
-    at their default launch), the HIP CSR kernel, and the oracle's C
-    executor on all host threads, each graph- or wall-timed on one resident
-    operand set.  Every GPU result is checked against the CPU result."""
+    the register-tiled executor with the configuration of the shipped tuning
+    table, the HIP CSR kernel, and the oracle's C executor on all host
+    threads, each graph- or wall-timed on one resident operand set.  Every
+    GPU result is checked against the CPU result."""
     import numpy as np
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")
@@ -392,7 +394,6 @@ def bench_sweep(args):
     from oracle import cbind
     wl = srt.workloads
     nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    maps = {(4, "orig"): srt.named_mapping(4, "orig"), (8, "orig"): srt.named_mapping(8, "orig")}
     rows = []
     for case in wl.gpu_cases():
         A, B = wl.case_inputs(case, seed=0)
@@ -402,20 +403,18 @@ def bench_sweep(args):
         csr_t = torch.from_numpy(A).to_sparse_csr()
         Bd = torch.from_numpy(B).cuda()
         C = torch.empty((case.M, case.N), device="cuda")
-        best = None
-        for key, mp in maps.items():
-            plan = srt.MatMulSpecialized.from_csr(csr_t, case.N, mapping_id=mp)
-            t = at.time_graph(lambda: plan(C, Bd), reps=50)
-            if best is None or t < best[0]:
-                best = (t, key, plan.get_config())
-            plan.close()
-        plan = srt.MatMulSpecialized.from_csr(csr_t, case.N, mapping_id=maps[best[1]])
+        # the configuration of the shipped tuning table (its lookup is what a
+        # caller gets from autotune.tuned_plan)
+        plan = at.tuned_plan(csr_t, case.N)
+        t = at.time_graph(lambda: plan(C, Bd), reps=50)
+        real = plan.get_config()
+        best = (t, (plan.mapping.M_r, "orig"), real)
         plan(C, Bd)
         parts = srt.csr._parts(csr_t)
         Ccsr = torch.empty_like(C)
         t_csr = at.time_graph(lambda: srt.csr.spmm(csr_t, Bd, Ccsr, _parts_cache=parts), reps=50)
         torch.cuda.synchronize()
-        pk = cbind.pack_dense(A, best[1][0], maps[best[1]].table)
+        pk = cbind.pack_dense_ktiled(A, best[1][0], plan.mapping.mapping.table, real.K_c)
         ref = cbind.exec_packed(pk, B, nsplit=plan.split_counts())
         exact = bool(np.array_equal(C.cpu().numpy().view(np.uint32), ref.view(np.uint32)))
         csr_err = float(np.abs(Ccsr.cpu().numpy() - ref).max())
@@ -427,7 +426,8 @@ def bench_sweep(args):
         t_cpu = (time.perf_counter() - t0) / reps
         plan.close()
         r = {"M": case.M, "K": case.K, "sparsity": case.sparsity, "N": case.N, "nnz": nnz,
-             "mapping": "%d:%s" % best[1], "vec_width": best[2].vec_width,
+             "mapping": "%d:%s" % best[1], "vec_width": best[2].vec_width, "K_c": real.K_c,
+             "waves": real.waves_per_block, "nsplit": real.nsplit,
              "exec_us": round(best[0] * 1e6, 2), "exec_gflops": round(flop / best[0] / 1e9, 1),
              "exec_hbm_frac": round(bytes_alg / best[0] / 8e12, 4),
              "csr_us": round(t_csr * 1e6, 2), "csr_gflops": round(flop / t_csr / 1e9, 1),

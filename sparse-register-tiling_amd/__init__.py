@@ -31,6 +31,6 @@ from .native import NativeMapping, executors, resolve_mapping
 from .matmul import MatMulSpecialized, SpMM_SOP, TileConfig, get_executor_id
 from .sop import (Acc, SOPModule, SOPTile, execute_tile, make_sop_module, panel_pattern_codes,
                   pattern_code, sort_and_encode_patterns)
-from . import cnn, csr, precision, serving
+from . import autotune, cnn, csr, precision, serving
 
 __version__ = "0.1.0"

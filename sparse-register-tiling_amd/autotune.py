@@ -30,6 +30,11 @@ DEFAULT_SHAPES = ([dict(b_staging=1, vec_width=1, unroll=u, waves_per_block=16, 
                    for u in (4, 8) for s in (2, 4, 16)]
                   + [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nsplit=s)
                      for v in (2, 4) for u in (4, 8) for (w, s) in ((8, 8), (16, 16), (4, 4))])
+# The reduced space the shipped table is tuned over (every pruned-FFN workload):
+# lane width x workgroup shape x K tile, global-memory B path, U = 4.
+TABLE_SHAPES = [dict(b_staging=2, vec_width=v, unroll=4, waves_per_block=w, nsplit=s, K_c=kc)
+                for v in (2, 4) for (w, s) in ((8, 8), (16, 16), (4, 4)) for kc in (0, 256)]
+TABLE_MAPPINGS = [(4, "orig"), (8, "orig")]
 BUCKETS = [(0.0, 0.7), (0.7, 0.8), (0.8, 0.9), (0.9, 1.0)]
 
 
@@ -96,7 +101,7 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
             rec = {"M_r": M_r, "mapping": name, "seconds": t, "padding": plan.stats()["padding"]}
             rec.update(shape)
             rec["realised"] = {k: getattr(real, k) for k in ("b_staging", "vec_width", "waves_per_block",
-                                                             "nsplit", "chunk_cols")}
+                                                             "nsplit", "chunk_cols", "K_c")}
             out.append(rec)
             plan.close()
         base.close()
@@ -135,3 +140,33 @@ def best_config(records: List[Dict]) -> Tuple[Tuple[int, str], TileConfig]:
     r = records[0]
     kw = {k: v for k, v in r.items() if k in TileConfig.__dataclass_fields__}
     return (r["M_r"], r["mapping"]), TileConfig(xcd_remap=1, **kw)
+
+
+def tune_workloads(cases, path: str, reps: int = 20, seed: int = 0, log=None) -> "TuningCache":
+    """Tune every workload (workloads.SpMMCase) over TABLE_MAPPINGS x
+    TABLE_SHAPES and store the ranked records in a TuningCache at `path`."""
+    from . import workloads as wl
+    cache = TuningCache(path)
+    for case in cases:
+        A, B = wl.case_inputs(case, seed=seed)
+        csr = torch.from_numpy(A).to_sparse_csr()
+        recs = tune(csr, case.N, mappings=TABLE_MAPPINGS, shapes=TABLE_SHAPES, reps=reps,
+                    B=torch.from_numpy(B).cuda())
+        cache.put(case.M, case.K, case.sparsity, case.N, recs)
+        if log:
+            log(case, recs[0])
+    return cache
+
+
+def tuned_plan(csr: torch.Tensor, N: int, path: str = SHIPPED) -> MatMulSpecialized:
+    """A plan with the best measured configuration of the tuning table for
+    this shape, sparsity bucket and N (the reference's heuristic-JSON lookup,
+    tools/paper/heuristic.py:26-50, experiment_runner.py:86-113); the default
+    configuration when the table has no entry."""
+    M, K = csr.shape
+    sparsity = 1.0 - csr.values().numel() / float(max(1, M * K))
+    recs = TuningCache(path).get(M, K, sparsity, N)
+    if not recs:
+        return MatMulSpecialized.from_csr(csr, N, mapping_id=named_mapping(4, "orig"))
+    (M_r, name), cfg = best_config(recs)
+    return MatMulSpecialized.from_csr(csr, N, mapping_id=named_mapping(M_r, name), tile_config=cfg)
