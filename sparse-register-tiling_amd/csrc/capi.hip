@@ -977,11 +977,12 @@ int srt_group_create(const srt_plan *const *plans, int count, const float *const
     if (!out || !plans) return fail(SRT_ERR_INVALID, "srt_group_create: null argument");
     *out = nullptr;
     if (count < 1) return fail(SRT_ERR_INVALID, "srt_group_create: count must be >= 1");
+    for (int i = 0; i < count; ++i)
+        if (!plans[i]) return fail(SRT_ERR_INVALID, "srt_group_create: null plan");
     auto g = std::make_unique<srt_group>();
     SRT_HIP_TRY(hipGetDevice(&g->device));
     for (int i = 0; i < count; ++i) {
         const srt_plan *p = plans[i];
-        if (!p) return fail(SRT_ERR_INVALID, "srt_group_create: null plan");
         if (p->device != g->device) return fail(SRT_ERR_INVALID, "srt_group_create: plan belongs to another device");
         if (i == 0) {
             g->exec = p->exec;

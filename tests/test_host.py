@@ -154,3 +154,17 @@ def test_default_tile_config(srt):
     # the reference's SOPConfig defaults, except K_c: 0 = one K tile (256 would tile K like the reference)
     assert (t.M_c, t.K_c, t.N_c, t.beta, t.max_tlb_entries, t.tlb_page_size) == (16, 0, 64, 1.0, 64, 4096)
     assert srt.TileConfig.from_c(t.to_c()) == t
+
+
+def test_group_abi_rejects_bad_arguments(srt):
+    """Grouped-execution entry points fail with a status, not a crash."""
+    L = srt._lib.lib()
+    out = ct.c_void_p()
+    assert L.srt_group_create(None, 1, None, None, None, None, None, None, ct.byref(out)) == srt._lib.SRT_ERR_INVALID
+    plans = (ct.c_void_p * 1)()
+    assert L.srt_group_create(plans, 0, None, None, None, None, None, None, ct.byref(out)) == srt._lib.SRT_ERR_INVALID
+    assert L.srt_group_create(plans, 1, None, None, None, None, None, None, ct.byref(out)) == srt._lib.SRT_ERR_INVALID
+    assert L.srt_group_execute(None, None) == srt._lib.SRT_ERR_INVALID
+    assert L.srt_group_info(None, None, None, None) == srt._lib.SRT_ERR_INVALID
+    assert L.srt_group_rebind(None, None, None, None, None, None, None) == srt._lib.SRT_ERR_INVALID
+    L.srt_group_destroy(None)
