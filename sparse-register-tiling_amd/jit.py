@@ -58,7 +58,13 @@ def ensure_executor(native, regen: bool = False, lean: bool = False) -> str:
             raise
     eid = executor_id(native.M_r, native.patterns)
     path = os.path.join(JIT_DIR, "libsrt_exec_%s.so" % eid)
-    if regen or not os.path.exists(path):
+    # A built executor is reused unless libsrt or the sources it was generated
+    # from are newer (the executor table layout lives in srt_internal.h).
+    deps = [_lib.LIB_PATH, os.path.join(PKG_DIR, "codegen.py"),
+            os.path.join(PKG_DIR, "csrc", "srt_internal.h"), os.path.join(PKG_DIR, "csrc", "exec_kernel.h")]
+    stale = not os.path.exists(path) or any(
+        os.path.exists(d) and os.path.getmtime(d) > os.path.getmtime(path) for d in deps)
+    if regen or stale:
         path = _compile(eid, gen_executor_sources(native.M_r, native.patterns, lean=lean))
     if path not in _loaded:
         _lib.lib()
