@@ -464,25 +464,26 @@ static int run_executor(const Packed &pk, const Schedule &sch, const ExecutorDes
     if (stage && !lds_fits(true, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce)) stage = false;
     if (!lds_fits(stage, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce))
         return fail(SRT_ERR_UNSUPPORTED, "chunk reduction needs more LDS than a workgroup has");
-    const int ui = log2_small(cfg.unroll);
+    // The batch records are cut for sch.unroll: the launch must use that U.
+    const int ui = log2_small(sch.unroll);
     const int vi = log2_small(V);
-    ExecLaunchFn fn = nullptr;
-    for (int u = ui; u >= 0 && !fn; --u) fn = exec->launch[stage][vi][u];
-    for (int u = ui + 1; u < 4 && !fn; ++u) fn = exec->launch[stage][vi][u];
-    if (!fn) return fail(SRT_ERR_UNSUPPORTED, "executor variant not generated");
+    ExecLaunchFn fn = exec->launch[stage][vi][ui];
+    if (!fn && stage) {
+        stage = false;
+        if (!lds_fits(false, pk.K, sch.waves_per_block, pk.M_r, V, sch.reduce))
+            return fail(SRT_ERR_UNSUPPORTED, "chunk reduction needs more LDS than a workgroup has");
+        fn = exec->launch[0][vi][ui];
+    }
+    if (!fn || !sch.prog) return fail(SRT_ERR_UNSUPPORTED, "executor variant not generated");
     // B rows are addressed with a 32-bit byte offset (buffer loads).
     const int64_t b_extent = (int64_t)pk.K * ldb * (int64_t)sizeof(float);
     if (b_extent > (int64_t)0xffffffffLL)
         return fail(SRT_ERR_UNSUPPORTED, "B larger than 4 GiB (K * ldb * 4 bytes)");
     ExecLaunch a;
-    a.groups = pk.groups;
-    a.grp_ptr = pk.grp_ptr;
+    a.prog = sch.prog;
     a.desc = sch.desc;
     a.n_wblocks = sch.n_wblocks;
     a.reduce = sch.reduce;
-    a.col_ptr = pk.col_ptr;
-    a.val_ptr = pk.val_ptr;
-    a.cols = pk.cols;
     a.vals = pk.vals;
     a.B = B;
     a.b_bytes = (uint32_t)std::max<int64_t>(b_extent, 4);
@@ -497,7 +498,7 @@ static int run_executor(const Packed &pk, const Schedule &sch, const ExecutorDes
     a.vec_width = V;
     a.waves_per_block = sch.waves_per_block;
     a.xcd_remap = cfg.xcd_remap;
-    a.unroll = cfg.unroll;
+    a.unroll = sch.unroll;
     SRT_HIP_TRY(fn(a, s));
     return SRT_OK;
 }
@@ -572,7 +573,9 @@ static int build_groups(Packed &pk, const int32_t *d_id_pattern, hipStream_t s) 
 
 void Schedule::release() {
     hipFree(desc);
+    hipFree(prog);
     desc = nullptr;
+    prog = nullptr;
     n_wblocks = 0;
     kp.clear();
 }
@@ -582,9 +585,20 @@ void Schedule::release() {
 // into workgroups of waves_per_block slots in panel order (a panel never
 // straddles two workgroups), and the wave descriptors are computed on the
 // device from the group lists.
-static int build_schedule(const Packed &pk, const srt_tile_config &cfg, Schedule &out,
+// The unroll the batch records are cut for: the requested one when the
+// executor has it for every lane width of the global-memory path, else 4
+// (which every generated executor has).
+static int schedule_unroll(const ExecutorDesc *exec, int want) {
+    const int ui = log2_small(want);
+    for (int v = 0; v < 3; ++v)
+        if (!exec->launch[0][v][ui]) return 4;
+    return want;
+}
+
+static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const ExecutorDesc *exec, Schedule &out,
                           hipStream_t s) {
     Schedule sch;
+    sch.unroll = schedule_unroll(exec, pow2_upto8(cfg.unroll) ? cfg.unroll : 4);
     sch.waves_per_block = cfg.waves_per_block;
     sch.nsplit = std::min(cfg.nsplit, cfg.waves_per_block);
     sch.chunk_cols = std::max(1, cfg.chunk_cols);
@@ -626,6 +640,19 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, Schedule
         SRT_HIP_TRY(launch_wave_desc(nslots, d_slots, pk.groups, pk.grp_ptr, pk.col_ptr,
                                      pk.id_pattern, sch.desc, s));
     }
+    // Batch records: at most one per packed column of each chunk plus a
+    // terminator per slot, and two records of look-ahead padding.
+    if (pk.total_vals + SRT_VAL_PAD > (int64_t)0x7fffffffLL)
+        return fail(SRT_ERR_UNSUPPORTED, "more than 2^31 packed values");
+    const size_t nrec = (size_t)pk.total_cols + (size_t)nslots + 2;
+    const int RS = rec_stride(sch.unroll);
+    if (nrec * RS * sizeof(int32_t) > (size_t)0xffffffffULL)
+        return fail(SRT_ERR_UNSUPPORTED, "batch records exceed 4 GiB");
+    SRT_TRY(dmalloc(&sch.prog, nrec * RS));
+    SRT_HIP_TRY(hipMemsetAsync(sch.prog, 0, sizeof(int32_t) * nrec * RS, s));
+    if (nslots)
+        SRT_HIP_TRY(launch_wave_prog(nslots, sch.unroll, pk.groups, pk.grp_ptr, pk.col_ptr, pk.val_ptr, pk.cols,
+                                     sch.desc, sch.prog, s));
     SRT_HIP_TRY(hipStreamSynchronize(s));
     out.release();
     out = std::move(sch);
@@ -790,7 +817,7 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
     plan->n_max = n;
     plan->cfg = plan->cfg_user;
     realise_config(plan->cfg, map.M_r, k, plan->pk.K_c, plan->pk.total_cols, std::max(n, 1));
-    rc = build_schedule(plan->pk, plan->cfg, plan->sch, (hipStream_t)stream);
+    rc = build_schedule(plan->pk, plan->cfg, plan->exec, plan->sch, (hipStream_t)stream);
     if (rc) {
         plan->pk.release();
         return rc;
@@ -800,7 +827,8 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
 }
 
 static bool same_schedule(const srt_tile_config &a, const srt_tile_config &b) {
-    return a.nsplit == b.nsplit && a.chunk_cols == b.chunk_cols && a.waves_per_block == b.waves_per_block;
+    return a.nsplit == b.nsplit && a.chunk_cols == b.chunk_cols && a.waves_per_block == b.waves_per_block &&
+           a.unroll == b.unroll;
 }
 
 int srt_plan_get_config(const srt_plan *p, srt_tile_config *out) {
@@ -824,7 +852,7 @@ int srt_plan_allocate_executor(srt_plan *p, int n) {
     // schedule if they changed.  Not to be called while the plan executes.
     srt_tile_config c = p->cfg_user;
     realise_config(c, p->pk.M_r, p->pk.K, p->pk.K_c, p->pk.total_cols, std::max(n, 1));
-    if (!same_schedule(c, p->cfg)) SRT_TRY(build_schedule(p->pk, c, p->sch, nullptr));
+    if (!same_schedule(c, p->cfg)) SRT_TRY(build_schedule(p->pk, c, p->exec, p->sch, nullptr));
     p->cfg = c;
     return SRT_OK;
 }
@@ -929,12 +957,8 @@ static int group_bind(srt_group &g, const float *const *B, const int64_t *ldb, f
         const srt_plan &p = *g.plans[i];
         if (n[i] == 0 || p.pk.M == 0 || p.sch.n_wblocks == 0) continue;
         ExecProblem q{};
-        q.groups = p.pk.groups;
-        q.grp_ptr = p.pk.grp_ptr;
+        q.prog = p.sch.prog;
         q.desc = p.sch.desc;
-        q.col_ptr = p.pk.col_ptr;
-        q.val_ptr = p.pk.val_ptr;
-        q.cols = p.pk.cols;
         q.vals = p.pk.vals;
         q.B = B[i];
         q.C = C[i];
@@ -993,6 +1017,8 @@ int srt_group_create(const srt_plan *const *plans, int count, const float *const
         } else if (p->sch.waves_per_block != g->waves) {
             return fail(SRT_ERR_INVALID, "srt_group_create: plans use different waves_per_block");
         }
+        if (p->sch.unroll != 4)
+            return fail(SRT_ERR_UNSUPPORTED, "srt_group_create: grouped launches need unroll 4");
         g->plans.push_back(p);
     }
     if (!g->exec->grouped[0]) return fail(SRT_ERR_UNSUPPORTED, "srt_group_create: executor has no grouped variant");
@@ -1234,7 +1260,7 @@ int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, fl
     realise_config(c, t->pk.M_r, t->pk.K, t->pk.K_c ? t->pk.K_c : t->pk.K, t->pk.total_cols, std::max(N_c, 1));
     std::lock_guard<std::mutex> g(t->mu);
     if (!t->sch.desc || !same_schedule(c, t->sch_cfg)) {
-        SRT_TRY(build_schedule(t->pk, c, t->sch, (hipStream_t)stream));
+        SRT_TRY(build_schedule(t->pk, c, t->exec, t->sch, (hipStream_t)stream));
         t->sch_cfg = c;
     }
     return run_executor(t->pk, t->sch, t->exec, c, B, ldb, C, ldc, N_c, load_c ? 1 : 0,

@@ -152,27 +152,29 @@ __device__ __forceinline__ void fma_pattern(float (&acc)[MR][V], const float *__
 // (kernel id, count) is cut into batches of U columns, the last one partial.
 // A batch is run by one case of the generated switch, keyed id * U + n - 1
 // for n columns: n unrolled columns of PC = popcount(pattern) fused
-// multiply-adds each, accumulator rows fixed at compile time.
+// multiply-adds each, accumulator rows fixed at compile time.  The batches
+// are precomputed per wave slot as records (key, first value, U B rows; see
+// rec_stride), so the loop spends no scalar instructions on walking groups.
 //
-// Software pipeline.  Column indices and values are read through the scalar
-// unit (wave-uniform SGPR operands).  Scalar loads and LDS reads share one
-// wait counter and scalar loads may return out of order, so a wait for a
-// scalar load also waits for every LDS read in flight.  Each iteration
-// therefore first consumes what the previous one loaded (one wait), then
-// issues everything the next iterations need, then runs its FMAs on
-// registers that have already arrived:
+// Software pipeline.  Records and values are read through the scalar unit
+// (wave-uniform SGPR operands).  Scalar loads and LDS reads share one wait
+// counter and scalar loads may return out of order, so a wait for a scalar
+// load also waits for every LDS read in flight.  Each iteration therefore
+// first consumes what the previous one loaded (one wait), then issues
+// everything the next iterations need, then runs its FMAs on registers that
+// have already arrived:
 //
-//   iteration i:  rows'  <- B rows of batch i+1   (indices loaded by iteration i-1)
-//                 cols'' <- column indices of batch i+2
+//   iteration i:  rows'  <- B rows of batch i+1   (its record loaded by iteration i-1)
+//                 rec''  <- record of batch i+2
 //                 vals'  <- first W values of batch i+1
 //                 FMAs of batch i from rows and vals
 //
-// All U rows of a batch are loaded whatever its size (indices past the chunk
-// are valid indices of later columns or zero padding), so the loads carry no
-// branches.  A batch that needs more than W values reads the rest itself.
-// The packed arrays carry SRT_COL_PAD / SRT_VAL_PAD zero entries at their
-// end, and group lists SRT_GROUP_PAD zero entries, so every look-ahead read
-// stays in bounds; entries past the chunk are never used.
+// All U rows of a batch are loaded whatever its size (record slots past n
+// repeat a valid row), so the loads carry no branches.  A batch that needs
+// more than W values reads the rest itself.  Every run ends in a terminator
+// record (key -1, valid row and value indices) and the record array carries
+// two records of padding, the value array SRT_VAL_PAD zero entries, so every
+// look-ahead read stays in bounds; what is read past the run is never used.
 constexpr int kValWindow = 16;
 
 // FMAs of one batch of NCOL columns of pattern PAT.
@@ -191,75 +193,34 @@ __device__ __forceinline__ void batch_fmas(float (&acc)[MR][V], const typename V
     }
 }
 
-// Batch generator over a wave's chunk: (id, pc, n) of successive batches.
-// Group entries are (id | pc << 16, count); the entry after the current one
-// is already in registers and the one after that is being loaded.
-template <int U>
-struct BatchGen {
-    const int2 *__restrict__ gl;  // entry of the current group
-    int2 nxt;                     // entry gl[1]
-    int gleft;                    // columns of the current group not yet batched
-    int rem;                      // columns of the chunk not yet batched
-    int key;                      // id * U of the current group
-    int pc;
-
-    __device__ __forceinline__ void start(const int2 *__restrict__ g, int skip, int total) {
-        gl = g;
-        const int2 e = g[0];
-        nxt = g[1];  // padded: in bounds
-        rem = total;
-        gleft = e.y - skip < total ? e.y - skip : total;
-        key = (e.x & 0xffff) * U;
-        pc = e.x >> 16;
-    }
-    // Next batch: n columns (0 at the end of the chunk), key and pc.
-    __device__ __forceinline__ int next(int &bkey, int &bpc) {
-        if (rem <= 0) {
-            bkey = 0;
-            bpc = 0;
-            return 0;
-        }
-        if (gleft <= 0) {
-            ++gl;
-            const int2 e = nxt;
-            nxt = gl[1];
-            gleft = e.y < rem ? e.y : rem;
-            key = (e.x & 0xffff) * U;
-            pc = e.x >> 16;
-        }
-        const int n = gleft < U ? gleft : U;
-        gleft -= n;
-        rem -= n;
-        bkey = key + n - 1;
-        bpc = pc;
-        return n;
-    }
-};
+// vals + i for a record's value index (< 2^31): a 32-bit byte offset, which
+// scalar loads take as their offset operand (no 64-bit address arithmetic).
+__device__ __forceinline__ const float *value_at(const float *vals, int i) {
+    return reinterpret_cast<const float *>(reinterpret_cast<const char *>(vals) + ((uint32_t)i << 2));
+}
 
 // Pipeline registers of one iteration.
 template <int V, int U>
 struct Stage {
     typename VecOf<V>::type rows[U];  // B rows of the batch
     float vals[kValWindow];           // its first values
-    int cols_next[U];                 // column indices of the following batch
+    int rec[2 + U];                   // record of the following batch (its used words:
+                                      // unused destination words of a wide scalar load
+                                      // would be reused early and force a wait)
 };
 
-// One pipeline iteration: batch (n, key, pc) in `in`, the following batch
-// of n1 columns is prepared in `out`, and the generator yields the batch
-// after it (n2).
+// One pipeline iteration: batch (key, first value vo) in `in`, whose
+// in.rec is the following batch's record; that batch is prepared in `out`
+// (rows, values, and the record after it from rp).
 template <int MR, int V, int U, class D, class R>
-__device__ __forceinline__ void pipe_step(float (&acc)[MR][V], const Stage<V, U> &in, Stage<V, U> &out,
-                                          int n, int key, int pc, int n1, int &n2, int &key2, int &pc2,
-                                          BatchGen<U> &gen, const int32_t *__restrict__ &cp,
-                                          const float *__restrict__ &vp, const R &bb) {
+__device__ __forceinline__ void pipe_step(float (&acc)[MR][V], const Stage<V, U> &in, Stage<V, U> &out, int key,
+                                          int vo, const int32_t *__restrict__ prog, uint32_t roff,
+                                          const float *__restrict__ vals, const R &bb) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) out.rows[u] = bb.load(in.cols_next[u]);
-    n2 = gen.next(key2, pc2);
-    load_uniform(cp + n + n1, out.cols_next);
-    load_uniform(vp + pc * n, out.vals);
-    D::template fmas<V, U>(key, acc, in.rows, in.vals, vp);
-    cp += n;
-    vp += pc * n;
+    for (int u = 0; u < U; ++u) out.rows[u] = bb.load(in.rec[2 + u]);
+    load_uniform(reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(prog) + roff), out.rec);
+    load_uniform(value_at(vals, in.rec[1]), out.vals);
+    D::template fmas<V, U>(key, acc, in.rows, in.vals, value_at(vals, vo));
 }
 
 // D::fmas<V, U>(key, acc, rows, vals, vp) is the generated switch over
@@ -271,9 +232,10 @@ __device__ __forceinline__ void pipe_step(float (&acc)[MR][V], const Stage<V, U>
 // columns, chunk j = [floor(j*n_p/k_p), floor((j+1)*n_p/k_p)), one wavefront
 // each; the chunks of a panel sit in consecutive wave slots of one workgroup
 // and light panels share workgroups.  Each wave reads its descriptor
-//   d0 = {panel, group of chunk start, skip in that group, first column}
+//   d0 = {panel, group of chunk start, first record of its run, first column}
 //   d1 = {first value, end column, k_p, j}
-// (panel -1 = idle slot).  Waves j > 0 park their partial accumulators in LDS
+// (panel -1 = idle slot); the executor uses the panel, the record run, k_p
+// and j.  Waves j > 0 park their partial accumulators in LDS
 // and wave j = 0 adds them in chunk order:
 //   C = ((chain_0 + chain_1) + chain_2) + ...
 // chain_0 starting from C (load_c) or 0, the others from 0.  With every
@@ -288,11 +250,9 @@ __device__ __forceinline__ void pipe_step(float (&acc)[MR][V], const Stage<V, U>
 // The work of one (logical) workgroup b of one executor launch; shared by
 // the single-problem kernel and the grouped one.
 template <int MR, int V, int U, bool STAGE, class D>
-__device__ __forceinline__ void exec_body(float *lds, const int2 *__restrict__ groups,
-                                          const int64_t *__restrict__ grp_ptr, const int4 *__restrict__ desc,
-                                          const int64_t *__restrict__ col_ptr,
-                                          const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
-                                          const float *__restrict__ vals, const float *__restrict__ B,
+__device__ __forceinline__ void exec_body(float *lds, const int32_t *__restrict__ prog,
+                                          const int4 *__restrict__ desc, const float *__restrict__ vals,
+                                          const float *__restrict__ B,
                                           uint32_t b_bytes, uint32_t ldb, float *__restrict__ C, int64_t ldc,
                                           int M, int N, int K, int n_wblocks, int reduce, int load_c, int b) {
     typedef typename VecOf<V>::type vec;
@@ -368,8 +328,14 @@ __device__ __forceinline__ void exec_body(float *lds, const int2 *__restrict__ g
                 }
             }
         }
-        int remaining = d1.y - d0.w;
-        if (remaining > 0) {
+        // Record run of this slot (its start replaces the descriptor's d0.z);
+        // records are addressed by a 32-bit byte offset (the host keeps the
+        // array below 4 GiB), the scalar loads' offset operand.
+        constexpr uint32_t RB = rec_stride(U) * 4u;
+        uint32_t roff = (uint32_t)d0.z * RB;
+        int r0[2 + U];
+        load_uniform(reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(prog) + roff), r0);
+        if (r0[0] >= 0) {
             typedef typename std::conditional<STAGE, LdsRows<V>, BRows<V>>::type Rows;
             Rows bb;
             if constexpr (STAGE) {
@@ -379,36 +345,25 @@ __device__ __forceinline__ void exec_body(float *lds, const int2 *__restrict__ g
                 bb.voff = (uint32_t)nl * 4u;
                 bb.row_bytes = ldb * 4u;
             }
-            const int32_t *__restrict__ cp = cols + col_ptr[p] + d0.w;
-            const float *__restrict__ vp = vals + val_ptr[p] + d1.x;
-            BatchGen<U> gen;
-            gen.start(groups + grp_ptr[p] + d0.y, d0.z, remaining);
             Stage<V, U> s0, s1;
-            int k0, q0, k1, q1, k2, q2;
-            int n0 = gen.next(k0, q0);
-            int n1 = gen.next(k1, q1);
-            {
-                int col[U];
-                load_uniform(cp, col);  // padded: in bounds, valid column indices
 #pragma unroll
-                for (int u = 0; u < U; ++u) s0.rows[u] = bb.load(col[u]);
-                load_uniform(cp + n0, s0.cols_next);
-                load_uniform(vp, s0.vals);
-            }
+            for (int u = 0; u < U; ++u) s0.rows[u] = bb.load(r0[2 + u]);
+            load_uniform(reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(prog) + roff + RB), s0.rec);
+            load_uniform(value_at(vals, r0[1]), s0.vals);
+            int key = r0[0], vo = r0[1];
+            roff += 2 * RB;
             // Two iterations per trip so the stages alternate without copies.
             while (true) {
-                int n2;
-                pipe_step<MR, V, U, D, Rows>(acc, s0, s1, n0, k0, q0, n1, n2, k2, q2, gen, cp, vp, bb);
-                if (n1 <= 0) break;
-                int n3, k3, q3;
-                pipe_step<MR, V, U, D, Rows>(acc, s1, s0, n1, k1, q1, n2, n3, k3, q3, gen, cp, vp, bb);
-                if (n2 <= 0) break;
-                n0 = n2;
-                k0 = k2;
-                q0 = q2;
-                n1 = n3;
-                k1 = k3;
-                q1 = q3;
+                const int key1 = s0.rec[0], vo1 = s0.rec[1];
+                pipe_step<MR, V, U, D, Rows>(acc, s0, s1, key, vo, prog, roff, vals, bb);
+                roff += RB;
+                if (key1 < 0) break;
+                const int key2 = s1.rec[0], vo2 = s1.rec[1];
+                pipe_step<MR, V, U, D, Rows>(acc, s1, s0, key1, vo1, prog, roff, vals, bb);
+                roff += RB;
+                if (key2 < 0) break;
+                key = key2;
+                vo = vo2;
             }
         }
     }
@@ -454,17 +409,15 @@ __device__ __forceinline__ void exec_body(float *lds, const int2 *__restrict__ g
 
 template <int MR, int V, int U, bool STAGE, class D>
 __global__ void __launch_bounds__(1024)
-    exec_kernel(const int2 *__restrict__ groups, const int64_t *__restrict__ grp_ptr,
-                const int4 *__restrict__ desc, const int64_t *__restrict__ col_ptr,
-                const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
+    exec_kernel(const int32_t *__restrict__ prog, const int4 *__restrict__ desc,
                 const float *__restrict__ vals, const float *__restrict__ B, uint32_t b_bytes,
                 uint32_t ldb, float *__restrict__ C, int64_t ldc, int M, int N, int K, int n_wblocks,
                 int reduce, int load_c, int xcd_remap) {
     extern __shared__ float lds[];  // STAGE: B tile [K][64*V]; reduction: [waves][MR][V][64]
     int b = blockIdx.x;
     if (xcd_remap) b = xcd_swizzle(b, gridDim.x);
-    exec_body<MR, V, U, STAGE, D>(lds, groups, grp_ptr, desc, col_ptr, val_ptr, cols, vals, B, b_bytes, ldb, C,
-                                  ldc, M, N, K, n_wblocks, reduce, load_c, b);
+    exec_body<MR, V, U, STAGE, D>(lds, prog, desc, vals, B, b_bytes, ldb, C, ldc, M, N, K, n_wblocks, reduce,
+                                  load_c, b);
 }
 
 // Grouped: several independent packed matrices (the layers of a network,
@@ -488,10 +441,9 @@ __global__ void __launch_bounds__(1024)
         const int mid = (lo + hi + 1) >> 1;
         if (probs[mid].block_begin <= b) lo = mid; else hi = mid - 1;
     }
-    exec_body<MR, V, U, STAGE, D>(lds, reinterpret_cast<const int2 *>(probs[lo].groups), probs[lo].grp_ptr,
-                                  reinterpret_cast<const int4 *>(probs[lo].desc), probs[lo].col_ptr,
-                                  probs[lo].val_ptr, probs[lo].cols, probs[lo].vals, probs[lo].B,
-                                  probs[lo].b_bytes, probs[lo].ldb, probs[lo].C, probs[lo].ldc, probs[lo].M,
+    exec_body<MR, V, U, STAGE, D>(lds, probs[lo].prog, reinterpret_cast<const int4 *>(probs[lo].desc),
+                                  probs[lo].vals, probs[lo].B, probs[lo].b_bytes, probs[lo].ldb, probs[lo].C,
+                                  probs[lo].ldc, probs[lo].M,
                                   probs[lo].N, probs[lo].K, probs[lo].n_wblocks, probs[lo].reduce,
                                   probs[lo].load_c, b - probs[lo].block_begin);
 }
@@ -525,8 +477,7 @@ hipError_t launch_exec(const ExecLaunch &a, hipStream_t s) {
         }
     }
     hipLaunchKernelGGL((exec_kernel<MR, V, U, STAGE, D>), dim3((unsigned)nblocks),
-                       dim3(64 * a.waves_per_block), lds, s, reinterpret_cast<const int2 *>(a.groups),
-                       a.grp_ptr, reinterpret_cast<const int4 *>(a.desc), a.col_ptr, a.val_ptr, a.cols,
+                       dim3(64 * a.waves_per_block), lds, s, a.prog, reinterpret_cast<const int4 *>(a.desc),
                        a.vals, a.B, a.b_bytes, (uint32_t)a.ldb, a.C, a.ldc, a.M, a.N, a.K, a.n_wblocks,
                        a.reduce, a.load_c, a.xcd_remap);
     return hipGetLastError();

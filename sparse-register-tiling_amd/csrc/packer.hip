@@ -302,6 +302,52 @@ __global__ void wave_desc_kernel(int nslots, const int32_t *__restrict__ slots,
     desc[2 * i + 1] = make_int4((int)(vs + skip * pc), (int)c1, kp, j);
 }
 
+// Batch records of every wave slot (one thread per slot).  Slot i's chunk is
+// packed columns [c0, c1) of panel p (its descriptor); its records start at
+// record index col_ptr[p] + c0 + i, so the runs of consecutive slots never
+// overlap (a run holds at most c1 - c0 batches and a terminator).  The start
+// index replaces the descriptor's group skip (d0.z), which only this builder
+// reads.
+__global__ void wave_prog_kernel(int nslots, int U, int RS, const int2 *__restrict__ groups,
+                                 const int64_t *__restrict__ grp_ptr, const int64_t *__restrict__ col_ptr,
+                                 const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
+                                 int4 *__restrict__ desc, int32_t *__restrict__ prog) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    const int4 d0 = desc[2 * i], d1 = desc[2 * i + 1];
+    const int p = d0.x;
+    if (p < 0) return;
+    const int64_t cbase = col_ptr[p];
+    int64_t c = cbase + d0.w;
+    const int64_t cend = cbase + d1.y;
+    int64_t v = val_ptr[p] + d1.x;
+    const int64_t start = c + i;
+    int32_t *rec = prog + start * RS;
+    const int2 *g = groups + grp_ptr[p] + d0.y;
+    int skip = d0.z;
+    while (c < cend) {
+        const int2 e = *g++;
+        const int id = e.x & 0xffff, pc = e.x >> 16;
+        int64_t take = e.y - skip;
+        if (take > cend - c) take = cend - c;
+        skip = 0;
+        while (take > 0) {
+            const int n = take < U ? (int)take : U;
+            rec[0] = id * U + n - 1;
+            rec[1] = (int)v;
+            for (int u = 0; u < U; ++u) rec[2 + u] = cols[c + (u < n ? u : n - 1)];
+            c += n;
+            v += (int64_t)pc * n;
+            take -= n;
+            rec += RS;
+        }
+    }
+    rec[0] = -1;  // terminator: its rows (column 0) and values (index v) stay valid reads
+    rec[1] = (int)v;
+    for (int u = 0; u < U; ++u) rec[2 + u] = 0;
+    desc[2 * i] = make_int4(d0.x, d0.y, (int)start, d0.w);
+}
+
 // ------------------------------------------------------- entry-list variants
 __global__ void encode_entries_kernel(int n, int M_r, const int32_t *__restrict__ pattern_id,
                                       const int32_t *__restrict__ pat_codes,
@@ -508,6 +554,16 @@ hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t
     if (P <= 0) return hipSuccess;
     hipLaunchKernelGGL(group_fill_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, NK, counts,
                        grp_ptr, id_pattern, groups);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_prog(int nslots, int U, const int32_t *groups, const int64_t *grp_ptr,
+                            const int64_t *col_ptr, const int64_t *val_ptr, const int32_t *cols,
+                            int32_t *desc, int32_t *prog, hipStream_t s) {
+    if (nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wave_prog_kernel, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, U, rec_stride(U),
+                       reinterpret_cast<const int2 *>(groups), grp_ptr, col_ptr, val_ptr, cols,
+                       reinterpret_cast<int4 *>(desc), prog);
     return hipGetLastError();
 }
 

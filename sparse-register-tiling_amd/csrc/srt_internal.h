@@ -25,15 +25,19 @@ int fail(int code, const std::string &msg);
 
 // ------------------------------------------------------ generated executors
 // Arguments of one executor launch over a packed matrix.
+// Batch records (the executor's instruction stream, built per schedule):
+// every wave slot owns a run of records, one per batch of U packed columns of
+// one kernel id, then a terminator:
+//   rec[0] = id * U + n - 1 (n columns; -1 terminates), rec[1] = index of the
+//   batch's first value in vals, rec[2 .. 2 + U) = its B rows (slots past n
+//   repeat a valid row; they are loaded and never used).
+constexpr int rec_stride(int U) { return U <= 2 ? 4 : U <= 6 ? 8 : 16; }
+
 struct ExecLaunch {
-    const int32_t *groups;   // (id | pc << 16, count) pairs of non-empty groups, id order
-    const int64_t *grp_ptr;  // [P + 1] range of panel p in groups (in pairs)
+    const int32_t *prog;     // batch records (see rec_stride)
     const int32_t *desc;     // [n_wblocks * waves_per_block] pairs of int4 wave descriptors
     int n_wblocks;           // workgroups per N tile
     int reduce;              // some panel runs on more than one wave
-    const int64_t *col_ptr;  // [P + 1]
-    const int64_t *val_ptr;  // [P + 1]
-    const int32_t *cols;
     const float *vals;
     const float *B;
     uint32_t b_bytes;  // bytes of B addressable through the buffer resource
@@ -51,12 +55,8 @@ typedef hipError_t (*ExecLaunchFn)(const ExecLaunch &, hipStream_t);
 
 // One problem of a grouped executor launch (device table entry).
 struct ExecProblem {
-    const int32_t *groups;
-    const int64_t *grp_ptr;
+    const int32_t *prog;
     const int32_t *desc;
-    const int64_t *col_ptr;
-    const int64_t *val_ptr;
-    const int32_t *cols;
     const float *vals;
     const float *B;
     float *C;
@@ -133,6 +133,8 @@ struct Packed {
 // laid into workgroups of waves_per_block wave slots in panel order.
 struct Schedule {
     int32_t *desc = nullptr;  // device: two int4 per wave slot (see exec_kernel.h)
+    int32_t *prog = nullptr;  // device: batch records of every wave slot
+    int unroll = 0;           // U the records are cut for
     int n_wblocks = 0;
     int waves_per_block = 0;
     int nsplit = 0;
@@ -160,6 +162,9 @@ hipError_t launch_group_count(int P, int NK, const int32_t *counts, int64_t *ngr
 hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t *grp_ptr,
                              const int32_t *id_pattern, int32_t *groups, hipStream_t s);
 // Wave descriptors from (panel, j, kp) triples (slots[3 * i]; panel -1 = idle).
+hipError_t launch_wave_prog(int nslots, int U, const int32_t *groups, const int64_t *grp_ptr,
+                            const int64_t *col_ptr, const int64_t *val_ptr, const int32_t *cols,
+                            int32_t *desc, int32_t *prog, hipStream_t s);
 hipError_t launch_wave_desc(int nslots, const int32_t *slots, const int32_t *groups,
                             const int64_t *grp_ptr, const int64_t *col_ptr,
                             const int32_t *id_pattern, int32_t *desc, hipStream_t s);
