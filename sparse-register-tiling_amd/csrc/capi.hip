@@ -633,7 +633,7 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
     Scratch sc;
     int32_t *d_slots = nullptr;
     SRT_TRY(sc.get(&d_slots, slots.size()));
-    SRT_TRY(dmalloc(&sch.desc, 8 * (size_t)std::max(nslots, 1)));
+    SRT_TRY(dmalloc(&sch.desc, 16 * (size_t)std::max(nslots, 1)));
     if (nslots) {
         SRT_HIP_TRY(hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * slots.size(),
                                    hipMemcpyHostToDevice, s));

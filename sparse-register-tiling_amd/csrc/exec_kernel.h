@@ -234,6 +234,7 @@ __device__ __forceinline__ void pipe_step(float (&acc)[MR][V], const Stage<V, U>
 // and light panels share workgroups.  Each wave reads its descriptor
 //   d0 = {panel, group of chunk start, first record of its run, first column}
 //   d1 = {first value, end column, k_p, j}
+//   then (U <= 6) a copy of its first batch record
 // (panel -1 = idle slot); the executor uses the panel, the record run, k_p
 // and j.  Waves j > 0 park their partial accumulators in LDS
 // and wave j = 0 adds them in chunk order:
@@ -261,9 +262,11 @@ __device__ __forceinline__ void exec_body(float *lds, const int32_t *__restrict_
     const int lane = threadIdx.x & 63;
     const int ntile = b / n_wblocks;
     const int sb = b - ntile * n_wblocks;
-    const int4 *__restrict__ dw = desc + ((int64_t)sb * waves + wave) * 2;
-    const int4 d0 = dw[0];
-    const int4 d1 = dw[1];
+    const int4 *__restrict__ dw = desc + ((int64_t)sb * waves + wave) * 4;
+    int dv[16];  // descriptor and (for 2 + U <= 8) the first batch record: one scalar load
+    load_uniform(reinterpret_cast<const int *>(dw), dv);
+    const int4 d0 = make_int4(dv[0], dv[1], dv[2], dv[3]);
+    const int4 d1 = make_int4(dv[4], dv[5], dv[6], dv[7]);
     const int p = d0.x;
     const bool live = p >= 0;  // wave-uniform
     if (!STAGE && !reduce && !live) return;  // no barrier below
@@ -334,7 +337,12 @@ __device__ __forceinline__ void exec_body(float *lds, const int32_t *__restrict_
         constexpr uint32_t RB = rec_stride(U) * 4u;
         uint32_t roff = (uint32_t)d0.z * RB;
         int r0[2 + U];
-        load_uniform(reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(prog) + roff), r0);
+        if constexpr (2 + U <= 8) {
+#pragma unroll
+            for (int k = 0; k < 2 + U; ++k) r0[k] = dv[8 + k];
+        } else {
+            load_uniform(reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(prog) + roff), r0);
+        }
         if (r0[0] >= 0) {
             typedef typename std::conditional<STAGE, LdsRows<V>, BRows<V>>::type Rows;
             Rows bb;

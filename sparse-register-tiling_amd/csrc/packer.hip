@@ -279,8 +279,8 @@ __global__ void wave_desc_kernel(int nslots, const int32_t *__restrict__ slots,
     if (i >= nslots) return;
     const int p = slots[3 * i], j = slots[3 * i + 1], kp = slots[3 * i + 2];
     if (p < 0) {
-        desc[2 * i] = make_int4(-1, 0, 0, 0);
-        desc[2 * i + 1] = make_int4(0, 0, 1, 0);
+        desc[4 * i] = make_int4(-1, 0, 0, 0);
+        desc[4 * i + 1] = make_int4(0, 0, 1, 0);
         return;
     }
     const int64_t ncols = col_ptr[p + 1] - col_ptr[p];
@@ -298,8 +298,8 @@ __global__ void wave_desc_kernel(int nslots, const int32_t *__restrict__ slots,
     }
     const int64_t skip = c0 - cs;
     const int pc = g < ng ? (gl[g].x >> 16) : 0;
-    desc[2 * i] = make_int4(p, g, (int)skip, (int)c0);
-    desc[2 * i + 1] = make_int4((int)(vs + skip * pc), (int)c1, kp, j);
+    desc[4 * i] = make_int4(p, g, (int)skip, (int)c0);
+    desc[4 * i + 1] = make_int4((int)(vs + skip * pc), (int)c1, kp, j);
 }
 
 // Batch records of every wave slot (one thread per slot).  Slot i's chunk is
@@ -314,7 +314,7 @@ __global__ void wave_prog_kernel(int nslots, int U, int RS, const int2 *__restri
                                  int4 *__restrict__ desc, int32_t *__restrict__ prog) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nslots) return;
-    const int4 d0 = desc[2 * i], d1 = desc[2 * i + 1];
+    const int4 d0 = desc[4 * i], d1 = desc[4 * i + 1];
     const int p = d0.x;
     if (p < 0) return;
     const int64_t cbase = col_ptr[p];
@@ -345,7 +345,14 @@ __global__ void wave_prog_kernel(int nslots, int U, int RS, const int2 *__restri
     rec[0] = -1;  // terminator: its rows (column 0) and values (index v) stay valid reads
     rec[1] = (int)v;
     for (int u = 0; u < U; ++u) rec[2 + u] = 0;
-    desc[2 * i] = make_int4(d0.x, d0.y, (int)start, d0.w);
+    desc[4 * i] = make_int4(d0.x, d0.y, (int)start, d0.w);
+    // The first record also sits in the descriptor's second half (when it
+    // fits), so a wave issues its first B loads after one scalar load.
+    if (2 + U <= 8) {
+        int *w = reinterpret_cast<int *>(desc + 4 * i + 2);
+        const int32_t *r0 = prog + start * RS;
+        for (int k = 0; k < 2 + U; ++k) w[k] = r0[k];
+    }
 }
 
 // ------------------------------------------------------- entry-list variants

@@ -132,7 +132,7 @@ struct Packed {
 // kp[p] = min(nsplit, max(1, ceil(ncols_p / chunk_cols))) waves; panels are
 // laid into workgroups of waves_per_block wave slots in panel order.
 struct Schedule {
-    int32_t *desc = nullptr;  // device: two int4 per wave slot (see exec_kernel.h)
+    int32_t *desc = nullptr;  // device: four int4 per wave slot (see exec_kernel.h)
     int32_t *prog = nullptr;  // device: batch records of every wave slot
     int unroll = 0;           // U the records are cut for
     int n_wblocks = 0;
