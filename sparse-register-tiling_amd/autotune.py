@@ -31,9 +31,9 @@ DEFAULT_SHAPES = ([dict(b_staging=1, vec_width=1, unroll=u, waves_per_block=16, 
                   + [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nsplit=s)
                      for v in (2, 4) for u in (4, 8) for (w, s) in ((8, 8), (16, 16), (4, 4))])
 # The reduced space the shipped table is tuned over (every pruned-FFN workload):
-# lane width x workgroup shape x K tile, global-memory B path, U = 4.
-TABLE_SHAPES = [dict(b_staging=2, vec_width=v, unroll=4, waves_per_block=w, nsplit=s, K_c=kc)
-                for v in (2, 4) for (w, s) in ((8, 8), (16, 16), (4, 4)) for kc in (0, 256)]
+# lane width x unroll x workgroup shape x K tile, global-memory B path.
+TABLE_SHAPES = [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nsplit=s, K_c=kc)
+                for v in (2, 4) for u in (2, 4) for (w, s) in ((8, 8), (16, 16), (4, 4)) for kc in (0, 256)]
 TABLE_MAPPINGS = [(4, "orig"), (8, "orig")]
 BUCKETS = [(0.0, 0.7), (0.7, 0.8), (0.8, 0.9), (0.9, 1.0)]
 
