@@ -478,7 +478,7 @@ def bench_cnn(args):
         t0 = time.perf_counter()
         recs = cnn.tune_fp32(layers, batch, X, mappings=[(4, "orig"), (8, "orig")], reps=10)
         best = recs[0]
-        shape = {k: best[k] for k in ("vec_width", "waves_per_block", "nsplit")}
+        shape = {k: best[k] for k in ("vec_width", "waves_per_block", "nsplit", "K_c")}
         net = cnn.CnnFp32(layers, batch, X, mapping=(best["M_r"], best["mapping"]), shape=shape)
         log("batch %d fp32 tuned in %.1f s: %s" % (batch, time.perf_counter() - t0, best))
         set_bytes = sum(x.numel() * 4 for x in X) + sum(y.numel() * 4 for y in net.Y)
