@@ -223,11 +223,12 @@ struct Elt<true> {
 // B arrives transposed (Bt: N x ldb, ldb >= KB * 32, k past K zero), so a B
 // tile is NT rows of 32 contiguous k: a straight 16-byte copy into LDS, and
 // each MFMA B operand is one 16-byte (8-byte) LDS read.  Software pipeline,
-// one barrier per K block: the B chunks and A operands of block kb+1 are
-// loaded into registers while the MFMAs of block kb run from the other LDS
-// buffer (the ring below keeps several blocks in flight).
+// one barrier per kStageBlocks K blocks: the B chunks and A operands of later
+// blocks are loaded into registers while the MFMAs of the current stage run
+// from the other LDS buffer (the ring below keeps several blocks in flight).
+constexpr int kStageBlocks = 2;
 template <bool FP8, int NT>
-using TileBuf = typename Elt<FP8>::T[2][NT * Elt<FP8>::kLdsK];
+using TileBuf = typename Elt<FP8>::T[2][kStageBlocks * NT * Elt<FP8>::kLdsK];
 
 // One output tile: panels (pg * WP + wave) * PW .. + PW, columns nt * NT ..,
 // K range z of ks.  Every wave of the workgroup calls it (barriers inside).
@@ -300,43 +301,52 @@ __device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, 
         load_a(areg[j], kb);
     }
     int buf = 0;
+    static_assert(DEPTH % kStageBlocks == 0, "the ring holds whole stages");
+    constexpr int SUB = NT * LDK;  // one K block's tile inside a stage buffer
     for (int base = kb0; base < kb1; base += DEPTH) {
 #pragma unroll
-        for (int j = 0; j < DEPTH; ++j) {
-            const int kb = base + j;
-            const bool run = kb < kb1;  // uniform; every wave takes every barrier
+        for (int j0 = 0; j0 < DEPTH; j0 += kStageBlocks) {
 #pragma unroll
-            for (int c = 0; c < CPT; ++c) {
-                const int ch = threadIdx.x + c * NTHR;
-                if (ch < CHUNKS) {
-                    const bool in = n0 + ch / CPR < N;
-                    *reinterpret_cast<uint4 *>(&tile[buf][(ch / CPR) * LDK + (ch % CPR) * PER]) =
-                        in ? breg[j][c] : uint4{0u, 0u, 0u, 0u};
+            for (int h = 0; h < kStageBlocks; ++h) {
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const int ch = threadIdx.x + c * NTHR;
+                    if (ch < CHUNKS) {
+                        const bool in = n0 + ch / CPR < N;
+                        *reinterpret_cast<uint4 *>(&tile[buf][h * SUB + (ch / CPR) * LDK + (ch % CPR) * PER]) =
+                            in ? breg[j0 + h][c] : uint4{0u, 0u, 0u, 0u};
+                    }
                 }
             }
             __syncthreads();
-            AOp a[PW];
 #pragma unroll
-            for (int i = 0; i < PW; ++i) a[i] = areg[j][i];
-            {
-                const int kn = kb + DEPTH < kb1 ? kb + DEPTH : kb1 - 1;
-                load_b(breg[j], kn);
-                load_a(areg[j], kn);
-            }
+            for (int h = 0; h < kStageBlocks; ++h) {
+                const int j = j0 + h;
+                const int kb = base + j;
+                const bool run = kb < kb1;  // uniform; every wave takes every barrier
+                AOp a[PW];
 #pragma unroll
-            for (int t = 0; t < TN && run; ++t) {
-                const T *bp = &tile[buf][(t * 16 + col) * LDK + kg * 8];
-                if constexpr (FP8) {
-                    const long bw = *reinterpret_cast<const long *>(bp);
+                for (int i = 0; i < PW; ++i) a[i] = areg[j][i];
+                {
+                    const int kn = kb + DEPTH < kb1 ? kb + DEPTH : kb1 - 1;
+                    load_b(breg[j], kn);
+                    load_a(areg[j], kn);
+                }
 #pragma unroll
-                    for (int i = 0; i < PW; ++i)
-                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], bw, acc[i][t], 0, 0, 0);
-                } else {
-                    const bf16x8 bw = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(bp));
+                for (int t = 0; t < TN && run; ++t) {
+                    const T *bp = &tile[buf][h * SUB + (t * 16 + col) * LDK + kg * 8];
+                    if constexpr (FP8) {
+                        const long bw = *reinterpret_cast<const long *>(bp);
 #pragma unroll
-                    for (int i = 0; i < PW; ++i)
-                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[i]), bw,
-                                                                            acc[i][t], 0, 0, 0);
+                        for (int i = 0; i < PW; ++i)
+                            acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[i], bw, acc[i][t], 0, 0, 0);
+                    } else {
+                        const bf16x8 bw = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(bp));
+#pragma unroll
+                        for (int i = 0; i < PW; ++i)
+                            acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[i]),
+                                                                                bw, acc[i][t], 0, 0, 0);
+                    }
                 }
             }
             buf ^= 1;
