@@ -70,6 +70,7 @@ def parse():
                     help="serving: offered loads (requests/s)")
     ap.add_argument("--requests", type=int, default=1500, help="serving: requests per load level")
     ap.add_argument("--slo-ms", type=float, default=2.0, help="serving: p99 target for the throughput ratio")
+    ap.add_argument("--mfma-shape", type=int, default=6, help="serving: MFMA tile shape of the grouped launch (0-7)")
     ap.add_argument("--cnn-batches", default="1,4,8", help="cnn: images per batch (N = H*W*batch)")
     ap.add_argument("--sweep-cpu-seconds", type=float, default=0.3, help="sweep: host time per case")
     ap.add_argument("--steps", type=int, default=2000)
@@ -290,10 +291,10 @@ def bench_precision(args):
             plan = srt.precision.PanelPlan.from_dense(torch.from_numpy(A), dtype)
             Bt, t = plan.prepare_b(Bd)
             C = torch.empty((M, 512), device="cuda")
+            # the default launch: (64 columns, 8 waves, 1 panel) tiles; K split 4
+            # for the 64-panel 1024x4096 weights (best of a graph-timed sweep
+            # over all 8 shapes x K splits 1-8 on MI355X)
             shape, ks = srt.precision.default_launch(plan.stats()["num_panels"], plan.stats()["k_blocks"], 512)
-            # measured best launch per shape family (graph-timed sweep on MI355X)
-            best = (4, 1) if M == 4096 and dtype == "fp8" else ((0, 1) if M == 4096 else (0 if dtype == "bf16" else 2, 8))
-            shape, ks = best
             plan.reserve(512, ks)
             sec = at.time_graph(lambda: plan(C, Bt, t, shape=shape, ksplit=ks), reps=50)
             plan(C, Bt, t, shape=shape, ksplit=ks)
@@ -342,7 +343,7 @@ def bench_serving(args):
         plans32.append(srt.MatMulSpecialized.from_csr(csr, 512, mapping_id=srt.named_mapping(4, "orig"),
                                                       tile_config=srt.TileConfig(xcd_remap=1)))
     log("packed 48 weights in %.1f s" % (time.perf_counter() - t0))
-    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, grid=1024)
+    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, shape=args.mfma_shape, grid=1024)
     levels = []
     for qps in [float(q) for q in args.qps.split(",")]:
         reqs = sv.poisson_requests(args.requests, qps, shapes, seed=int(qps) % 1000 + 1)

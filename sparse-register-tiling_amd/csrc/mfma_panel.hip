@@ -523,7 +523,8 @@ static hipError_t launch_exec_t(const MfmaLaunch &a, hipStream_t s) {
 }
 
 // Launch shapes: (N tile, waves per workgroup, panels per wave).
-constexpr int kShapes[][3] = {{64, 4, 2}, {128, 4, 2}, {64, 4, 1}, {128, 2, 2}, {32, 4, 1}, {128, 4, 1}};
+constexpr int kShapes[][3] = {{64, 4, 2}, {128, 4, 2}, {64, 4, 1}, {128, 2, 2}, {32, 4, 1}, {128, 4, 1},
+                              {64, 8, 1}, {128, 8, 1}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <bool FP8>
@@ -535,6 +536,8 @@ static hipError_t launch_exec_dt(const MfmaLaunch &a, hipStream_t s) {
         case 3: return launch_exec_t<FP8, 128, 2, 2>(a, s);
         case 4: return launch_exec_t<FP8, 32, 4, 1>(a, s);
         case 5: return launch_exec_t<FP8, 128, 4, 1>(a, s);
+        case 6: return launch_exec_t<FP8, 64, 8, 1>(a, s);
+        case 7: return launch_exec_t<FP8, 128, 8, 1>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -555,6 +558,8 @@ static hipError_t launch_grouped_dt(int shape, const MfmaProblem *probs, const i
         case 3: return launch_grouped_t<FP8, 128, 2, 2>(probs, hdr, grid, s);
         case 4: return launch_grouped_t<FP8, 32, 4, 1>(probs, hdr, grid, s);
         case 5: return launch_grouped_t<FP8, 128, 4, 1>(probs, hdr, grid, s);
+        case 6: return launch_grouped_t<FP8, 64, 8, 1>(probs, hdr, grid, s);
+        case 7: return launch_grouped_t<FP8, 128, 8, 1>(probs, hdr, grid, s);
         default: return hipErrorInvalidValue;
     }
 }

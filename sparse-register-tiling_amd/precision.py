@@ -27,7 +27,8 @@ from .native import ptr, stream_handle
 
 DTYPES = {"bf16": _lib.SRT_DT_BF16, "fp8": _lib.SRT_DT_FP8}
 # launch shapes: index -> (N tile, waves per workgroup, panels per wave)
-SHAPES = {0: (64, 4, 2), 1: (128, 4, 2), 2: (64, 4, 1), 3: (128, 2, 2), 4: (32, 4, 1), 5: (128, 4, 1)}
+SHAPES = {0: (64, 4, 2), 1: (128, 4, 2), 2: (64, 4, 1), 3: (128, 2, 2), 4: (32, 4, 1), 5: (128, 4, 1),
+          6: (64, 8, 1), 7: (128, 8, 1)}
 
 
 def _index_tensor(x, device) -> torch.Tensor:
@@ -37,10 +38,11 @@ def _index_tensor(x, device) -> torch.Tensor:
     return t.contiguous()
 
 
-def default_launch(num_panels: int, k_blocks: int, n: int, target_groups: int = 512) -> Tuple[int, int]:
+def default_launch(num_panels: int, k_blocks: int, n: int, target_groups: int = 256) -> Tuple[int, int]:
     """(shape, ksplit) giving about `target_groups` workgroups: 64-column
-    tiles of 8 panels, the K range split when panels x tiles are too few."""
-    shape = 0
+    tiles of 8 panels (one per wave, 8 waves), the K range split when panels
+    x tiles are too few."""
+    shape = 6
     nt, wp, pw = SHAPES[shape]
     groups = -(-num_panels // (wp * pw)) * -(-n // nt)
     ks = 1

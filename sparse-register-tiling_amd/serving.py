@@ -174,7 +174,7 @@ class Server:
     """Resident weights + admission + continuous batching (see module doc)."""
 
     def __init__(self, weights: Sequence[PanelPlan], budget_bytes: int = 256 << 20, max_batch: int = 64,
-                 slots: int = 3, shape: int = 0, grid: int = 1024, use_graphs: bool = True):
+                 slots: int = 3, shape: int = 6, grid: int = 1024, use_graphs: bool = True):
         if not weights:
             raise ValueError("no weights")
         dt = {w.dtype for w in weights}
