@@ -85,6 +85,13 @@ typedef struct srt_tile_config {
                               N_c columns) into LDS once and read packed columns
                               from there; 2 = read B rows from global memory per
                               packed column; 0 = choose (global memory)          */
+    int layout;            /* wave slots -> workgroups: 1 = panels in order, a panel
+                              on kp waves as chunk_cols says; 2 = balanced: panels
+                              binned heaviest-first into the least-loaded of G
+                              workgroups (G x N tiles = whole rounds of the chip's
+                              CUs at about chunk_cols columns per wave), each
+                              workgroup's slots then given one by one to its
+                              panel with the longest chunk; 0 = choose            */
 } srt_tile_config;
 
 /* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54,

@@ -101,7 +101,7 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
             rec = {"M_r": M_r, "mapping": name, "seconds": t, "padding": plan.stats()["padding"]}
             rec.update(shape)
             rec["realised"] = {k: getattr(real, k) for k in ("b_staging", "vec_width", "waves_per_block",
-                                                             "nsplit", "chunk_cols", "K_c")}
+                                                             "nsplit", "chunk_cols", "K_c", "layout")}
             out.append(rec)
             plan.close()
         base.close()

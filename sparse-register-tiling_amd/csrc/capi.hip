@@ -9,8 +9,10 @@
 #include <cstring>
 #include <deque>
 #include <fstream>
+#include <functional>
 #include <memory>
 #include <mutex>
+#include <queue>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -288,6 +290,7 @@ void srt_tile_config_default(srt_tile_config *c) {
     c->nsplit = 0;
     c->chunk_cols = 0;
     c->b_staging = 0;
+    c->layout = 0;
 }
 
 // ----------------------------------------------------------------- mapping
@@ -402,6 +405,19 @@ static bool lds_fits(bool stage, int K, int waves, int M_r, int V, bool reduce) 
     return std::max(red, tile_lds(K, V)) <= lds_limit();
 }
 
+// Compute units of the current device (256 on an MI355X).
+static int cu_count() {
+    static int cached[64];  // per device, 0 = not queried
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cached[dev] = v;
+    }
+    return cached[dev];
+}
+
 // Realise the GPU fields of a tile config for a matrix of K columns with
 // `total_cols` packed columns and B width n (the choices the reference's
 // heuristic JSON made per sparsity bucket are made here from the launch
@@ -433,6 +449,7 @@ static void realise_config(srt_tile_config &c, int M_r, int K, int K_tile, int64
     }
     if (c.xcd_remap != 0 && c.xcd_remap != 1) c.xcd_remap = 1;
     if (!pow2_upto8(c.unroll)) c.unroll = 4;
+    if (c.layout != 1 && c.layout != 2) c.layout = 1;
     c.M_c = M_r * c.waves_per_block;
     c.N_c = 64 * c.vec_width;
     c.K_c = K_tile;
@@ -595,44 +612,144 @@ static int schedule_unroll(const ExecutorDesc *exec, int want) {
     return want;
 }
 
-static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const ExecutorDesc *exec, Schedule &out,
-                          hipStream_t s) {
+// N tiles of a launch of width n under a realised config.
+static int config_tiles(const srt_tile_config &c, int n) {
+    const int nt = 64 * std::max(1, c.vec_width);
+    return std::max(1, (n + nt - 1) / nt);
+}
+
+// Layout 2: panels binned heaviest first (ties in panel order) into the
+// least-loaded of G workgroups that still has a free slot, G per N tile such
+// that G x n_tiles is a whole number of rounds of the chip's CUs at about
+// chunk_cols packed columns per wave (so no CU runs one workgroup more than
+// another); then each workgroup's free slots go one at a time to its panel
+// whose chunk is longest, while that chunk is longer than 2 * kMinChunk.
+// Fills kp and appends (p, j, kp) per slot, W slots per workgroup.
+static void balanced_slots(const Packed &pk, Schedule &sch, int n_tiles, std::vector<int32_t> &slots) {
+    constexpr int64_t kMinChunk = 8;
+    const int W = sch.waves_per_block, P = pk.P;
+    std::vector<int64_t> nc(P);
+    int64_t total = 0;
+    for (int p = 0; p < P; ++p) {
+        nc[p] = pk.h_col_ptr[p + 1] - pk.h_col_ptr[p];
+        total += nc[p];
+    }
+    const int64_t cus = cu_count();
+    const int64_t per_round = cus * W * (int64_t)sch.chunk_cols;
+    const int64_t rounds = std::max<int64_t>(1, (total * n_tiles + per_round / 2) / per_round);
+    int64_t G = (rounds * cus + n_tiles - 1) / n_tiles;
+    G = std::min<int64_t>(G, P);
+    G = std::max<int64_t>(G, (P + W - 1) / W);
+    G = std::max<int64_t>(G, 1);
+    std::vector<int> order(P);
+    for (int p = 0; p < P; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nc[a] > nc[b]; });
+    std::vector<std::vector<int>> bins((size_t)G);
+    std::vector<int64_t> load((size_t)G, 0);
+    typedef std::pair<int64_t, int64_t> Entry;  // (load, bin): the smallest pops first
+    std::priority_queue<Entry, std::vector<Entry>, std::greater<Entry>> heap;
+    for (int64_t b = 0; b < G; ++b) heap.push(Entry(0, b));
+    for (int p : order) {  // G * W >= P, so a bin with a free slot remains
+        const int64_t b = heap.top().second;
+        heap.pop();
+        bins[b].push_back(p);
+        load[b] += nc[p];
+        if ((int)bins[b].size() < W) heap.push(Entry(load[b], b));
+    }
+    for (int64_t b = 0; b < G; ++b) {
+        const std::vector<int> &L = bins[b];
+        std::vector<int> k(L.size(), 1);
+        for (int free = W - (int)L.size(); free > 0; --free) {
+            int best = -1;
+            int64_t longest = 2 * kMinChunk;
+            for (size_t i = 0; i < L.size(); ++i) {
+                const int64_t len = (nc[L[i]] + k[i] - 1) / k[i];
+                if (k[i] < sch.nsplit && len > longest) {
+                    longest = len;
+                    best = (int)i;
+                }
+            }
+            if (best < 0) break;
+            ++k[best];
+        }
+        int used = 0;
+        for (size_t i = 0; i < L.size(); ++i) {
+            sch.kp[L[i]] = k[i];
+            if (k[i] > 1) sch.reduce = 1;
+            for (int j = 0; j < k[i]; ++j) {
+                slots.push_back(L[i]);
+                slots.push_back(j);
+                slots.push_back(k[i]);
+            }
+            used += k[i];
+        }
+        for (; used < W; ++used) {
+            slots.push_back(-1);
+            slots.push_back(0);
+            slots.push_back(1);
+        }
+    }
+}
+
+static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const ExecutorDesc *exec, int n_tiles,
+                          Schedule &out, hipStream_t s) {
     Schedule sch;
     sch.unroll = schedule_unroll(exec, pow2_upto8(cfg.unroll) ? cfg.unroll : 4);
     sch.waves_per_block = cfg.waves_per_block;
     sch.nsplit = std::min(cfg.nsplit, cfg.waves_per_block);
     sch.chunk_cols = std::max(1, cfg.chunk_cols);
+    sch.layout = cfg.layout;
+    sch.n_tiles = n_tiles;
     sch.kp.assign(pk.P, 1);
     std::vector<int32_t> slots;  // (p, j, kp) per slot
-    int used = 0;
-    auto close_block = [&]() {
-        for (; used > 0 && used < sch.waves_per_block; ++used) {
-            slots.push_back(-1);
-            slots.push_back(0);
-            slots.push_back(1);
+    if (cfg.layout == 2) {
+        balanced_slots(pk, sch, n_tiles, slots);
+    } else {
+        int used = 0;
+        auto close_block = [&]() {
+            for (; used > 0 && used < sch.waves_per_block; ++used) {
+                slots.push_back(-1);
+                slots.push_back(0);
+                slots.push_back(1);
+            }
+            used = 0;
+        };
+        for (int p = 0; p < pk.P; ++p) {
+            const int64_t nc = pk.h_col_ptr[p + 1] - pk.h_col_ptr[p];
+            const int64_t want = (nc + sch.chunk_cols - 1) / sch.chunk_cols;
+            const int kp = (int)std::max<int64_t>(1, std::min<int64_t>(sch.nsplit, want));
+            sch.kp[p] = kp;
+            if (kp > 1) sch.reduce = 1;
+            if (used + kp > sch.waves_per_block) close_block();
+            for (int j = 0; j < kp; ++j) {
+                slots.push_back(p);
+                slots.push_back(j);
+                slots.push_back(kp);
+            }
+            used += kp;
         }
-        used = 0;
-    };
-    for (int p = 0; p < pk.P; ++p) {
-        const int64_t nc = pk.h_col_ptr[p + 1] - pk.h_col_ptr[p];
-        const int64_t want = (nc + sch.chunk_cols - 1) / sch.chunk_cols;
-        const int kp = (int)std::max<int64_t>(1, std::min<int64_t>(sch.nsplit, want));
-        sch.kp[p] = kp;
-        if (kp > 1) sch.reduce = 1;
-        if (used + kp > sch.waves_per_block) close_block();
-        for (int j = 0; j < kp; ++j) {
-            slots.push_back(p);
-            slots.push_back(j);
-            slots.push_back(kp);
-        }
-        used += kp;
+        close_block();
     }
-    close_block();
     const int nslots = (int)(slots.size() / 3);
     sch.n_wblocks = nslots / std::max(1, sch.waves_per_block);
+    // Record run of each slot: its chunk's columns plus a terminator, runs
+    // laid out in slot order.
+    std::vector<int64_t> rec_start((size_t)std::max(nslots, 1), 0);
+    int64_t nrun = 0;
+    for (int i = 0; i < nslots; ++i) {
+        rec_start[i] = nrun;
+        const int p = slots[3 * i], j = slots[3 * i + 1], kp = slots[3 * i + 2];
+        if (p >= 0) {
+            const int64_t nc = pk.h_col_ptr[p + 1] - pk.h_col_ptr[p];
+            nrun += (int64_t)(j + 1) * nc / kp - (int64_t)j * nc / kp;
+        }
+        nrun += 1;
+    }
     Scratch sc;
     int32_t *d_slots = nullptr;
+    int64_t *d_start = nullptr;
     SRT_TRY(sc.get(&d_slots, slots.size()));
+    SRT_TRY(sc.get(&d_start, rec_start.size()));
     SRT_TRY(dmalloc(&sch.desc, 16 * (size_t)std::max(nslots, 1)));
     if (nslots) {
         SRT_HIP_TRY(hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * slots.size(),
@@ -641,18 +758,20 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
                                      pk.id_pattern, sch.desc, s));
     }
     // Batch records: at most one per packed column of each chunk plus a
-    // terminator per slot, and two records of look-ahead padding.
+    // terminator per slot (nrun in all), and two records of look-ahead padding.
     if (pk.total_vals + SRT_VAL_PAD > (int64_t)0x7fffffffLL)
         return fail(SRT_ERR_UNSUPPORTED, "more than 2^31 packed values");
-    const size_t nrec = (size_t)pk.total_cols + (size_t)nslots + 2;
+    const size_t nrec = (size_t)nrun + 2;
     const int RS = rec_stride(sch.unroll);
     if (nrec * RS * sizeof(int32_t) > (size_t)0xffffffffULL)
         return fail(SRT_ERR_UNSUPPORTED, "batch records exceed 4 GiB");
     SRT_TRY(dmalloc(&sch.prog, nrec * RS));
     SRT_HIP_TRY(hipMemsetAsync(sch.prog, 0, sizeof(int32_t) * nrec * RS, s));
-    if (nslots)
+    if (nslots) {
+        SRT_HIP_TRY(hipMemcpyAsync(d_start, rec_start.data(), sizeof(int64_t) * nslots, hipMemcpyHostToDevice, s));
         SRT_HIP_TRY(launch_wave_prog(nslots, sch.unroll, pk.groups, pk.grp_ptr, pk.col_ptr, pk.val_ptr, pk.cols,
-                                     sch.desc, sch.prog, s));
+                                     d_start, sch.desc, sch.prog, s));
+    }
     SRT_HIP_TRY(hipStreamSynchronize(s));
     out.release();
     out = std::move(sch);
@@ -817,7 +936,8 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
     plan->n_max = n;
     plan->cfg = plan->cfg_user;
     realise_config(plan->cfg, map.M_r, k, plan->pk.K_c, plan->pk.total_cols, std::max(n, 1));
-    rc = build_schedule(plan->pk, plan->cfg, plan->exec, plan->sch, (hipStream_t)stream);
+    rc = build_schedule(plan->pk, plan->cfg, plan->exec, config_tiles(plan->cfg, std::max(n, 1)), plan->sch,
+                        (hipStream_t)stream);
     if (rc) {
         plan->pk.release();
         return rc;
@@ -826,9 +946,11 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
     return SRT_OK;
 }
 
-static bool same_schedule(const srt_tile_config &a, const srt_tile_config &b) {
+// Whether a schedule built for config b (n_tiles N tiles) serves config a at
+// a_tiles N tiles (only the balanced layout depends on the tile count).
+static bool same_schedule(const srt_tile_config &a, int a_tiles, const srt_tile_config &b, const Schedule &sch) {
     return a.nsplit == b.nsplit && a.chunk_cols == b.chunk_cols && a.waves_per_block == b.waves_per_block &&
-           a.unroll == b.unroll;
+           a.unroll == b.unroll && a.layout == b.layout && (a.layout != 2 || a_tiles == sch.n_tiles);
 }
 
 int srt_plan_get_config(const srt_plan *p, srt_tile_config *out) {
@@ -852,7 +974,8 @@ int srt_plan_allocate_executor(srt_plan *p, int n) {
     // schedule if they changed.  Not to be called while the plan executes.
     srt_tile_config c = p->cfg_user;
     realise_config(c, p->pk.M_r, p->pk.K, p->pk.K_c, p->pk.total_cols, std::max(n, 1));
-    if (!same_schedule(c, p->cfg)) SRT_TRY(build_schedule(p->pk, c, p->exec, p->sch, nullptr));
+    const int tiles = config_tiles(c, std::max(n, 1));
+    if (!same_schedule(c, tiles, p->cfg, p->sch)) SRT_TRY(build_schedule(p->pk, c, p->exec, tiles, p->sch, nullptr));
     p->cfg = c;
     return SRT_OK;
 }
@@ -1259,8 +1382,9 @@ int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, fl
     else srt_tile_config_default(&c);
     realise_config(c, t->pk.M_r, t->pk.K, t->pk.K_c ? t->pk.K_c : t->pk.K, t->pk.total_cols, std::max(N_c, 1));
     std::lock_guard<std::mutex> g(t->mu);
-    if (!t->sch.desc || !same_schedule(c, t->sch_cfg)) {
-        SRT_TRY(build_schedule(t->pk, c, t->exec, t->sch, (hipStream_t)stream));
+    const int tiles = config_tiles(c, std::max(N_c, 1));
+    if (!t->sch.desc || !same_schedule(c, tiles, t->sch_cfg, t->sch)) {
+        SRT_TRY(build_schedule(t->pk, c, t->exec, tiles, t->sch, (hipStream_t)stream));
         t->sch_cfg = c;
     }
     return run_executor(t->pk, t->sch, t->exec, c, B, ldb, C, ldc, N_c, load_c ? 1 : 0,

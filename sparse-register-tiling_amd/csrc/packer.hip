@@ -304,14 +304,16 @@ __global__ void wave_desc_kernel(int nslots, const int32_t *__restrict__ slots,
 
 // Batch records of every wave slot (one thread per slot).  Slot i's chunk is
 // packed columns [c0, c1) of panel p (its descriptor); its records start at
-// record index col_ptr[p] + c0 + i, so the runs of consecutive slots never
-// overlap (a run holds at most c1 - c0 batches and a terminator).  The start
-// index replaces the descriptor's group skip (d0.z), which only this builder
-// reads.
+// record index rec_start[i], the sum over the slots before it of their chunk
+// lengths plus one, so no two runs overlap whatever order the slots hold the
+// panels in (a run holds at most c1 - c0 batches and a terminator).  The
+// start index replaces the descriptor's group skip (d0.z), which only this
+// builder reads.
 __global__ void wave_prog_kernel(int nslots, int U, int RS, const int2 *__restrict__ groups,
                                  const int64_t *__restrict__ grp_ptr, const int64_t *__restrict__ col_ptr,
                                  const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ cols,
-                                 int4 *__restrict__ desc, int32_t *__restrict__ prog) {
+                                 const int64_t *__restrict__ rec_start, int4 *__restrict__ desc,
+                                 int32_t *__restrict__ prog) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nslots) return;
     const int4 d0 = desc[4 * i], d1 = desc[4 * i + 1];
@@ -321,7 +323,7 @@ __global__ void wave_prog_kernel(int nslots, int U, int RS, const int2 *__restri
     int64_t c = cbase + d0.w;
     const int64_t cend = cbase + d1.y;
     int64_t v = val_ptr[p] + d1.x;
-    const int64_t start = c + i;
+    const int64_t start = rec_start[i];
     int32_t *rec = prog + start * RS;
     const int2 *g = groups + grp_ptr[p] + d0.y;
     int skip = d0.z;
@@ -566,10 +568,10 @@ hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t
 
 hipError_t launch_wave_prog(int nslots, int U, const int32_t *groups, const int64_t *grp_ptr,
                             const int64_t *col_ptr, const int64_t *val_ptr, const int32_t *cols,
-                            int32_t *desc, int32_t *prog, hipStream_t s) {
+                            const int64_t *rec_start, int32_t *desc, int32_t *prog, hipStream_t s) {
     if (nslots <= 0) return hipSuccess;
     hipLaunchKernelGGL(wave_prog_kernel, dim3((nslots + 255) / 256), dim3(256), 0, s, nslots, U, rec_stride(U),
-                       reinterpret_cast<const int2 *>(groups), grp_ptr, col_ptr, val_ptr, cols,
+                       reinterpret_cast<const int2 *>(groups), grp_ptr, col_ptr, val_ptr, cols, rec_start,
                        reinterpret_cast<int4 *>(desc), prog);
     return hipGetLastError();
 }

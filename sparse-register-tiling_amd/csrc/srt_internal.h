@@ -139,6 +139,8 @@ struct Schedule {
     int waves_per_block = 0;
     int nsplit = 0;
     int chunk_cols = 0;
+    int layout = 1;           // realised tile config layout (1 panel order, 2 balanced)
+    int n_tiles = 1;          // N tiles the balanced layout was sized for
     int reduce = 0;
     std::vector<int32_t> kp;  // per panel
     void release();
@@ -164,7 +166,7 @@ hipError_t launch_group_fill(int P, int NK, const int32_t *counts, const int64_t
 // Wave descriptors from (panel, j, kp) triples (slots[3 * i]; panel -1 = idle).
 hipError_t launch_wave_prog(int nslots, int U, const int32_t *groups, const int64_t *grp_ptr,
                             const int64_t *col_ptr, const int64_t *val_ptr, const int32_t *cols,
-                            int32_t *desc, int32_t *prog, hipStream_t s);
+                            const int64_t *rec_start, int32_t *desc, int32_t *prog, hipStream_t s);
 hipError_t launch_wave_desc(int nslots, const int32_t *slots, const int32_t *groups,
                             const int64_t *grp_ptr, const int64_t *col_ptr,
                             const int32_t *id_pattern, int32_t *desc, hipStream_t s);

@@ -40,8 +40,10 @@ class TileConfig:
     vec_width (fp32 columns per lane), waves_per_block (panels per
     workgroup), xcd_remap, unroll (packed columns in flight per wave),
     nsplit (waves sharing one panel; 1 keeps the reference's single chain),
-    chunk_cols (target packed columns per wave) and b_staging (1: B tile
-    staged in LDS, 2: B rows read from global memory)."""
+    chunk_cols (target packed columns per wave), b_staging (1: B tile
+    staged in LDS, 2: B rows read from global memory) and layout (1: panels
+    laid into workgroups in order; 2: panels binned heaviest-first into
+    whole rounds of the chip's CUs, slots given to the longest chunks)."""
     M_c: int = 16
     K_c: int = 0
     N_c: int = 64
@@ -58,6 +60,7 @@ class TileConfig:
     nsplit: int = 0
     chunk_cols: int = 0
     b_staging: int = 0
+    layout: int = 0
 
     def to_c(self) -> _lib.TileConfigC:
         c = _lib.TileConfigC()

@@ -1,0 +1,100 @@
+"""GPU parity of the balanced workgroup layout (TileConfig.layout = 2: panels
+binned heaviest-first into whole rounds of the chip's CUs, wave slots given
+to the longest chunks, so slots no longer hold the panels in order): C is
+bit-identical to the oracle's chain over the same packed data with the
+plan's per-panel chunk splits, on uneven rows, empty panels, row and column
+tails, K tiles, every lane width and workgroup size, and after the schedule
+is rebuilt for another B width."""
+import sys
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import _oracle_table, _plan, _run, _bitexact
+
+pytestmark = pytest.mark.gpu
+
+
+def _uneven(M, K, density, seed):
+    """Rows of very different lengths and a block of empty rows (whole empty
+    panels), values N(0, 1)."""
+    rng = np.random.default_rng(seed)
+    scale = rng.lognormal(0.0, 1.0, (M, 1))
+    p = np.clip(density * scale / scale.mean(), 0.0, 1.0)
+    A = np.where(rng.random((M, K)) < p, rng.standard_normal((M, K)), 0.0).astype(np.float32)
+    A[M // 3:M // 3 + 24] = 0.0
+    return A
+
+
+def _check(srt, oracle, A, B, M_r, name, **cfg):
+    import torch
+    so, cb = oracle
+    plan = _plan(srt, torch, A, B.shape[1], M_r, name, layout=2, xcd_remap=1, **cfg)
+    real = plan.get_config()
+    assert real.layout == 2
+    pk = cb.pack_dense_ktiled(A, M_r, _oracle_table(so, M_r, name), real.K_c) if real.K_c < A.shape[1] \
+        else cb.pack_dense(A, M_r, _oracle_table(so, M_r, name))
+    C = _run(srt, torch, plan, B)
+    kp = plan.split_counts()
+    _bitexact(C, cb.exec_packed(pk, B, nsplit=kp))
+    assert np.all(np.abs(C - cb.dense_f64(A, B)) <= so.fp64_error_bound(A, B))
+    return plan, kp
+
+
+@pytest.mark.parametrize("W,S", [(16, 16), (16, 8), (8, 8), (4, 4), (8, 1)])
+@pytest.mark.parametrize("V", [1, 2, 4])
+@pytest.mark.parametrize("U", [2, 4])
+def test_balanced_layout_uneven(srt, oracle, W, S, V, U):
+    print("layout2 uneven", W, S, V, U, flush=True)
+    sys.stdout.flush()
+    A = _uneven(770, 300, 0.12, seed=5)
+    B = srt.workloads.dense_b(300, 264, seed=6)
+    plan, kp = _check(srt, oracle, A, B, 4, "orig", waves_per_block=W, nsplit=S, vec_width=V, unroll=U,
+                      chunk_cols=16)
+    assert kp.max() <= S
+    plan.close()
+
+
+@pytest.mark.parametrize("M_r,name", [(4, "identity"), (8, "orig"), (2, "identity")])
+@pytest.mark.parametrize("k_tile", [0, 128])
+def test_balanced_layout_mappings_and_k_tiles(srt, oracle, M_r, name, k_tile):
+    print("layout2", M_r, name, k_tile, flush=True)
+    sys.stdout.flush()
+    A = _uneven(1030, 520, 0.08, seed=7)
+    B = srt.workloads.dense_b(520, 200, seed=8)
+    plan, _ = _check(srt, oracle, A, B, M_r, name, K_c=k_tile, waves_per_block=16, nsplit=16)
+    plan.close()
+
+
+def test_balanced_layout_2048x512_workload(srt, oracle):
+    """2048 x 512 (90 %) x 512 with the balanced layout on 16-wave workgroups."""
+    A, B = srt.workloads.case_inputs(srt.workloads.NORTH_STAR, seed=0)
+    plan, kp = _check(srt, oracle, A, B, 4, "orig", K_c=256, waves_per_block=16, nsplit=16, vec_width=4,
+                      unroll=2, chunk_cols=24)
+    assert kp.max() > 1
+    plan.close()
+
+
+def test_balanced_layout_rebuilt_for_another_width(srt, oracle):
+    """allocate_executor with a wider B rebuilds the balanced schedule (its
+    workgroup count depends on the N tiles); results stay exact at both."""
+    import torch
+    so, cb = oracle
+    A = _uneven(600, 256, 0.1, seed=9)
+    plan = _plan(srt, torch, A, 64, 4, "orig", layout=2, waves_per_block=8, vec_width=1)
+    pk = cb.pack_dense(A, 4, _oracle_table(so, 4, "orig"))
+    for n in (64, 448):
+        plan.allocate_executor(n)
+        B = srt.workloads.dense_b(256, n, seed=n)
+        C = _run(srt, torch, plan, B)
+        _bitexact(C, cb.exec_packed(pk, B, nsplit=plan.split_counts()))
+    plan.close()
+
+
+def test_balanced_layout_tiny(srt, oracle):
+    """Fewer panels than one workgroup's slots, and a single row."""
+    for M, K, N in ((9, 5, 3), (1, 7, 64), (40, 33, 130)):
+        A = _uneven(M, K, 0.5, seed=M)
+        B = srt.workloads.dense_b(K, N, seed=K)
+        plan, _ = _check(srt, oracle, A, B, 4, "orig", waves_per_block=16, nsplit=16)
+        plan.close()
