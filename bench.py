@@ -54,7 +54,7 @@ COPY_BYTES = 256 << 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "executor_north_star_traffic.txt")
 EXEC_KERNEL = "exec_kernel"
 # realised tile-config fields that name a measured configuration
-GPU_FIELDS = ("K_c", "vec_width", "unroll", "waves_per_block", "nsplit", "chunk_cols", "b_staging")
+GPU_FIELDS = ("K_c", "vec_width", "unroll", "waves_per_block", "nsplit", "chunk_cols", "b_staging", "layout")
 
 
 def log(*a):
@@ -83,6 +83,8 @@ def parse():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--staging", type=int, default=0, help="B path: 1 LDS tile, 2 global rows, 0 choose")
     ap.add_argument("--k-tile", type=int, default=0, help="K_c: K tile of the packing (0 = one tile)")
+    ap.add_argument("--layout", type=int, default=0,
+                    help="workgroup layout: 1 panels in order, 2 balanced over the CUs (0 = choose)")
     ap.add_argument("--config", help="JSON file written by --save-config")
     ap.add_argument("--save-config", help="write the chosen configuration here")
     ap.add_argument("--replicas", type=int, default=0, help="0 = enough to exceed 2x the 256 MB cache")
@@ -242,13 +244,13 @@ def choose_config(args, srt, csr, N, Bd):
             csr.shape[0], csr.shape[1], 1.0 - csr.values().numel() / float(csr.shape[0] * csr.shape[1]), N)
         (M_r, name), cfg = autotune.best_config(recs)
         tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
-                                             "chunk_cols", "xcd_remap", "b_staging", "K_c")}
+                                             "chunk_cols", "xcd_remap", "b_staging", "K_c", "layout")}
         log("shipped tuning record:", recs[0])
     elif args.mapping == "auto":
         recs = autotune.tune(csr, N, reps=args.tune_reps, B=Bd)
         (M_r, name), cfg = autotune.best_config(recs)
         tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
-                                             "chunk_cols", "xcd_remap", "b_staging", "K_c")}
+                                             "chunk_cols", "xcd_remap", "b_staging", "K_c", "layout")}
         tune_top = [{k: (round(v * 1e6, 3) if k == "seconds" else v) for k, v in r.items()}
                     for r in recs[:5]]
         log("autotune best (us):", tune_top[0])
@@ -257,11 +259,12 @@ def choose_config(args, srt, csr, N, Bd):
         M_r = int(mr)
         tile = dict(vec_width=args.vec_width, unroll=args.unroll, waves_per_block=args.waves,
                     nsplit=args.nsplit, chunk_cols=args.chunk, xcd_remap=1, b_staging=args.staging,
-                    K_c=args.k_tile)
+                    K_c=args.k_tile, layout=args.layout)
     if args.save_config:
         flags = ("--mapping %d:%s --vec-width %d --unroll %d --waves %d --nsplit %d --chunk %d --staging %d "
-                 "--k-tile %d" % (M_r, name, tile["vec_width"], tile["unroll"], tile["waves_per_block"],
-                                  tile["nsplit"], tile["chunk_cols"], tile.get("b_staging", 0), tile.get("K_c", 0)))
+                 "--k-tile %d --layout %d" % (M_r, name, tile["vec_width"], tile["unroll"], tile["waves_per_block"],
+                                              tile["nsplit"], tile["chunk_cols"], tile.get("b_staging", 0),
+                                              tile.get("K_c", 0), tile.get("layout", 0)))
         with open(args.save_config, "w") as f:
             json.dump({"M_r": M_r, "mapping": name, "tile": tile, "flags": flags}, f)
     return M_r, name, srt.TileConfig(**tile), tune_top
@@ -428,7 +431,7 @@ def bench_sweep(args):
         plan.close()
         r = {"M": case.M, "K": case.K, "sparsity": case.sparsity, "N": case.N, "nnz": nnz,
              "mapping": "%d:%s" % best[1], "vec_width": best[2].vec_width, "K_c": real.K_c,
-             "waves": real.waves_per_block, "nsplit": real.nsplit,
+             "waves": real.waves_per_block, "nsplit": real.nsplit, "unroll": real.unroll, "layout": real.layout,
              "exec_us": round(best[0] * 1e6, 2), "exec_gflops": round(flop / best[0] / 1e9, 1),
              "exec_hbm_frac": round(bytes_alg / best[0] / 8e12, 4),
              "csr_us": round(t_csr * 1e6, 2), "csr_gflops": round(flop / t_csr / 1e9, 1),
@@ -479,7 +482,7 @@ def bench_cnn(args):
         t0 = time.perf_counter()
         recs = cnn.tune_fp32(layers, batch, X, mappings=[(4, "orig"), (8, "orig")], reps=10)
         best = recs[0]
-        shape = {k: best[k] for k in ("vec_width", "waves_per_block", "nsplit", "K_c")}
+        shape = {k: best[k] for k in ("vec_width", "waves_per_block", "nsplit", "K_c", "layout")}
         net = cnn.CnnFp32(layers, batch, X, mapping=(best["M_r"], best["mapping"]), shape=shape)
         log("batch %d fp32 tuned in %.1f s: %s" % (batch, time.perf_counter() - t0, best))
         set_bytes = sum(x.numel() * 4 for x in X) + sum(y.numel() * 4 for y in net.Y)

@@ -32,8 +32,9 @@ DEFAULT_SHAPES = ([dict(b_staging=1, vec_width=1, unroll=u, waves_per_block=16, 
                      for v in (2, 4) for u in (4, 8) for (w, s) in ((8, 8), (16, 16), (4, 4))])
 # The reduced space the shipped table is tuned over (every pruned-FFN workload):
 # lane width x unroll x workgroup shape x K tile, global-memory B path.
-TABLE_SHAPES = [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nsplit=s, K_c=kc)
-                for v in (2, 4) for u in (2, 4) for (w, s) in ((8, 8), (16, 16), (4, 4)) for kc in (0, 256)]
+TABLE_SHAPES = [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nsplit=s, K_c=kc, layout=lay)
+                for v in (2, 4) for u in (2, 4) for (w, s) in ((16, 16), (16, 8), (8, 8), (4, 4)) for kc in (0, 256)
+                for lay in (1, 2)]
 TABLE_MAPPINGS = [(4, "orig"), (8, "orig")]
 BUCKETS = [(0.0, 0.7), (0.7, 0.8), (0.8, 0.9), (0.9, 1.0)]
 
@@ -81,16 +82,20 @@ def time_plan(plan: MatMulSpecialized, B: torch.Tensor, C: torch.Tensor, reps: i
 
 def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAULT_MAPPINGS,
          shapes: Sequence[Dict] = DEFAULT_SHAPES, reps: int = 50,
-         B: Optional[torch.Tensor] = None) -> List[Dict]:
+         B: Optional[torch.Tensor] = None, refine: int = 4) -> List[Dict]:
     """Time every (mapping, shape) candidate; returns records sorted fastest
-    first.  A shape is a dict of TileConfig fields."""
+    first.  A shape is a dict of TileConfig fields.  The `refine` fastest
+    are timed again with three times the replays (their order decides the
+    choice, so timing noise of the short first pass does not)."""
     M, K = csr.shape
     if B is None:
         B = torch.rand((K, N), device="cuda") * 2 - 1
     C = torch.empty((M, N), device="cuda")
     out = []
+    bases = {}
     for M_r, name in mappings:
         base = MatMulSpecialized.from_csr(csr, N, mapping_id=named_mapping(M_r, name))
+        bases[(M_r, name)] = base
         for shape in shapes:
             if N % max(1, shape.get("vec_width", 1)):
                 continue
@@ -104,8 +109,16 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
                                                              "nsplit", "chunk_cols", "K_c", "layout")}
             out.append(rec)
             plan.close()
-        base.close()
     out.sort(key=lambda r: r["seconds"])
+    for rec in out[:refine]:
+        shape = {k: v for k, v in rec.items() if k in TileConfig.__dataclass_fields__}
+        plan = MatMulSpecialized.from_csr(csr, N, mapping_id=bases[(rec["M_r"], rec["mapping"])].mapping,
+                                          tile_config=TileConfig(xcd_remap=1, **shape))
+        rec["seconds"] = time_plan(plan, B, C, 3 * reps)
+        plan.close()
+    for base in bases.values():
+        base.close()
+    out[:refine] = sorted(out[:refine], key=lambda r: r["seconds"])
     return out
 
 

@@ -205,8 +205,9 @@ class GroupedPanels:
 # Candidates of the FP32 autotuner: mapping (panel height, pattern set) x
 # (lane width, waves per workgroup, waves per panel, K tile).
 FP32_MAPPINGS = [(4, "orig"), (8, "orig"), (4, "identity"), (8, "identity")]
-FP32_SHAPES = [dict(vec_width=v, waves_per_block=w, nsplit=s, K_c=kc)
-               for v in (4, 2) for (w, s) in ((8, 1), (8, 4), (8, 8), (16, 16), (4, 4)) for kc in (0, 256)]
+FP32_SHAPES = [dict(vec_width=v, waves_per_block=w, nsplit=s, K_c=kc, layout=lay)
+               for v in (4, 2) for (w, s) in ((8, 1), (8, 4), (8, 8), (16, 16), (4, 4)) for kc in (0, 256)
+               for lay in (1, 2)]
 
 
 class CnnFp32:

@@ -449,7 +449,7 @@ static void realise_config(srt_tile_config &c, int M_r, int K, int K_tile, int64
     }
     if (c.xcd_remap != 0 && c.xcd_remap != 1) c.xcd_remap = 1;
     if (!pow2_upto8(c.unroll)) c.unroll = 4;
-    if (c.layout != 1 && c.layout != 2) c.layout = 1;
+    if (c.layout != 1 && c.layout != 2) c.layout = 2;
     c.M_c = M_r * c.waves_per_block;
     c.N_c = 64 * c.vec_width;
     c.K_c = K_tile;

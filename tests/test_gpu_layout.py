@@ -26,12 +26,12 @@ def _uneven(M, K, density, seed):
     return A
 
 
-def _check(srt, oracle, A, B, M_r, name, **cfg):
+def _check(srt, oracle, A, B, M_r, name, layout=2, **cfg):
     import torch
     so, cb = oracle
-    plan = _plan(srt, torch, A, B.shape[1], M_r, name, layout=2, xcd_remap=1, **cfg)
+    plan = _plan(srt, torch, A, B.shape[1], M_r, name, layout=layout, xcd_remap=1, **cfg)
     real = plan.get_config()
-    assert real.layout == 2
+    assert real.layout == (layout or 2)
     pk = cb.pack_dense_ktiled(A, M_r, _oracle_table(so, M_r, name), real.K_c) if real.K_c < A.shape[1] \
         else cb.pack_dense(A, M_r, _oracle_table(so, M_r, name))
     C = _run(srt, torch, plan, B)
@@ -52,6 +52,27 @@ def test_balanced_layout_uneven(srt, oracle, W, S, V, U):
     plan, kp = _check(srt, oracle, A, B, 4, "orig", waves_per_block=W, nsplit=S, vec_width=V, unroll=U,
                       chunk_cols=16)
     assert kp.max() <= S
+    plan.close()
+
+
+@pytest.mark.parametrize("W,S", [(16, 16), (8, 8), (4, 2)])
+@pytest.mark.parametrize("V", [1, 4])
+def test_panel_order_layout_uneven(srt, oracle, W, S, V):
+    """layout = 1: panels laid into workgroups in panel order."""
+    print("layout1 uneven", W, S, V, flush=True)
+    sys.stdout.flush()
+    A = _uneven(770, 300, 0.12, seed=5)
+    B = srt.workloads.dense_b(300, 264, seed=6)
+    plan, kp = _check(srt, oracle, A, B, 4, "orig", layout=1, waves_per_block=W, nsplit=S, vec_width=V,
+                      chunk_cols=16)
+    assert kp.max() <= S
+    plan.close()
+
+
+def test_default_layout_is_balanced(srt, oracle):
+    A = _uneven(300, 128, 0.1, seed=4)
+    B = srt.workloads.dense_b(128, 96, seed=4)
+    plan, _ = _check(srt, oracle, A, B, 4, "orig", layout=0)
     plan.close()
 
 

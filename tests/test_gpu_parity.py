@@ -117,16 +117,17 @@ def test_pruned_ffn_cases(srt, oracle, torch_cuda, case, M_r, name):
                                                     (16, 16, 8, 1), (8, 4, 32, 0), (16, 2, 0, 1),
                                                     (2, 2, 0, 1)])
 @pytest.mark.parametrize("staging", [1, 2])
-def test_launch_variants_bitexact(srt, oracle, torch_cuda, V, U, waves, nsplit, chunk, xcd, staging):
+@pytest.mark.parametrize("layout", [1, 2])
+def test_launch_variants_bitexact(srt, oracle, torch_cuda, V, U, waves, nsplit, chunk, xcd, staging, layout):
     so, cb = oracle
     torch = torch_cuda
-    _log("variant V=%d U=%d waves=%d nsplit=%d chunk=%d xcd=%d staging=%d"
-         % (V, U, waves, nsplit, chunk, xcd, staging))
+    _log("variant V=%d U=%d waves=%d nsplit=%d chunk=%d xcd=%d staging=%d layout=%d"
+         % (V, U, waves, nsplit, chunk, xcd, staging, layout))
     A, B = srt.workloads.case_inputs(srt.workloads.SpMMCase(520, 300, 0.85, 264), seed=3)
     plan = _plan(srt, torch, A, 264, 8, "orig", vec_width=V, unroll=U, waves_per_block=waves,
-                 xcd_remap=xcd, nsplit=nsplit, chunk_cols=chunk, b_staging=staging)
+                 xcd_remap=xcd, nsplit=nsplit, chunk_cols=chunk, b_staging=staging, layout=layout)
     cfg = plan.get_config()
-    assert cfg.b_staging == staging
+    assert cfg.b_staging == staging and cfg.layout == layout
     assert cfg.unroll == U and cfg.nsplit <= cfg.waves_per_block
     assert cfg.N_c == 64 * cfg.vec_width and cfg.M_c == 8 * cfg.waves_per_block
     kp = plan.split_counts()
@@ -158,18 +159,19 @@ def test_lds_tile_limits(srt, oracle, torch_cuda, K, V, waves, nsplit):
 
 @pytest.mark.parametrize("nsplit,chunk", [(1, 0), (2, 64), (4, 32), (8, 16), (16, 8)])
 @pytest.mark.parametrize("M_r,name", [(8, "identity"), (4, "orig")])
-def test_split_chains_on_uneven_rows(srt, oracle, torch_cuda, nsplit, chunk, M_r, name):
+@pytest.mark.parametrize("layout", [1, 2])
+def test_split_chains_on_uneven_rows(srt, oracle, torch_cuda, nsplit, chunk, M_r, name, layout):
     """Heavy and empty panels (magnitude pruning with lognormal row scales),
     load-balanced splits, against the oracle's chunked chains."""
     so, cb = oracle
     torch = torch_cuda
-    _log("split", nsplit, chunk, M_r, name)
+    _log("split", nsplit, chunk, M_r, name, layout)
     A, B = srt.workloads.case_inputs(srt.workloads.SpMMCase(2048, 512, 0.9, 512), seed=0)
     plan = _plan(srt, torch, A, 512, M_r, name, nsplit=nsplit, waves_per_block=16, chunk_cols=chunk,
-                 vec_width=1)
+                 vec_width=1, layout=layout)
     kp = plan.split_counts()
     assert kp.max() <= nsplit
-    if nsplit > 1:
+    if nsplit > 1 and layout == 1:
         assert kp.max() > kp.min()          # heavy panels get more waves
     pk = cb.pack_dense(A, M_r, _oracle_table(so, M_r, name))
     C = _run(srt, torch, plan, B)
