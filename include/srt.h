@@ -91,7 +91,7 @@ typedef struct srt_tile_config {
                               workgroups (G x N tiles = whole rounds of the chip's
                               CUs at about chunk_cols columns per wave), each
                               workgroup's slots then given one by one to its
-                              panel with the longest chunk; 0 = choose            */
+                              panel with the longest chunk; 0 = choose (2)        */
 } srt_tile_config;
 
 /* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54,
