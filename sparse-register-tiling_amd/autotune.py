@@ -86,7 +86,8 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
     """Time every (mapping, shape) candidate; returns records sorted fastest
     first.  A shape is a dict of TileConfig fields.  The `refine` fastest
     are timed again with three times the replays (their order decides the
-    choice, so timing noise of the short first pass does not)."""
+    choice, so timing noise of the short first pass does not); they lead the
+    list ("refined": True), sorted among themselves, then the rest, sorted."""
     M, K = csr.shape
     if B is None:
         B = torch.rand((K, N), device="cuda") * 2 - 1
@@ -110,7 +111,10 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
             out.append(rec)
             plan.close()
     out.sort(key=lambda r: r["seconds"])
-    for rec in out[:refine]:
+    for i, rec in enumerate(out):
+        rec["refined"] = i < refine
+        if i >= refine:
+            continue
         shape = {k: v for k, v in rec.items() if k in TileConfig.__dataclass_fields__}
         plan = MatMulSpecialized.from_csr(csr, N, mapping_id=bases[(rec["M_r"], rec["mapping"])].mapping,
                                           tile_config=TileConfig(xcd_remap=1, **shape))

@@ -15,7 +15,11 @@ def test_table_covers_every_workload(srt):
     for case in srt.workloads.gpu_cases():
         recs = cache.get(case.M, case.K, case.sparsity, case.N)
         assert recs, case
-        assert recs == sorted(recs, key=lambda r: r["seconds"])
+        refined = [r for r in recs if r.get("refined")]
+        rest = recs[len(refined):]
+        assert recs[:len(refined)] == refined and not any(r.get("refined") for r in rest)
+        assert refined == sorted(refined, key=lambda r: r["seconds"])
+        assert rest == sorted(rest, key=lambda r: r["seconds"])
 
 
 @pytest.mark.parametrize("case", [(2048, 512, 0.9, 512), (512, 2048, 0.7, 128), (1024, 4096, 0.95, 256)])
@@ -34,3 +38,20 @@ def test_tuned_plan_exact(srt, oracle, case):
     pk = cb.pack_dense_ktiled(A, rec["M_r"], plan.mapping.mapping.table, real.K_c)
     _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=plan.split_counts()))
     plan.close()
+
+
+def test_tune_refines_the_fastest(srt):
+    """autotune.tune: the `refine` fastest lead the list, re-timed and sorted
+    among themselves; the rest follow sorted; every record names its layout."""
+    import torch
+    at = srt.autotune
+    A, _ = srt.workloads.case_inputs(srt.workloads.SpMMCase(512, 512, 0.9, 128), seed=1)
+    csr = torch.from_numpy(A).to_sparse_csr()
+    shapes = [dict(b_staging=2, vec_width=2, unroll=u, waves_per_block=w, nsplit=w, layout=lay)
+              for u in (2, 4) for w in (8, 16) for lay in (1, 2)]
+    recs = at.tune(csr, 128, mappings=[(4, "orig")], shapes=shapes, reps=5, refine=3)
+    assert len(recs) == len(shapes)
+    assert [r["refined"] for r in recs] == [True] * 3 + [False] * (len(shapes) - 3)
+    assert [r["seconds"] for r in recs[:3]] == sorted(r["seconds"] for r in recs[:3])
+    assert [r["seconds"] for r in recs[3:]] == sorted(r["seconds"] for r in recs[3:])
+    assert all(r["realised"]["layout"] == r["layout"] for r in recs)
