@@ -293,7 +293,8 @@ int srt_mfma_convert_b(int dtype, int64_t rows, int cols, const float *B, int64_
 /* C[:m, :n] = A . B with B given transposed as produced by srt_mfma_convert_b
  * (Bt: n x ldb in the plan's dtype, ldb >= k rounded up to 32, rows 16-byte
  * aligned; col_scales for FP8).  shape selects (N tile, waves, panels per wave): 0 (64,4,2)
- * 1 (128,4,2) 2 (64,4,1) 3 (128,2,2) 4 (32,4,1) 5 (128,4,1) 6 (64,8,1) 7 (128,8,1); the ksplit K
+ * 1 (128,4,2) 2 (64,4,1) 3 (128,2,2) 4 (32,4,1) 5 (128,4,1) 6 (64,8,1) 7 (128,8,1) 8 (64,16,1)
+ * 9 (32,8,1); the ksplit K
  * ranges are added in order. */
 int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
                           const float *col_scales, int n, int shape, int ksplit, srt_stream_t stream);

@@ -524,7 +524,7 @@ static hipError_t launch_exec_t(const MfmaLaunch &a, hipStream_t s) {
 
 // Launch shapes: (N tile, waves per workgroup, panels per wave).
 constexpr int kShapes[][3] = {{64, 4, 2}, {128, 4, 2}, {64, 4, 1}, {128, 2, 2}, {32, 4, 1}, {128, 4, 1},
-                              {64, 8, 1}, {128, 8, 1}};
+                              {64, 8, 1}, {128, 8, 1}, {64, 16, 1}, {32, 8, 1}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 template <bool FP8>
@@ -538,6 +538,8 @@ static hipError_t launch_exec_dt(const MfmaLaunch &a, hipStream_t s) {
         case 5: return launch_exec_t<FP8, 128, 4, 1>(a, s);
         case 6: return launch_exec_t<FP8, 64, 8, 1>(a, s);
         case 7: return launch_exec_t<FP8, 128, 8, 1>(a, s);
+        case 8: return launch_exec_t<FP8, 64, 16, 1>(a, s);
+        case 9: return launch_exec_t<FP8, 32, 8, 1>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -560,6 +562,8 @@ static hipError_t launch_grouped_dt(int shape, const MfmaProblem *probs, const i
         case 5: return launch_grouped_t<FP8, 128, 4, 1>(probs, hdr, grid, s);
         case 6: return launch_grouped_t<FP8, 64, 8, 1>(probs, hdr, grid, s);
         case 7: return launch_grouped_t<FP8, 128, 8, 1>(probs, hdr, grid, s);
+        case 8: return launch_grouped_t<FP8, 64, 16, 1>(probs, hdr, grid, s);
+        case 9: return launch_grouped_t<FP8, 32, 8, 1>(probs, hdr, grid, s);
         default: return hipErrorInvalidValue;
     }
 }

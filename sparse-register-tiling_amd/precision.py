@@ -28,7 +28,7 @@ from .native import ptr, stream_handle
 DTYPES = {"bf16": _lib.SRT_DT_BF16, "fp8": _lib.SRT_DT_FP8}
 # launch shapes: index -> (N tile, waves per workgroup, panels per wave)
 SHAPES = {0: (64, 4, 2), 1: (128, 4, 2), 2: (64, 4, 1), 3: (128, 2, 2), 4: (32, 4, 1), 5: (128, 4, 1),
-          6: (64, 8, 1), 7: (128, 8, 1)}
+          6: (64, 8, 1), 7: (128, 8, 1), 8: (64, 16, 1), 9: (32, 8, 1)}
 
 
 def _index_tensor(x, device) -> torch.Tensor:

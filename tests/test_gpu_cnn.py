@@ -140,7 +140,7 @@ def test_grouped_fp32_rejects_mixed_plans(env):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "fp8"])
-@pytest.mark.parametrize("shape", [0, 1, 4])
+@pytest.mark.parametrize("shape", [0, 1, 4, 6, 8])
 def test_grouped_panels_match_single(env, dtype, shape):
     srt, torch, layers = env
     batch = 1

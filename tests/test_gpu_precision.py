@@ -78,7 +78,7 @@ def test_small_shapes(srt, torch_cuda, dtype, shape_mkn):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "fp8"])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", list(range(10)))
 @pytest.mark.parametrize("ksplit", [1, 2, 5])
 def test_launch_shapes_and_splits(srt, torch_cuda, dtype, shape, ksplit):
     _log("precision shape", dtype, shape, ksplit)
