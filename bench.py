@@ -483,8 +483,17 @@ def bench_cnn(args):
         recs = cnn.tune_fp32(layers, batch, X, mappings=[(4, "orig"), (8, "orig")], reps=10)
         best = recs[0]
         shape = {k: best[k] for k in ("vec_width", "waves_per_block", "nsplit", "K_c", "layout")}
-        net = cnn.CnnFp32(layers, batch, X, mapping=(best["M_r"], best["mapping"]), shape=shape)
-        log("batch %d fp32 tuned in %.1f s: %s" % (batch, time.perf_counter() - t0, best))
+        mp = (best["M_r"], best["mapping"])
+        # per-layer split / layout / K tile inside the tuned group, kept if faster
+        layer_shapes, t_ref = cnn.refine_fp32_layers(layers, batch, X, mp, shape, reps=10)
+        uni = cnn.CnnFp32(layers, batch, X, mapping=mp, shape=shape)
+        t_uni = cnn.time_graph(uni.group, 10)
+        uni.close()
+        if t_ref >= t_uni:
+            layer_shapes = None
+        net = cnn.CnnFp32(layers, batch, X, mapping=mp, shape=shape, layer_shapes=layer_shapes)
+        log("batch %d fp32 tuned in %.1f s: %s; per-layer refinement %.2f us vs %.2f us uniform"
+            % (batch, time.perf_counter() - t0, best, t_ref * 1e6, t_uni * 1e6))
         set_bytes = sum(x.numel() * 4 for x in X) + sum(y.numel() * 4 for y in net.Y)
         groups = {}
 
@@ -514,7 +523,8 @@ def bench_cnn(args):
                      "hbm_frac": round(b32 / t_grp / 8e12, 4), "algorithmic_bytes": b32,
                      "rel_err_vs_fp64_max": float("%.3e" % err32), "operand_sets": nsets,
                      "tuned": {"mapping": "%d:%s" % (best["M_r"], best["mapping"]), **shape,
-                               "lane_width": best["vec_width"], "workgroups": best["workgroups"]}})
+                               "lane_width": best["vec_width"], "workgroups": best["workgroups"],
+                               "per_layer": layer_shapes}})
         log(rows[-1])
         net.close()
         # ---- BF16: autotune the MFMA tile shape, time with rotated operands

@@ -210,20 +210,39 @@ FP32_SHAPES = [dict(vec_width=v, waves_per_block=w, nsplit=s, K_c=kc, layout=lay
                for lay in (1, 2)]
 
 
+# Tile-config fields one layer of a group may set for itself (the grouped
+# launch shares the executor, the lane width and the workgroup size).
+LAYER_FIELDS = ("nsplit", "layout", "K_c", "chunk_cols")
+
+
 class CnnFp32:
     """The FP32 network form: one plan per layer (shared mapping and
-    waves_per_block) and one grouped launch over all of them."""
+    waves_per_block) and one grouped launch over all of them.  `layer_shapes`
+    (one dict per layer, keys from LAYER_FIELDS) overrides `shape` per layer."""
 
     def __init__(self, layers: Sequence[CnnLayer], batch: int, X: Sequence[torch.Tensor],
                  mapping: Tuple[int, str] = (4, "orig"), shape: Optional[Dict] = None,
-                 out: Optional[Sequence[torch.Tensor]] = None):
+                 out: Optional[Sequence[torch.Tensor]] = None, layer_shapes: Optional[Sequence[Dict]] = None):
         self.layers, self.batch = list(layers), batch
         shape = dict(FP32_SHAPES[0] if shape is None else shape)
         self.mapping, self.shape = mapping, shape
+        if layer_shapes is not None:
+            if len(layer_shapes) != len(self.layers):
+                raise ValueError("one layer shape per layer")
+            for ls in layer_shapes:
+                bad = set(ls) - set(LAYER_FIELDS)
+                if bad:
+                    raise ValueError("a layer may set only %s, not %s" % (LAYER_FIELDS, sorted(bad)))
+        self.layer_shapes = [dict(ls) for ls in layer_shapes] if layer_shapes is not None else None
         nm = resolve_mapping(named_mapping(*mapping))
-        cfg = TileConfig(xcd_remap=1, b_staging=2, unroll=4, **shape)
-        self.plans = [MatMulSpecialized.from_csr(l.A, layer_n(l, batch), mapping_id=nm, tile_config=cfg)
-                      for l in self.layers]
+
+        def cfg(i):
+            sh = dict(shape)
+            if self.layer_shapes is not None:
+                sh.update(self.layer_shapes[i])
+            return TileConfig(xcd_remap=1, b_staging=2, unroll=4, **sh)
+        self.plans = [MatMulSpecialized.from_csr(l.A, layer_n(l, batch), mapping_id=nm, tile_config=cfg(i))
+                      for i, l in enumerate(self.layers)]
         self.X = list(X)
         self.Y = list(out) if out is not None else [
             torch.empty((l.M, layer_n(l, batch)), device="cuda") for l in self.layers]
@@ -283,6 +302,41 @@ def tune_fp32(layers: Sequence[CnnLayer], batch: int, X: Sequence[torch.Tensor],
             net.close()
     out.sort(key=lambda r: r["seconds"])
     return out
+
+
+def layer_candidates(waves_per_block: int) -> List[Dict]:
+    """Per-layer settings refine_fp32_layers tries."""
+    splits = sorted({1, 4, 8, waves_per_block} & set(range(1, waves_per_block + 1)))
+    return [dict(nsplit=s, layout=lay, K_c=kc) for s in splits for lay in (1, 2) for kc in (0, 256)]
+
+
+def refine_fp32_layers(layers: Sequence[CnnLayer], batch: int, X: Sequence[torch.Tensor],
+                       mapping: Tuple[int, str], shape: Dict, reps: int = 10,
+                       candidates: Optional[Sequence[Dict]] = None) -> Tuple[List[Dict], float]:
+    """Per-layer split / layout / K tile for a tuned group shape: each layer
+    takes the candidate its own launch runs fastest with (same mapping, lane
+    width and workgroup size as the group); returns (layer_shapes, seconds of
+    the grouped launch with them)."""
+    cands = list(candidates) if candidates is not None else layer_candidates(shape["waves_per_block"])
+    nm = resolve_mapping(named_mapping(*mapping))
+    chosen = []
+    for l, x in zip(layers, X):
+        y = torch.empty((l.M, layer_n(l, batch)), device="cuda")
+        best = None
+        for c in cands:
+            sh = dict(shape)
+            sh.update(c)
+            plan = MatMulSpecialized.from_csr(l.A, layer_n(l, batch), mapping_id=nm,
+                                              tile_config=TileConfig(xcd_remap=1, b_staging=2, unroll=4, **sh))
+            t = time_graph(lambda: plan(y, x), reps)
+            plan.close()
+            if best is None or t < best[0]:
+                best = (t, c)
+        chosen.append(dict(best[1]))
+    net = CnnFp32(layers, batch, X, mapping=mapping, shape=shape, layer_shapes=chosen)
+    t = time_graph(net.group, reps)
+    net.close()
+    return chosen, t
 
 
 def tune_panels(layers: Sequence[CnnLayer], batch: int, X: Sequence[torch.Tensor], dtype: str = "bf16",

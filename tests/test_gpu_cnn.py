@@ -74,6 +74,42 @@ def test_grouped_fp32_matches_oracle(env, oracle, batch, mapping, shape):
     net.close()
 
 
+@pytest.mark.parametrize("batch", [1, 4])
+def test_grouped_fp32_per_layer_shapes(env, oracle, batch):
+    """Layers of one group with their own split, layout and K tile: still
+    bit-identical to the oracle's chain over each layer's own packing and to
+    one launch per layer."""
+    srt, torch, layers = env
+    so, cb = oracle
+    _log("cnn fp32 per-layer shapes batch=%d" % batch)
+    X = srt.cnn.activations(layers, batch, seed=6)
+    cands = srt.cnn.layer_candidates(8)
+    layer_shapes = [cands[(3 * i) % len(cands)] for i in range(len(layers))]
+    net = srt.cnn.CnnFp32(layers, batch, X, mapping=(4, "orig"),
+                          shape=dict(vec_width=4, waves_per_block=8, nsplit=8), layer_shapes=layer_shapes)
+    for y in net.Y:
+        y.fill_(float("nan"))
+    net()
+    torch.cuda.synchronize()
+    table = _oracle_table(so, 4, "orig")
+    for l, p, x, y, ls in zip(layers, net.plans, X, net.Y, layer_shapes):
+        real = p.get_config()
+        assert real.layout == ls["layout"] and real.nsplit == ls["nsplit"]
+        A = l.A.to_dense().numpy()
+        B = x.cpu().numpy()
+        pk = cb.pack_dense_ktiled(A, 4, table, real.K_c) if real.K_c < A.shape[1] else cb.pack_dense(A, 4, table)
+        ref = cb.exec_packed(pk, B, nsplit=p.split_counts())
+        assert np.array_equal(_bits(y.cpu().numpy()), _bits(ref)), l.name
+    got = [y.clone() for y in net.Y]
+    net.per_layer()
+    torch.cuda.synchronize()
+    for a, b in zip(got, net.Y):
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+    net.close()
+    with pytest.raises(ValueError):
+        srt.cnn.CnnFp32(layers[:2], batch, X[:2], layer_shapes=[dict(vec_width=2), dict(nsplit=1)])
+
+
 def test_grouped_fp32_graph_replay_and_rebind(env):
     srt, torch, layers = env
     _log("cnn fp32 graph + rebind")
