@@ -51,7 +51,7 @@ METRIC = "SpMM GFLOP/s (2·nnz·N/s) + %HBM roofline; served p99 latency vs plai
 HBM_PEAK_GBS = 8000.0
 MALL_BYTES = 256 << 20
 COPY_BYTES = 256 << 20
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "executor_north_star_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "executor_2048x512_s90_n512_traffic.txt")
 EXEC_KERNEL = "exec_kernel"
 # realised tile-config fields that name a measured configuration
 GPU_FIELDS = ("K_c", "vec_width", "unroll", "waves_per_block", "nsplit", "chunk_cols", "b_staging", "layout")
@@ -610,7 +610,7 @@ def main():
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")
     wl = srt.workloads
-    case = wl.NORTH_STAR
+    case = wl.PRUNED_2048_512_S90_N512
     A, B = wl.case_inputs(case, seed=0)
     nnz = int(np.count_nonzero(A))
     M, K, N = case.M, case.K, case.N

@@ -32,7 +32,7 @@ GPU_SHAPES = [(512, 512), (2048, 512), (512, 2048), (1024, 4096)]
 GPU_SPARSITIES = [0.70, 0.80, 0.90, 0.95]
 GPU_NS = [128, 256, 512]
 # The headline case: 2048 x 512 at 90% sparsity times 512 x 512.
-NORTH_STAR = SpMMCase(2048, 512, 0.90, 512)
+PRUNED_2048_512_S90_N512 = SpMMCase(2048, 512, 0.90, 512)
 CPU_CASE = SpMMCase(2048, 512, 0.80, 256)
 
 

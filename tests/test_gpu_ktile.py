@@ -43,7 +43,7 @@ def test_reference_default_k_tile_on_the_2048x512_workload(srt, oracle):
     """K_c = 256, the reference's SOPConfig default, on 2048 x 512 (90 %) x 512."""
     import torch
     so, cb = oracle
-    A, B = srt.workloads.case_inputs(srt.workloads.NORTH_STAR, seed=0)
+    A, B = srt.workloads.case_inputs(srt.workloads.PRUNED_2048_512_S90_N512, seed=0)
     plan = _plan(srt, torch, A, 512, 4, "orig", K_c=256)
     assert plan.stats()["k_tiles"] == 2
     pk = cb.pack_dense_ktiled(A, 4, _oracle_table(so, 4, "orig"), 256)

@@ -89,7 +89,7 @@ def test_balanced_layout_mappings_and_k_tiles(srt, oracle, M_r, name, k_tile):
 
 def test_balanced_layout_2048x512_workload(srt, oracle):
     """2048 x 512 (90 %) x 512 with the balanced layout on 16-wave workgroups."""
-    A, B = srt.workloads.case_inputs(srt.workloads.NORTH_STAR, seed=0)
+    A, B = srt.workloads.case_inputs(srt.workloads.PRUNED_2048_512_S90_N512, seed=0)
     plan, kp = _check(srt, oracle, A, B, 4, "orig", K_c=256, waves_per_block=16, nsplit=16, vec_width=4,
                       unroll=2, chunk_cols=24)
     assert kp.max() > 1
