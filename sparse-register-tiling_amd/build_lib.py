@@ -111,7 +111,28 @@ def build(incremental: bool = False, jobs: int = 0, verbose: bool = True) -> str
     os.replace(LIB + ".tmp", LIB)
     if verbose:
         print("built %s (%d executors, %.1fs)" % (LIB, len(gen_files), time.time() - t0), flush=True)
+    build_testbed_demo(verbose)
     return LIB
+
+
+CPP_DIR = os.path.join(PKG_DIR, "cpp")
+DEMO = os.path.join(CPP_DIR, "bin", "testbed_demo")
+
+
+def build_testbed_demo(verbose: bool = True) -> str:
+    """The C++ testbed-style caller of the C ABI (cpp/testbed_demo.cpp),
+    linked against libsrt.so."""
+    os.makedirs(os.path.dirname(DEMO), exist_ok=True)
+    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=" + ARCH, "-Wall", "-I" + INCLUDE, "-I" + CPP_DIR,
+           os.path.join(CPP_DIR, "testbed_demo.cpp"), "-o", DEMO + ".tmp", "-L" + PKG_DIR, "-lsrt",
+           "-Wl,-rpath," + PKG_DIR]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("testbed demo build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(DEMO + ".tmp", DEMO)
+    if verbose:
+        print("built %s" % DEMO, flush=True)
+    return DEMO
 
 
 if __name__ == "__main__":
