@@ -142,3 +142,38 @@ def test_invalid_inputs(srt, torch_cuda):
         srt.precision.PanelPlan(2, 4, torch.ones(2), torch.tensor([0, 2, 1]), torch.tensor([0, 1]), "bf16")
     with pytest.raises(ValueError):
         srt.precision.PanelPlan(2, 4, torch.ones(1), torch.tensor([0, 1, 1]), torch.tensor([0]), "fp16")
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("shape,ksplit", [(6, 4), (0, 2), (4, 5), (8, 3)])
+def test_k_split_sum_is_deterministic(srt, torch_cuda, dtype, shape, ksplit):
+    """The K splits' partial tiles are added in split order: repeated launches
+    (and a graph replaying them) give the same bits."""
+    torch = torch_cuda
+    _log("precision split determinism", dtype, shape, ksplit)
+    A = _sparse(1000, 1300, 0.3, 21)
+    B = np.random.default_rng(22).uniform(-1, 1, (1300, 200)).astype(np.float32)
+    plan = srt.precision.PanelPlan.from_dense(torch.from_numpy(A), dtype)
+    Bq, t = plan.prepare_b(torch.from_numpy(B).cuda())
+    plan.reserve(200, ksplit)
+    C = torch.full((1000, 200), float("nan"), device="cuda")
+    plan(C, Bq, t, shape=shape, ksplit=ksplit)
+    torch.cuda.synchronize()
+    first = C.clone()
+    _within(first.cpu().numpy(), A, B, dtype)
+    for _ in range(5):
+        C.fill_(float("nan"))
+        plan(C, Bq, t, shape=shape, ksplit=ksplit)
+    torch.cuda.synchronize()
+    assert torch.equal(C.view(torch.int32), first.view(torch.int32))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan(C, Bq, t, shape=shape, ksplit=ksplit)
+    for _ in range(3):
+        C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(C.view(torch.int32), first.view(torch.int32))
