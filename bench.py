@@ -432,6 +432,7 @@ def bench_sweep(args):
         r = {"M": case.M, "K": case.K, "sparsity": case.sparsity, "N": case.N, "nnz": nnz,
              "mapping": "%d:%s" % best[1], "vec_width": best[2].vec_width, "K_c": real.K_c,
              "waves": real.waves_per_block, "nsplit": real.nsplit, "unroll": real.unroll, "layout": real.layout,
+             "b_staging": real.b_staging,
              "exec_us": round(best[0] * 1e6, 2), "exec_gflops": round(flop / best[0] / 1e9, 1),
              "exec_hbm_frac": round(bytes_alg / best[0] / 8e12, 4),
              "csr_us": round(t_csr * 1e6, 2), "csr_gflops": round(flop / t_csr / 1e9, 1),
