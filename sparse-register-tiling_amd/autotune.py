@@ -189,3 +189,17 @@ def tuned_plan(csr: torch.Tensor, N: int, path: str = SHIPPED) -> MatMulSpeciali
         return MatMulSpecialized.from_csr(csr, N, mapping_id=named_mapping(4, "orig"))
     (M_r, name), cfg = best_config(recs)
     return MatMulSpecialized.from_csr(csr, N, mapping_id=named_mapping(M_r, name), tile_config=cfg)
+
+
+def main(argv=None) -> None:
+    """Re-measure the tuning table over the 48 pruned-FFN workloads on the
+    current GPU (a few minutes on an MI355X):
+    python -c "import importlib; importlib.import_module('sparse-register-tiling_amd.autotune').main()" --out t.json"""
+    import argparse
+    from . import workloads as wl
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--out", required=True, help="JSON table to write (the shipped one is %s)" % SHIPPED)
+    ap.add_argument("--reps", type=int, default=20, help="launches per timing (the shortlist gets 3x)")
+    a = ap.parse_args(argv)
+    tune_workloads(wl.gpu_cases(), a.out, reps=a.reps,
+                   log=lambda c, r: print(c, "%.2f us" % (r["seconds"] * 1e6), flush=True))
