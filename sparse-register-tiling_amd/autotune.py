@@ -37,7 +37,7 @@ TABLE_SHAPES = [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nspl
                 for kc in (0, 256) for lay in (1, 2)] + \
                [dict(b_staging=1, vec_width=1, unroll=u, waves_per_block=w, nsplit=s, K_c=0, layout=lay)
                 for u in (2, 4) for (w, s) in ((16, 16), (16, 8), (8, 8)) for lay in (1, 2)]
-TABLE_MAPPINGS = [(4, "orig"), (8, "orig")]
+TABLE_MAPPINGS = [(4, "orig"), (8, "orig"), (4, "identity"), (2, "identity")]
 BUCKETS = [(0.0, 0.7), (0.7, 0.8), (0.8, 0.9), (0.9, 1.0)]
 
 
