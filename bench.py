@@ -481,20 +481,26 @@ def bench_cnn(args):
         fl = cnn.flops(layers, batch)
         # ---- FP32: autotune, then time the best group with rotated operands
         t0 = time.perf_counter()
-        recs = cnn.tune_fp32(layers, batch, X, mappings=[(4, "orig"), (8, "orig")], reps=10)
-        best = recs[0]
-        shape = {k: best[k] for k in ("vec_width", "waves_per_block", "nsplit", "K_c", "layout")}
-        mp = (best["M_r"], best["mapping"])
-        # per-layer split / layout / K tile inside the tuned group, kept if faster
-        layer_shapes, t_ref = cnn.refine_fp32_layers(layers, batch, X, mp, shape, reps=10)
-        uni = cnn.CnnFp32(layers, batch, X, mapping=mp, shape=shape)
-        t_uni = cnn.time_graph(uni.group, 10)
-        uni.close()
-        if t_ref >= t_uni:
-            layer_shapes = None
+        recs = cnn.tune_fp32(layers, batch, X, mappings=[(4, "orig"), (8, "orig"), (4, "identity"), (2, "identity")],
+                             reps=10)
+        # the fastest uniform group of each pattern set, each also with a
+        # per-layer split / layout / K tile refinement; the fastest is kept
+        trials = []
+        per_map = {}
+        for r in recs:
+            per_map.setdefault((r["M_r"], r["mapping"]), r)
+        for cand in per_map.values():
+            sh = {k: cand[k] for k in ("vec_width", "waves_per_block", "nsplit", "K_c", "layout")}
+            m = (cand["M_r"], cand["mapping"])
+            uni = cnn.CnnFp32(layers, batch, X, mapping=m, shape=sh)
+            trials.append((cnn.time_graph(uni.group, 20), cand, sh, m, None))
+            uni.close()
+            ls, t_ref = cnn.refine_fp32_layers(layers, batch, X, m, sh, reps=10)
+            trials.append((t_ref, cand, sh, m, ls))
+        t_sel, best, shape, mp, layer_shapes = min(trials, key=lambda x: x[0])
         net = cnn.CnnFp32(layers, batch, X, mapping=mp, shape=shape, layer_shapes=layer_shapes)
-        log("batch %d fp32 tuned in %.1f s: %s; per-layer refinement %.2f us vs %.2f us uniform"
-            % (batch, time.perf_counter() - t0, best, t_ref * 1e6, t_uni * 1e6))
+        log("batch %d fp32 tuned in %.1f s: %s; per-layer %s, %.2f us"
+            % (batch, time.perf_counter() - t0, best, "refined" if layer_shapes else "uniform", t_sel * 1e6))
         set_bytes = sum(x.numel() * 4 for x in X) + sum(y.numel() * 4 for y in net.Y)
         groups = {}
 
