@@ -78,9 +78,10 @@ typedef struct srt_tile_config {
     int nsplit;            /* most waves one panel's packed columns are split over,
                               1..waves_per_block (partial sums are added in chunk
                               order; 1 = the reference's single chain); 0 = choose */
-    int chunk_cols;        /* target packed columns per wave: panel p runs on
-                              kp = min(nsplit, max(1, ceil(ncols_p / chunk_cols)))
-                              waves (load balance); 0 = choose                      */
+    int chunk_cols;        /* target packed columns per wave: with layout 1 panel p
+                              runs on kp = min(nsplit, max(1, ceil(ncols_p / chunk_cols)))
+                              waves; with layout 2 it sizes the workgroup count (see
+                              layout); 0 = choose                                   */
     int b_staging;         /* B operand path: 1 = copy the workgroup's B tile (K rows x
                               N_c columns) into LDS once and read packed columns
                               from there; 2 = read B rows from global memory per
