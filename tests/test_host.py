@@ -153,7 +153,12 @@ def test_default_tile_config(srt):
     t = srt.TileConfig.from_c(c)
     # the reference's SOPConfig defaults, except K_c: 0 = one K tile (256 would tile K like the reference)
     assert (t.M_c, t.K_c, t.N_c, t.beta, t.max_tlb_entries, t.tlb_page_size) == (16, 0, 64, 1.0, 64, 4096)
+    # GPU fields default to "choose" (the balanced workgroup layout, among others)
+    assert (t.vec_width, t.waves_per_block, t.unroll, t.nsplit, t.chunk_cols, t.b_staging, t.layout) == (0,) * 7
     assert srt.TileConfig.from_c(t.to_c()) == t
+    # the ctypes mirror has every field of srt_tile_config, in order
+    assert [f[0] for f in srt._lib.TileConfigC._fields_][-3:] == ["chunk_cols", "b_staging", "layout"]
+    assert ct.sizeof(srt._lib.TileConfigC) == 17 * 4
 
 
 def test_group_abi_rejects_bad_arguments(srt):
