@@ -90,6 +90,12 @@ def parse():
     ap.add_argument("--replicas", type=int, default=0, help="0 = enough to exceed 2x the 256 MB cache")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="plain: leave out the compact served-vs-plain comparison")
+    ap.add_argument("--serving-qps", default="20000,40000,80000,160000",
+                    help="plain: offered loads of the compact served-vs-plain comparison")
+    ap.add_argument("--serving-requests", type=int, default=600,
+                    help="plain: requests per load level of the compact served-vs-plain comparison")
     ap.add_argument("--tune-reps", type=int, default=40)
     ap.add_argument("--tune", action="store_true",
                     help="autotune even when the shipped table has this workload")
@@ -185,13 +191,46 @@ def committed_traffic(config):
 # --------------------------------------------------------------------------
 # CPU baseline
 # --------------------------------------------------------------------------
+def host_description():
+    """CPU model, sockets, cores and AVX-512 presence of the host (from
+    /proc/cpuinfo; lscpu's socket count when it is available)."""
+    info = {"cpu_model": None, "sockets": None, "logical_cpus": os.cpu_count(), "avx512f": False}
+    physical = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and info["cpu_model"] is None:
+                    info["cpu_model"] = val
+                elif key == "physical id":
+                    physical.add(val)
+                elif key == "flags":
+                    flags = val.split()
+                    info["avx512f"] = "avx512f" in flags
+                    info["avx512_flags"] = sorted(x for x in set(flags) if x.startswith("avx512"))
+    except OSError:
+        pass
+    if physical:
+        info["sockets"] = len(physical)
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Socket(s):"):
+                info["sockets"] = int(line.split(":")[1])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def cpu_baselines(A, B, mapping, seconds):
     """Time the oracle's C executor (all host threads and 1 thread) and MKL CSR."""
     import numpy as np
     import torch
     from oracle import cbind
     nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    pk = cbind.pack_dense(A, mapping.M_r, mapping.table)
+    pk = cbind.pack_dense(A, mapping.M_r, mapping.table, kernel_list=mapping.kernel_list)
     nnz = int(np.count_nonzero(A))
     flop = 2.0 * nnz * B.shape[1]
 
@@ -225,6 +264,7 @@ def cpu_baselines(A, B, mapping, seconds):
         "mkl_csr_gflops": round(flop / t_mkl / 1e9, 2),
         "mkl_csr": "torch CPU sparse_csr @ dense (MKL=%s), %d threads, %d runs"
                    % (torch.backends.mkl.is_available(), nthreads, runs_mkl),
+        "host": host_description(),
     }
 
 
@@ -327,12 +367,11 @@ def bench_precision(args):
                       "unit": "GFLOP/s", "n_gpus": 1, "data": "synthetic", "results": rows}), flush=True)
 
 
-def bench_serving(args):
+def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample=16):
     """48 resident BF16 weights (12 x {1024x1024 x2, 4096x1024, 1024x4096},
     90 % magnitude-pruned), Poisson requests with log-uniform N in [1, 512]:
     the continuous-batching server against one plain FP32 launch per request,
-    same arrivals, at several offered loads."""
-    import numpy as np
+    same arrivals, at the given offered loads and at saturation."""
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")
     sv = srt.serving
@@ -346,10 +385,10 @@ def bench_serving(args):
         plans32.append(srt.MatMulSpecialized.from_csr(csr, 512, mapping_id=srt.named_mapping(4, "orig"),
                                                       tile_config=srt.TileConfig(xcd_remap=1)))
     log("packed 48 weights in %.1f s" % (time.perf_counter() - t0))
-    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, shape=args.mfma_shape, grid=1024)
+    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, shape=mfma_shape, grid=1024)
     levels = []
-    for qps in [float(q) for q in args.qps.split(",")]:
-        reqs = sv.poisson_requests(args.requests, qps, shapes, seed=int(qps) % 1000 + 1)
+    for qps in qps_levels:
+        reqs = sv.poisson_requests(requests, qps, shapes, seed=int(qps) % 1000 + 1)
         st = server.serve(reqs)
         plain_reqs = [sv.Request(rid=r.rid, weight=r.weight, B=r.B, arrival=r.arrival) for r in reqs]
         Bs32 = [r.B.float() for r in plain_reqs]
@@ -359,28 +398,44 @@ def bench_serving(args):
             % (qps, st["p50_ms"], st["p99_ms"], st["req_per_s"], pst["p50_ms"], pst["p99_ms"], pst["req_per_s"]))
         del reqs, plain_reqs, Bs32
     # saturation: everything arrives at once
-    reqs = sv.poisson_requests(args.requests, 1e9, shapes, seed=99)
+    reqs = sv.poisson_requests(requests, 1e9, shapes, seed=99)
     sat = server.serve(reqs)
     plain_reqs = [sv.Request(rid=r.rid, weight=r.weight, B=r.B, arrival=r.arrival) for r in reqs]
     psat = sv.serve_plain(plans32, plain_reqs, [r.B.float() for r in plain_reqs])
-    ok_s = [l["offered_qps"] for l in levels if l["served"]["p99_ms"] <= args.slo_ms]
-    ok_p = [l["offered_qps"] for l in levels if l["plain_fp32"]["p99_ms"] <= args.slo_ms]
+    ok_s = [l["offered_qps"] for l in levels if l["served"]["p99_ms"] <= slo_ms]
+    ok_p = [l["offered_qps"] for l in levels if l["plain_fp32"]["p99_ms"] <= slo_ms]
     ratio = (max(ok_s) / max(ok_p)) if ok_s and ok_p else None
     # accuracy of the served form on a sample
     from oracle import precision_oracle as po
-    sample = sv.poisson_requests(16, 1e9, shapes, seed=7)
+    sample = sv.poisson_requests(accuracy_sample, 1e9, shapes, seed=7)
     for r in sample:
         server.run_batch([r])
     errs = [po.rel_error_vs_fp64(r.C.cpu().numpy(), Ws[r.weight], r.B.float().cpu().numpy()) for r in sample]
-    out = {"form": "serving", "metric": "served p99 latency vs plain; requests/s at p99 <= %.1f ms" % args.slo_ms,
-           "unit": "req/s", "n_gpus": 1, "data": "synthetic", "dtype": "bf16 (served) / f32 (plain)",
-           "weights": "48 x {1024x1024 x2, 4096x1024, 1024x4096} per 12 layers, 90% magnitude-pruned",
-           "requests_per_level": args.requests, "slo_ms": args.slo_ms,
+    # resident weight bytes: stored 16x32 blocks (BF16) against the nonzero values
+    resident = sum(p.stats()["stored_blocks"] * 512 * 2 for p in panels)
+    nnz = sum(int((W != 0).sum()) for W in Ws)
+    out = {"weights": "48 x {1024x1024 x2, 4096x1024, 1024x4096} per 12 layers, 90% magnitude-pruned",
+           "requests_per_level": requests, "slo_ms": slo_ms,
            "max_qps_within_slo": {"served": max(ok_s) if ok_s else None, "plain_fp32": max(ok_p) if ok_p else None},
            "throughput_ratio_at_equal_p99": ratio,
            "saturation_req_per_s": {"served": sat["req_per_s"], "plain_fp32": psat["req_per_s"]},
            "saturation_ratio": sat["req_per_s"] / psat["req_per_s"],
-           "served_rel_err_vs_fp64_max": max(errs), "levels": levels}
+           "p99_ms_at_slo_load": None, "served_rel_err_vs_fp64_max": max(errs),
+           "weight_nnz": nnz, "weight_nnz_bytes_bf16": 2 * nnz, "resident_weight_bytes": resident,
+           "levels": levels}
+    if ok_s and ok_p:
+        at = min(max(ok_s), max(ok_p))
+        lv = [l for l in levels if l["offered_qps"] == at][0]
+        out["p99_ms_at_slo_load"] = {"offered_qps": at, "served": lv["served"]["p99_ms"],
+                                     "plain_fp32": lv["plain_fp32"]["p99_ms"]}
+    return out
+
+
+def bench_serving(args):
+    """The serving form's own JSON line (see serving_comparison)."""
+    res = serving_comparison([float(q) for q in args.qps.split(",")], args.requests, args.slo_ms, args.mfma_shape)
+    out = {"form": "serving", "metric": "served p99 latency vs plain; requests/s at p99 <= %.1f ms" % args.slo_ms,
+           "unit": "req/s", "n_gpus": 1, "data": "synthetic", "dtype": "bf16 (served) / f32 (plain)", **res}
     print(json.dumps(out), flush=True)
 
 
@@ -418,7 +473,8 @@ def bench_sweep(args):
         Ccsr = torch.empty_like(C)
         t_csr = at.time_graph(lambda: srt.csr.spmm(csr_t, Bd, Ccsr, _parts_cache=parts), reps=50)
         torch.cuda.synchronize()
-        pk = cbind.pack_dense_ktiled(A, best[1][0], plan.mapping.mapping.table, real.K_c)
+        pk = cbind.pack_dense_ktiled(A, best[1][0], plan.mapping.mapping.table, real.K_c,
+                                     kernel_list=plan.mapping.mapping.kernel_list)
         ref = cbind.exec_packed(pk, B, nsplit=plan.split_counts())
         exact = bool(np.array_equal(C.cpu().numpy().view(np.uint32), ref.view(np.uint32)))
         csr_err = float(np.abs(Ccsr.cpu().numpy() - ref).max())
@@ -699,10 +755,17 @@ def main():
     e.record(stream)
     torch.cuda.synchronize()
     warm_us = s.elapsed_time(e) * 1e3 / 200
-    # results of the graph replays equal the eager result
-    for j in (0, R - 1):
+    # results of the graph replays equal the eager result: replicas the graphs
+    # reached hold their replayed result; the rest are launched once here, so
+    # every replica is checked
+    launched = min(R, args.warmup + args.steps)
+    for j in range(launched, R):
+        plans[j](Cs[j], Bs[j])
+    torch.cuda.synchronize()
+    for j in range(R):
         if not torch.equal(Cs[j], C0):
-            raise SystemExit("graph replay result differs from the eager launch (replica %d)" % j)
+            raise SystemExit("replica %d differs from the eager launch (%s)"
+                             % (j, "graph replay" if j < launched else "eager launch"))
 
     # vendor-library references on one resident operand set (warm): the dense
     # FP32 product (torch.matmul -> hipBLASLt/rocBLAS, graph-timed) and the
@@ -761,6 +824,20 @@ def main():
         out["autotune_top"] = tune_top
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baselines(A, B, mapping.mapping, args.cpu_seconds)
+    if not args.no_serving:
+        del plans, Bs, Cs, g_steps, g_warm, g_hot
+        torch.cuda.empty_cache()
+        sv = serving_comparison([float(q) for q in args.serving_qps.split(",")], args.serving_requests,
+                                args.slo_ms, args.mfma_shape, accuracy_sample=8)
+        out["serving"] = {k: sv[k] for k in ("weights", "requests_per_level", "slo_ms", "max_qps_within_slo",
+                                             "throughput_ratio_at_equal_p99", "p99_ms_at_slo_load",
+                                             "saturation_req_per_s", "saturation_ratio",
+                                             "served_rel_err_vs_fp64_max", "weight_nnz_bytes_bf16",
+                                             "resident_weight_bytes")}
+        out["serving"]["p50_p99_ms"] = [{"offered_qps": l["offered_qps"],
+                                         "served": [l["served"]["p50_ms"], l["served"]["p99_ms"]],
+                                         "plain_fp32": [l["plain_fp32"]["p50_ms"], l["plain_fp32"]["p99_ms"]]}
+                                        for l in sv["levels"]]
     print(json.dumps(out), flush=True)
 
 

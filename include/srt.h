@@ -16,8 +16,13 @@
  *   - Work is enqueued on the caller's stream (srt_stream_t is hipStream_t;
  *     NULL = the default stream).  Calls that must read sizes back (plan
  *     creation, pack_tile) synchronise that stream before returning.
- *   - Plans and packed tiles are immutable after creation; concurrent
- *     executes of one plan on different streams are safe.
+ *   - A plan's packed data is immutable after creation, and concurrent
+ *     executes of one plan on different streams are safe.  The launch
+ *     schedule may change (srt_plan_allocate_executor, srt_tile_execute with
+ *     another configuration), but a schedule once built stays alive until
+ *     the plan / tile is destroyed, so a graph captured before the change
+ *     replays unchanged.  Calls that change the schedule must not run
+ *     concurrently with other calls on the same plan.
  *   - CSR inputs: row pointer and column index arrays of index_bytes = 4
  *     (int32) or 8 (int64) bytes each, zero-based, no duplicate entries.
  */
@@ -104,6 +109,19 @@ void srt_tile_config_default(srt_tile_config *cfg);
  * ---------------------------------------------------------------------- */
 typedef struct srt_mapping srt_mapping;
 
+/* Kernel ids.  A mapping's kernel patterns get ids by the reference
+ * generator's rule: the kernel list it is handed (sbench/SOP/sop_driver.py:93)
+ * stable-sorted by popcount, id = position in the sorted list
+ * (tools/codegen/gen_vector_specialized.py:113,130-137).  The ids fix the
+ * packed column order and so the accumulation order of every output.
+ *   - srt_mapping_create_ordered takes the kernel list, e.g. a strategy's
+ *     all_patterns(): merge then split patterns (sbench/SOP/
+ *     sop_hand_configs.py:16-17, passed at sop_bench.py:105,154).
+ *   - srt_mapping_read and srt_mapping_create build it as the reference's
+ *     load_mapping does for a file (sop_bench.py:72-95): every pattern added
+ *     to a Python set in file (code) order, then list(set) -- CPython 3.10's
+ *     iteration order for small ints, restated in C. */
+
 /* Read mapping_<mapping_id>.txt from the first directory that has it.
  * Replaces sop::read_pattern_mapping(mapping_id, dirs)
  * (cpp_testbed/demo/count_nkerns.cpp:54-55; file format sbench/SOP/mapping.py:14-17). */
@@ -116,11 +134,19 @@ int srt_mapping_read(const char *mapping_id, const char *const *search_dirs, int
 int srt_mapping_create(int M_r, const int32_t *code_ptr, const int32_t *code_patterns,
                        srt_mapping **out);
 
+/* As srt_mapping_create, with the kernel list the ids derive from
+ * (kernel_list[num_kernels]; repeated patterns keep their first position;
+ * listed patterns the table never uses still get ids, as in the reference's
+ * generator).  Fails if the table uses a pattern the list lacks.
+ * Replaces make_sop_module(acc, patterns, mapping) (sbench/SOP/sop_driver.py:88-129). */
+int srt_mapping_create_ordered(int M_r, const int32_t *code_ptr, const int32_t *code_patterns,
+                               const int32_t *kernel_list, int num_kernels, srt_mapping **out);
+
 /* M_r and the number of kernel patterns (MicroKernel::num_nkern_patterns, template:42). */
 int srt_mapping_info(const srt_mapping *m, int *M_r, int *num_patterns);
 
-/* id -> kernel pattern bitmask, num_patterns entries (decode_nkern_pattern, template:192).
- * Ids follow (popcount, bitmask) order. */
+/* id -> kernel pattern bitmask, num_patterns entries (decode_nkern_pattern, template:192),
+ * in id order (see "Kernel ids" above). */
 int srt_mapping_pattern_table(const srt_mapping *m, int32_t *patterns_out);
 
 void srt_mapping_destroy(srt_mapping *m);
@@ -172,6 +198,18 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
  * (KD_IntelLoadBalanced, nano/runtime_yaml_config.register:85-87) have no
  * published partition; this one is stated so callers can reproduce the order. */
 int srt_plan_split_counts(const srt_plan *p, int32_t *kp);
+
+/* Workgroups and threads per workgroup of one srt_plan_execute at width n
+ * (aligned operands; the lane width may drop for misaligned ones).  No
+ * counterpart in the reference: the launch geometry the measurement harness
+ * gives its launch-floor probe. */
+int srt_plan_launch_shape(const srt_plan *p, int n, int *grid, int *block);
+
+/* Launch-floor probe with the executor's geometry (measurement only): kind 0
+ * an empty kernel; kind 1 every wave walking `depth` dependent scalar loads
+ * through buf (buf_bytes >= 128, zero-filled; sink: one writable int). */
+int srt_probe_launch_floor(int kind, int grid, int block, const void *buf, int64_t buf_bytes, int depth,
+                           void *sink, srt_stream_t stream);
 
 typedef struct srt_plan_stats {
     int M, K, M_r, num_panels, num_patterns;
@@ -242,6 +280,20 @@ int srt_tile_pack_panel(srt_tile *t, int64_t panel_id, int64_t col_offset, const
 
 /* SOPTile::pack_tile (template:219-261): every panel must be packed. */
 int srt_tile_pack_tile(srt_tile *t, srt_stream_t stream);
+
+/* The packed tile in the reference's buffer layout (SOPTile::pack_tile,
+ * template:219-261), built by srt_tile_pack_tile: one device allocation of
+ * required_bytes = sum over panels of 4*num_nnz + 4*num_col_indices +
+ * 4*num_nkern + 3*64 (template:222-229), holding per panel the values, the
+ * column indices and the nkern_counts, each segment starting on a 64-byte
+ * boundary.  used_bytes = end of the last segment (<= required_bytes).
+ * panel_offsets (3 * num_panels entries, may be NULL) receives the byte
+ * offsets of each panel's three segments.  The buffer belongs to the tile. */
+int srt_tile_buffer(const srt_tile *t, const void **buffer, size_t *required_bytes, size_t *used_bytes,
+                    int64_t *panel_offsets);
+
+/* Copy the first `bytes` (<= required_bytes) of that buffer to host memory. */
+int srt_tile_copy_buffer(const srt_tile *t, void *host_out, size_t bytes);
 
 /* SOPTile::padding (template:125-127). */
 int srt_tile_padding(const srt_tile *t, int64_t *padding);

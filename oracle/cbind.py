@@ -76,7 +76,7 @@ def build_lut(M_r: int, table: Dict[int, List[int]], patterns: List[int]):
     return ptr, np.array(ids if ids else [0], dtype=np.int32)
 
 
-def pack_dense_ktiled(A: np.ndarray, M_r: int, table: Dict[int, List[int]], k_tile: int):
+def pack_dense_ktiled(A: np.ndarray, M_r: int, table: Dict[int, List[int]], k_tile: int, kernel_list=None):
     """K-tiled packing (count_nkerns.cpp:52-126: A cut into K_c-column tiles,
     each panel packed per tile): every K slice is packed on its own with its
     column offset and the slices' packed columns are laid one after another
@@ -84,8 +84,9 @@ def pack_dense_ktiled(A: np.ndarray, M_r: int, table: Dict[int, List[int]], k_ti
     A = np.ascontiguousarray(A, dtype=np.float32)
     M, K = A.shape
     if not (0 < k_tile < K):
-        return pack_dense(A, M_r, table)
-    slices = [pack_dense(A[:, k0:min(K, k0 + k_tile)], M_r, table, col_offset=k0) for k0 in range(0, K, k_tile)]
+        return pack_dense(A, M_r, table, kernel_list=kernel_list)
+    slices = [pack_dense(A[:, k0:min(K, k0 + k_tile)], M_r, table, col_offset=k0, kernel_list=kernel_list)
+              for k0 in range(0, K, k_tile)]
     P, NK = slices[0]["P"], slices[0]["NK"]
     cols, vals = [], []
     col_ptr = np.zeros(P + 1, dtype=np.int64)
@@ -107,11 +108,13 @@ def pack_dense_ktiled(A: np.ndarray, M_r: int, table: Dict[int, List[int]], k_ti
     }
 
 
-def pack_dense(A: np.ndarray, M_r: int, table: Dict[int, List[int]], col_offset: int = 0):
-    """Pack a dense row-major fp32 matrix with the mapping ``table``."""
+def pack_dense(A: np.ndarray, M_r: int, table: Dict[int, List[int]], col_offset: int = 0, kernel_list=None):
+    """Pack a dense row-major fp32 matrix with the mapping ``table``; kernel
+    ids from ``kernel_list`` (default: the table's own, see
+    ``sop_oracle.kernel_pattern_table``)."""
     A = np.ascontiguousarray(A, dtype=np.float32)
     M, K = A.shape
-    patterns = so.kernel_pattern_table(table)
+    patterns = so.kernel_pattern_table(table, kernel_list)
     NK = len(patterns)
     lut_ptr, lut_ids = build_lut(M_r, table, patterns)
     id_pat = np.array(patterns, dtype=np.int32)

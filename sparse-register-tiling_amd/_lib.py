@@ -76,6 +76,7 @@ SIGNATURES = {
     "srt_tile_config_default": (None, [ct.POINTER(TileConfigC)]),
     "srt_mapping_read": (_i, [ct.c_char_p, ct.POINTER(ct.c_char_p), _i, ct.POINTER(_vp)]),
     "srt_mapping_create": (_i, [_i, _p_i32, _p_i32, ct.POINTER(_vp)]),
+    "srt_mapping_create_ordered": (_i, [_i, _p_i32, _p_i32, _p_i32, _i, ct.POINTER(_vp)]),
     "srt_mapping_info": (_i, [_vp, ct.POINTER(_i), ct.POINTER(_i)]),
     "srt_mapping_pattern_table": (_i, [_vp, _p_i32]),
     "srt_mapping_destroy": (None, [_vp]),
@@ -96,12 +97,16 @@ SIGNATURES = {
     "srt_group_execute": (_i, [_vp, _vp]),
     "srt_group_info": (_i, [_vp, _vp, _vp, _vp]),
     "srt_group_destroy": (None, [_vp]),
+    "srt_plan_launch_shape": (_i, [_vp, _i, ct.POINTER(_i), ct.POINTER(_i)]),
+    "srt_probe_launch_floor": (_i, [_i, _i, _i, _vp, _i64, _i, _vp, _vp]),
     "srt_plan_split_counts": (_i, [_vp, _vp]),
     "srt_tile_split_counts": (_i, [_vp, _vp]),
     "srt_sort_and_encode_patterns": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "srt_tile_create": (_i, [_vp, _i64, _i64, _i64, ct.POINTER(_vp)]),
     "srt_tile_pack_panel": (_i, [_vp, _i64, _i64, _vp, _vp, _i, _vp, _i, _vp]),
     "srt_tile_pack_tile": (_i, [_vp, _vp]),
+    "srt_tile_buffer": (_i, [_vp, ct.POINTER(_vp), ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t), _p_i64]),
+    "srt_tile_copy_buffer": (_i, [_vp, _vp, ct.c_size_t]),
     "srt_tile_padding": (_i, [_vp, _p_i64]),
     "srt_tile_execute": (_i, [_vp, _i, _vp, _i64, _vp, _i64, _i, ct.POINTER(TileConfigC), _vp]),
     "srt_tile_copy_packed": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _p_i64, _p_i64]),

@@ -39,7 +39,7 @@ CFLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I" + INCLUDE, "-I" + CSRC,
 ]
-HAND_SOURCES = ["capi.hip", "packer.hip", "csr_spmm.hip", "mfma_panel.hip", "mfma_capi.hip"]
+HAND_SOURCES = ["capi.hip", "packer.hip", "csr_spmm.hip", "mfma_panel.hip", "mfma_capi.hip", "probe.hip"]
 
 
 def _load_sibling(name):
@@ -51,10 +51,15 @@ def _load_sibling(name):
 
 
 def pattern_sets() -> List[Tuple[int, List[int]]]:
-    """Pattern sets that get a prebuilt executor: every shipped mapping."""
+    """Id-ordered pattern lists that get a prebuilt executor: every shipped
+    strategy (its own kernel list) and every shipped mapping file read as the
+    reference's load_mapping reads it (the file's set)."""
     mapping = _load_sibling("mapping")
     mapping.generate_shipped_mappings()
     sets = {}
+    for name in mapping.STRATEGIES:
+        m = mapping.strategy_mapping(name)
+        sets[(m.M_r, tuple(m.patterns))] = True
     for f in sorted(os.listdir(mapping.MAPPINGS_DIR)):
         if f.startswith("mapping_") and f.endswith(".txt"):
             m = mapping.read_mapping_file(os.path.join(mapping.MAPPINGS_DIR, f))

@@ -4,7 +4,9 @@ The reference generates one AVX-512 micro-kernel header per pattern set
 (``UKernelCodegenBase(...).gen_header``, used at sbench/SOP/sop_driver.py:93-107;
 legacy generator tools/codegen/gen_vector_specialized.py:112-349).  Here the
 unrolled per-pattern body is the ``srt::exec_kernel`` template
-(csrc/exec_kernel.h); the generator emits, for a pattern set in id order,
+(csrc/exec_kernel.h); the generator emits, for a pattern list in id order
+(the mapping's kernel list stable-sorted by popcount, as the legacy generator
+sorts, :113; the same set in another order is another executor),
 
 * the pattern table,
 * the kernel instantiations for both B paths (rows read from global memory,
@@ -27,13 +29,21 @@ def popcount(x: int) -> int:
     return bin(int(x)).count("1")
 
 
-def canonical_patterns(patterns: Iterable[int]) -> List[int]:
-    """Kernel pattern set in id order: (popcount, bitmask)."""
-    return sorted({int(p) for p in patterns if int(p)}, key=lambda p: (popcount(p), p))
+def id_ordered_patterns(patterns: Iterable[int]) -> List[int]:
+    """The kernel patterns of an executor in id order, as given (a mapping's
+    ``patterns``: its kernel list stable-sorted by popcount).  The order is
+    part of the executor: it fixes the packed column order and so the
+    accumulation order of every output."""
+    out = [int(p) for p in patterns]
+    if any(p <= 0 for p in out) or len(set(out)) != len(out):
+        raise ValueError("kernel patterns must be distinct and nonzero")
+    if any(popcount(a) > popcount(b) for a, b in zip(out, out[1:])):
+        raise ValueError("kernel patterns are not in id order (stable popcount order)")
+    return out
 
 
 def nanokernel_hash(M_r: int, patterns: Sequence[int]) -> str:
-    key = "%d:%s" % (M_r, ",".join(str(p) for p in canonical_patterns(patterns)))
+    key = "%d:%s" % (M_r, ",".join(str(p) for p in id_ordered_patterns(patterns)))
     return hashlib.md5(key.encode()).hexdigest()[:10]
 
 
@@ -99,7 +109,9 @@ def gen_executor_sources(M_r: int, patterns: Sequence[int], lean: bool = False) 
     variants of a large pattern set compile in parallel.  `lean` keeps only
     the global-memory B path at U = GROUPED_UNROLL (every lane width): the
     default launch, for executors built at run time."""
-    pats = canonical_patterns(patterns)
+    if not patterns:
+        raise ValueError("empty pattern set")
+    pats = id_ordered_patterns(patterns)
     if not pats:
         raise ValueError("empty pattern set")
     if max(pats) >= (1 << M_r):

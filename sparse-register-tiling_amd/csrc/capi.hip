@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -77,9 +78,91 @@ size_t Packed::bytes() const {
 }
 
 // ------------------------------------------------------------------ mapping
+
+// Iteration order of a CPython set of small non-negative ints (hash(i) == i)
+// built by adding `keys` in order.  The reference's load_mapping collects a
+// mapping file's kernel patterns into a set (sbench/SOP/sop_bench.py:72-95)
+// and hands list(set) to its generator (sop_driver.py:93), so this order
+// fixes the kernel ids of a mapping read from a file.  Restated from the
+// published algorithm of CPython 3.10's set (open addressing over a power-of-
+// two table of at least 8 slots; 9 linear probes after the home slot
+// hash & mask, then i = 5i + 1 + perturb with perturb >>= 5; the table grows
+// to the smallest power of two above 4 * used once fill * 5 >= mask * 3,
+// re-inserting in slot order).  Checked against the interpreter's own sets.
+static std::vector<int32_t> python_set_order(const std::vector<int32_t> &keys) {
+    constexpr size_t kProbes = 9, kShift = 5, kMinSize = 8;
+    std::vector<int64_t> table(kMinSize, -1);  // -1: empty slot
+    size_t mask = kMinSize - 1, used = 0;
+    auto insert_clean = [](std::vector<int64_t> &t, size_t m, int64_t key) {
+        size_t perturb = (size_t)key, i = (size_t)key & m;
+        while (true) {
+            if (t[i] < 0) {
+                t[i] = key;
+                return;
+            }
+            if (i + kProbes <= m)
+                for (size_t j = 1; j <= kProbes; ++j)
+                    if (t[i + j] < 0) {
+                        t[i + j] = key;
+                        return;
+                    }
+            perturb >>= kShift;
+            i = (i * 5 + 1 + perturb) & m;
+        }
+    };
+    for (int32_t k : keys) {
+        const int64_t key = k;
+        size_t i = (size_t)key & mask, perturb = (size_t)key;
+        size_t slot = SIZE_MAX;
+        bool present = false;
+        if (table[i] < 0) {
+            slot = i;
+        } else {
+            while (true) {
+                if (table[i] == key) {
+                    present = true;
+                    break;
+                }
+                if (i + kProbes <= mask) {
+                    for (size_t j = 1; j <= kProbes && slot == SIZE_MAX && !present; ++j) {
+                        if (table[i + j] < 0) slot = i + j;
+                        else if (table[i + j] == key) present = true;
+                    }
+                    if (present || slot != SIZE_MAX) break;
+                }
+                perturb >>= kShift;
+                i = (i * 5 + 1 + perturb) & mask;
+                if (table[i] < 0) {
+                    slot = i;
+                    break;
+                }
+            }
+        }
+        if (present) continue;
+        table[slot] = key;
+        ++used;  // no deletions: fill == used
+        if (used * 5 < mask * 3) continue;
+        const size_t minused = used > 50000 ? used * 2 : used * 4;
+        size_t size = kMinSize;
+        while (size <= minused) size <<= 1;
+        std::vector<int64_t> grown(size, -1);
+        for (int64_t e : table)
+            if (e >= 0) insert_clean(grown, size - 1, e);
+        table.swap(grown);
+        mask = size - 1;
+    }
+    std::vector<int32_t> out;
+    for (int64_t e : table)
+        if (e >= 0) out.push_back((int32_t)e);
+    return out;
+}
+
 struct MappingImpl {
     int M_r = 0;
     std::vector<std::vector<int32_t>> table;  // code -> kernel patterns (list order)
+    std::vector<int32_t> code_order;          // codes in file / array order
+    bool has_kernel_list = false;             // kernel list given (else: the file's set)
+    std::vector<int32_t> kernel_list;         // the list fed to the generator
     std::vector<int32_t> patterns;            // id -> pattern
     std::vector<int32_t> pattern_id;          // pattern -> id or -1
     std::vector<int32_t> lut_ptr, lut_ids;
@@ -101,9 +184,20 @@ struct MappingImpl {
     }
 
     // Validates the table and derives the id order and the code -> ids LUT.
+    // Ids: the kernel list stable-sorted by popcount
+    // (tools/codegen/gen_vector_specialized.py:113,130-137); the list is the
+    // one given, or, for a table read from a file, list(set) of its patterns
+    // in file order (python_set_order).
     int finalize() {
         const int ncode = 1 << M_r;
         if ((int)table.size() != ncode) table.resize(ncode);
+        if (code_order.empty())
+            for (int c = 1; c < ncode; ++c) code_order.push_back(c);
+        std::vector<int32_t> file_list;
+        for (int c : code_order)
+            if (c > 0 && c < ncode)
+                for (int32_t p : table[c])
+                    if (p > 0) file_list.push_back(p);
         std::vector<char> seen(ncode, 0);
         for (int c = 1; c < ncode; ++c) {
             unsigned uni = 0;
@@ -129,11 +223,22 @@ struct MappingImpl {
             table[c] = kept;
         }
         patterns.clear();
+        std::vector<char> listed(ncode, 0);
+        for (int32_t p : has_kernel_list ? kernel_list : python_set_order(file_list)) {
+            if (p == 0) continue;
+            if (p < 0 || p >= ncode)
+                return fail(SRT_ERR_INVALID, "kernel list pattern " + std::to_string(p) +
+                                                 " does not fit M_r=" + std::to_string(M_r));
+            if (listed[p]) continue;  // a repeated pattern keeps its first position
+            listed[p] = 1;
+            patterns.push_back(p);
+        }
         for (int p = 1; p < ncode; ++p)
-            if (seen[p]) patterns.push_back(p);
+            if (seen[p] && !listed[p])
+                return fail(SRT_ERR_INVALID, "the mapping uses kernel pattern " + std::to_string(p) +
+                                                 ", which the kernel list lacks");
         std::stable_sort(patterns.begin(), patterns.end(), [](int32_t a, int32_t b) {
-            const int pa = __builtin_popcount((unsigned)a), pb = __builtin_popcount((unsigned)b);
-            return pa != pb ? pa < pb : a < b;
+            return __builtin_popcount((unsigned)a) < __builtin_popcount((unsigned)b);
         });
         if (patterns.size() > 4096)
             return fail(SRT_ERR_UNSUPPORTED, "more than 4096 kernel patterns in one mapping");
@@ -217,6 +322,7 @@ static int parse_mapping(std::istream &in, MappingImpl &m, const std::string &wh
             }
         }
         m.table[code] = pats;
+        m.code_order.push_back((int)code);
     }
     return m.finalize();
 }
@@ -239,6 +345,7 @@ struct srt_plan {
     int n_max = 0;
     int n_alloc = 0;
     int device = 0;
+    std::vector<RetiredSchedule> retired;  // schedules of earlier widths, alive until destroy
 };
 
 struct PanelBuf {
@@ -263,6 +370,13 @@ struct srt_tile {
     mutable std::mutex mu;
     mutable Schedule sch;
     mutable srt_tile_config sch_cfg{};
+    mutable std::vector<RetiredSchedule> retired;  // earlier schedules, alive until destroy
+    // The reference's packed buffer (template:219-261): per panel
+    // [values][pad64][col_indices][pad64][nkern_counts][pad64] in one
+    // allocation of required_bytes; byte offsets of the three segments per panel.
+    void *ref_buffer = nullptr;
+    size_t ref_required = 0, ref_used = 0;
+    std::vector<int64_t> ref_offsets;
 };
 
 extern "C" {
@@ -316,10 +430,20 @@ int srt_mapping_read(const char *mapping_id, const char *const *dirs, int num_di
 
 int srt_mapping_create(int M_r, const int32_t *code_ptr, const int32_t *code_patterns,
                        srt_mapping **out) {
+    return srt_mapping_create_ordered(M_r, code_ptr, code_patterns, nullptr, -1, out);
+}
+
+int srt_mapping_create_ordered(int M_r, const int32_t *code_ptr, const int32_t *code_patterns,
+                               const int32_t *kernel_list, int num_kernels, srt_mapping **out) {
     if (!code_ptr || !out) return fail(SRT_ERR_INVALID, "srt_mapping_create: null argument");
     if (M_r < 1 || M_r > SRT_MAX_MR) return fail(SRT_ERR_INVALID, "M_r must be in [1, 16]");
+    if (num_kernels > 0 && !kernel_list) return fail(SRT_ERR_INVALID, "null kernel list");
     auto impl = std::make_shared<MappingImpl>();
     impl->M_r = M_r;
+    if (num_kernels >= 0) {
+        impl->has_kernel_list = true;
+        impl->kernel_list.assign(kernel_list, kernel_list + num_kernels);
+    }
     const int ncode = 1 << M_r;
     impl->table.assign(ncode, {});
     for (int c = 0; c < ncode; ++c) {
@@ -691,8 +815,12 @@ static void balanced_slots(const Packed &pk, Schedule &sch, int n_tiles, std::ve
     }
 }
 
+// Builds the launch schedule of `cfg` into `out`.  The schedule `out` held
+// before moves to `retired` and stays alive: a graph captured with it keeps
+// replaying from valid memory until the owner is destroyed.
 static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const ExecutorDesc *exec, int n_tiles,
-                          Schedule &out, hipStream_t s) {
+                          Schedule &out, std::vector<RetiredSchedule> &retired, const srt_tile_config &out_cfg,
+                          hipStream_t s) {
     Schedule sch;
     sch.unroll = schedule_unroll(exec, pow2_upto8(cfg.unroll) ? cfg.unroll : 4);
     sch.waves_per_block = cfg.waves_per_block;
@@ -773,8 +901,37 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
                                      d_start, sch.desc, sch.prog, s));
     }
     SRT_HIP_TRY(hipStreamSynchronize(s));
-    out.release();
+    if (out.desc || out.prog) retired.push_back({out_cfg, std::move(out)});
     out = std::move(sch);
+    return SRT_OK;
+}
+
+// Whether a schedule built for config b (n_tiles N tiles) serves config a at
+// a_tiles N tiles (only the balanced layout depends on the tile count).
+static bool same_schedule(const srt_tile_config &a, int a_tiles, const srt_tile_config &b, const Schedule &sch) {
+    return a.nsplit == b.nsplit && a.chunk_cols == b.chunk_cols && a.waves_per_block == b.waves_per_block &&
+           a.unroll == b.unroll && a.layout == b.layout && (a.layout != 2 || a_tiles == sch.n_tiles);
+}
+
+// Makes `cur` the schedule of config c: a retired schedule of the same
+// shape comes back (the current one retires in its place), else a new one
+// is built.  Every schedule stays alive until its owner is destroyed.
+static int switch_schedule(const Packed &pk, const srt_tile_config &c, int tiles, const ExecutorDesc *exec,
+                           Schedule &cur, srt_tile_config &cur_cfg, std::vector<RetiredSchedule> &retired,
+                           hipStream_t s) {
+    if (cur.desc && same_schedule(c, tiles, cur_cfg, cur)) return SRT_OK;
+    for (RetiredSchedule &r : retired)
+        if (same_schedule(c, tiles, r.cfg, r.sch)) {
+            std::swap(r.sch, cur);
+            std::swap(r.cfg, cur_cfg);
+            if (!r.sch.desc && !r.sch.prog) {  // nothing was current: drop the empty entry
+                r = std::move(retired.back());
+                retired.pop_back();
+            }
+            return SRT_OK;
+        }
+    SRT_TRY(build_schedule(pk, c, exec, tiles, cur, retired, cur_cfg, s));
+    cur_cfg = c;
     return SRT_OK;
 }
 
@@ -794,8 +951,9 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, int K_c, const f
                     const void *Li, int index_bytes, hipStream_t s) {
     DeviceLut *lut = nullptr;
     SRT_TRY(map.device_lut(&lut));
-    // The row pointer is walked by the scatter kernel: check it on the host
+    // The row pointer is walked by the key kernel: check it on the host
     // first (non-negative start, non-decreasing).
+    int64_t rp_first = 0, rp_last = 0;
     {
         std::vector<int64_t> rp(m + 1);
         if (index_bytes == 8) {
@@ -811,6 +969,8 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, int K_c, const f
         for (int i = 0; i < m; ++i)
             if (rp[i + 1] < rp[i]) return fail(SRT_ERR_INVALID, "CSR row pointer decreases at row " + std::to_string(i));
         if (rp[m] > rp[0] && (!Lx || !Li)) return fail(SRT_ERR_INVALID, "null CSR value or index array");
+        rp_first = rp[0];
+        rp_last = rp[m];
     }
     const int M_r = map.M_r;
     const int NK = (int)map.patterns.size();
@@ -829,20 +989,20 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, int K_c, const f
     pk.NK = NK;
     pk.KT = KT;
     pk.K_c = k > 0 ? Kc : 0;
+    // Memory proportional to the nonzeros: every CSR entry becomes a key
+    // (its cell = panel * k + column), the keys are radix-sorted stably (a
+    // cell's entries stay in row order), and the packer walks the cells.
+    const int64_t nnz = rp_last - rp_first;
+    if (nnz > (int64_t)0x7fffffffLL) return fail(SRT_ERR_UNSUPPORTED, "more than 2^31 - 1 nonzeros");
     Scratch sc;
-    uint32_t *codes = nullptr;
-    float *dense = nullptr;
     int *bad = nullptr;
     int64_t *ncols = nullptr, *nvals = nullptr, *pad = nullptr, *seg_col = nullptr, *seg_val = nullptr;
-    const size_t pk_cells = (size_t)P * (size_t)ld;
-    SRT_TRY(sc.get(&codes, pk_cells));
-    SRT_TRY(sc.get(&dense, pk_cells * M_r));
+    int64_t *seg_cells = nullptr;
     SRT_TRY(sc.get(&bad, 1));
     SRT_TRY(sc.get(&ncols, Q));
     SRT_TRY(sc.get(&nvals, Q));
     SRT_TRY(sc.get(&pad, Q));
-    SRT_HIP_TRY(hipMemsetAsync(codes, 0, sizeof(uint32_t) * pk_cells, s));
-    SRT_HIP_TRY(hipMemsetAsync(dense, 0, sizeof(float) * pk_cells * M_r, s));
+    SRT_TRY(sc.get(&seg_cells, (size_t)Q + 1));
     SRT_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), s));
     SRT_TRY(dmalloc(&pk.counts, (size_t)Q * NK));
     SRT_TRY(dmalloc(&pk.col_ptr, (size_t)P + 1));
@@ -854,36 +1014,67 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, int K_c, const f
         seg_col = pk.col_ptr;
         seg_val = pk.val_ptr;
     }
-    SRT_HIP_TRY(launch_scatter_csr(m, k, ld, M_r, Lx, Lp, Li, index_bytes, codes, dense, bad, s));
-    SRT_HIP_TRY(launch_count_codes(Q, Kc, *lut, codes, pk.counts, ncols, nvals, s));
-    SRT_HIP_TRY(launch_exclusive_scan(Q, ncols, seg_col, s));
-    SRT_HIP_TRY(launch_exclusive_scan(Q, nvals, seg_val, s));
+    const size_t n = (size_t)std::max<int64_t>(nnz, 1);
+    uint64_t *keys = nullptr, *keys_alt = nullptr, *cell_key = nullptr;
+    uint32_t *idx = nullptr, *idx_alt = nullptr, *head = nullptr, *cell_of = nullptr, *cell_code = nullptr;
+    uint8_t *ent_row = nullptr;
+    int64_t *cell_first = nullptr;
+    SRT_TRY(sc.get(&keys, n));
+    SRT_TRY(sc.get(&keys_alt, n));
+    SRT_TRY(sc.get(&idx, n));
+    SRT_TRY(sc.get(&idx_alt, n));
+    SRT_TRY(sc.get(&ent_row, n));
+    SRT_TRY(sc.get(&head, n));
+    SRT_TRY(sc.get(&cell_of, n));
+    SRT_TRY(sc.get(&cell_key, n));
+    SRT_TRY(sc.get(&cell_code, n));
+    SRT_TRY(sc.get(&cell_first, n + 1));
+    SRT_HIP_TRY(launch_cell_keys(m, k, M_r, Lp, Li, index_bytes, keys, idx, ent_row, bad, s));
     int h_bad = 0;
-    int64_t tot_c = 0, tot_v = 0;
     SRT_HIP_TRY(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-    SRT_HIP_TRY(hipMemcpyAsync(&tot_c, seg_col + Q, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    SRT_HIP_TRY(hipMemcpyAsync(&tot_v, seg_val + Q, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    // nnz from the row pointer
-    int64_t lp0 = 0, lpm = 0;
-    if (index_bytes == 8) {
-        SRT_HIP_TRY(hipMemcpyAsync(&lp0, Lp, 8, hipMemcpyDeviceToHost, s));
-        SRT_HIP_TRY(hipMemcpyAsync(&lpm, (const int64_t *)Lp + m, 8, hipMemcpyDeviceToHost, s));
-    } else {
-        int32_t a = 0, b = 0;
-        SRT_HIP_TRY(hipMemcpyAsync(&a, Lp, 4, hipMemcpyDeviceToHost, s));
-        SRT_HIP_TRY(hipMemcpyAsync(&b, (const int32_t *)Lp + m, 4, hipMemcpyDeviceToHost, s));
-        SRT_HIP_TRY(hipStreamSynchronize(s));
-        lp0 = a;
-        lpm = b;
-    }
     SRT_HIP_TRY(hipStreamSynchronize(s));
     if (h_bad) return fail(SRT_ERR_INVALID, "CSR column index out of range [0, k)");
-    pk.nnz = lpm - lp0;
+    int64_t ncells = 0;
+    if (nnz > 0) {
+        int end_bit = 1;  // bits of the largest key, P * k - 1
+        while (end_bit < 64 && ((uint64_t)P * (uint64_t)k - 1) >> end_bit) ++end_bit;
+        size_t sort_bytes = 0, scan_bytes = 0;
+        bool in_alt = false;
+        SRT_HIP_TRY(sort_cell_keys(nnz, end_bit, keys, keys_alt, idx, idx_alt, nullptr, &sort_bytes, &in_alt, s));
+        SRT_HIP_TRY(scan_cell_heads(nnz, head, cell_of, nullptr, &scan_bytes, s));
+        char *temp = nullptr;
+        SRT_TRY(sc.get(&temp, std::max<size_t>(std::max(sort_bytes, scan_bytes), 1)));
+        SRT_HIP_TRY(sort_cell_keys(nnz, end_bit, keys, keys_alt, idx, idx_alt, temp, &sort_bytes, &in_alt, s));
+        if (in_alt) {
+            std::swap(keys, keys_alt);
+            std::swap(idx, idx_alt);
+        }
+        SRT_HIP_TRY(launch_cell_heads(nnz, keys, idx, ent_row, head, bad, s));
+        SRT_HIP_TRY(scan_cell_heads(nnz, head, cell_of, temp, &scan_bytes, s));
+        SRT_HIP_TRY(launch_cell_fill(nnz, keys, idx, ent_row, head, cell_of, cell_key, cell_code, cell_first, s));
+        uint32_t last_of = 0, last_head = 0;
+        SRT_HIP_TRY(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipMemcpyAsync(&last_of, cell_of + nnz - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipMemcpyAsync(&last_head, head + nnz - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipStreamSynchronize(s));
+        if (h_bad) return fail(SRT_ERR_INVALID, "CSR holds an entry twice (same row and column)");
+        ncells = (int64_t)last_of + last_head;
+    }
+    SRT_HIP_TRY(launch_segment_bounds(Q, KT, k, Kc, ncells, cell_key, seg_cells, s));
+    SRT_HIP_TRY(launch_count_cells(Q, *lut, seg_cells, cell_code, pk.counts, ncols, nvals, s));
+    SRT_HIP_TRY(launch_exclusive_scan(Q, ncols, seg_col, s));
+    SRT_HIP_TRY(launch_exclusive_scan(Q, nvals, seg_val, s));
+    int64_t tot_c = 0, tot_v = 0;
+    SRT_HIP_TRY(hipMemcpyAsync(&tot_c, seg_col + Q, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipMemcpyAsync(&tot_v, seg_val + Q, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    pk.nnz = nnz;
     pk.total_cols = tot_c;
     pk.total_vals = tot_v;
     SRT_TRY(alloc_packed_arrays(pk, s));
-    SRT_HIP_TRY(launch_fill_codes(Q, Kc, 0, KT, *lut, codes, dense, seg_col, seg_val, pk.counts,
-                                  pk.cols, pk.vals, pad, s));
+    SRT_HIP_TRY(launch_fill_cells(Q, k, KT, *lut, seg_cells, cell_key, cell_code, cell_first, idx, ent_row,
+                                  Lx ? Lx + rp_first : nullptr, seg_col, seg_val, pk.counts, pk.cols, pk.vals,
+                                  pad, s));
     if (KT > 1) {  // panel offsets = every KT-th segment offset
         SRT_HIP_TRY(hipMemcpy2DAsync(pk.col_ptr, sizeof(int64_t), seg_col, sizeof(int64_t) * KT,
                                      sizeof(int64_t), (size_t)P + 1, hipMemcpyDeviceToDevice, s));
@@ -937,20 +1128,13 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
     plan->cfg = plan->cfg_user;
     realise_config(plan->cfg, map.M_r, k, plan->pk.K_c, plan->pk.total_cols, std::max(n, 1));
     rc = build_schedule(plan->pk, plan->cfg, plan->exec, config_tiles(plan->cfg, std::max(n, 1)), plan->sch,
-                        (hipStream_t)stream);
+                        plan->retired, plan->cfg, (hipStream_t)stream);
     if (rc) {
         plan->pk.release();
         return rc;
     }
     *out = plan.release();
     return SRT_OK;
-}
-
-// Whether a schedule built for config b (n_tiles N tiles) serves config a at
-// a_tiles N tiles (only the balanced layout depends on the tile count).
-static bool same_schedule(const srt_tile_config &a, int a_tiles, const srt_tile_config &b, const Schedule &sch) {
-    return a.nsplit == b.nsplit && a.chunk_cols == b.chunk_cols && a.waves_per_block == b.waves_per_block &&
-           a.unroll == b.unroll && a.layout == b.layout && (a.layout != 2 || a_tiles == sch.n_tiles);
 }
 
 int srt_plan_get_config(const srt_plan *p, srt_tile_config *out) {
@@ -970,12 +1154,14 @@ int srt_plan_allocate_executor(srt_plan *p, int n) {
     if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
     p->n_alloc = n;
     if (n > p->n_max) p->n_max = n;
-    // Re-realise the choices that depend on the B width and rebuild the
-    // schedule if they changed.  Not to be called while the plan executes.
+    // Re-realise the choices that depend on the B width and switch to that
+    // schedule (schedules of other widths stay alive: graphs captured with
+    // them replay unchanged).  Not concurrently with other calls on the plan.
     srt_tile_config c = p->cfg_user;
     realise_config(c, p->pk.M_r, p->pk.K, p->pk.K_c, p->pk.total_cols, std::max(n, 1));
     const int tiles = config_tiles(c, std::max(n, 1));
-    if (!same_schedule(c, tiles, p->cfg, p->sch)) SRT_TRY(build_schedule(p->pk, c, p->exec, tiles, p->sch, nullptr));
+    srt_tile_config sch_cfg = p->cfg;
+    SRT_TRY(switch_schedule(p->pk, c, tiles, p->exec, p->sch, sch_cfg, p->retired, nullptr));
     p->cfg = c;
     return SRT_OK;
 }
@@ -987,6 +1173,15 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
     SRT_HIP_TRY(hipGetDevice(&dev));
     if (dev != p->device) return fail(SRT_ERR_INVALID, "plan belongs to another device");
     return run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream);
+}
+
+int srt_plan_launch_shape(const srt_plan *p, int n, int *grid, int *block) {
+    if (!p || n < 0) return fail(SRT_ERR_INVALID, "null plan or negative n");
+    const int V = std::max(1, p->cfg.vec_width);
+    const int64_t tiles = ((int64_t)n + 64 * V - 1) / (64 * V);
+    if (grid) *grid = (int)std::min<int64_t>((int64_t)p->sch.n_wblocks * tiles, 0x7fffffff);
+    if (block) *block = 64 * p->sch.waves_per_block;
+    return SRT_OK;
 }
 
 int srt_plan_split_counts(const srt_plan *p, int32_t *kp) {
@@ -1034,6 +1229,7 @@ void srt_plan_destroy(srt_plan *p) {
     if (!p) return;
     p->pk.release();
     p->sch.release();
+    for (RetiredSchedule &r : p->retired) r.sch.release();
     delete p;
 }
 
@@ -1351,6 +1547,40 @@ int srt_tile_pack_tile(srt_tile *t, srt_stream_t stream) {
             SRT_HIP_TRY(hipMemcpyAsync(pk.vals + vp[p], pb.vals, sizeof(float) * pb.nvals,
                                        hipMemcpyDeviceToDevice, s));
     }
+    // The reference's single buffer: required bytes
+    // sum(4 nnz + 4 ncols + 4 num_nkern + 3 * 64) (template:222-229), each
+    // segment copied in and the pointer moved to the next 64-byte boundary
+    // (template:231-258).
+    auto align64 = [](size_t x) { return (x + 63) / 64 * 64; };
+    size_t req = 0, off = 0;
+    t->ref_offsets.assign(3 * (size_t)P, 0);
+    for (int p = 0; p < P; ++p) {
+        const PanelBuf &pb = t->panels[p];
+        req += 4 * (size_t)pb.nvals + 4 * (size_t)pb.ncols + 4 * (size_t)NK + 3 * 64;
+        t->ref_offsets[3 * p] = (int64_t)off;
+        off = align64(off + 4 * (size_t)pb.nvals);
+        t->ref_offsets[3 * p + 1] = (int64_t)off;
+        off = align64(off + 4 * (size_t)pb.ncols);
+        t->ref_offsets[3 * p + 2] = (int64_t)off;
+        off = align64(off + 4 * (size_t)NK);
+    }
+    t->ref_required = req;
+    t->ref_used = off;
+    SRT_HIP_TRY(hipMalloc(&t->ref_buffer, std::max<size_t>(req, 64)));
+    SRT_HIP_TRY(hipMemsetAsync(t->ref_buffer, 0, std::max<size_t>(req, 64), s));
+    char *rb = static_cast<char *>(t->ref_buffer);
+    for (int p = 0; p < P; ++p) {
+        const PanelBuf &pb = t->panels[p];
+        if (pb.nvals)
+            SRT_HIP_TRY(hipMemcpyAsync(rb + t->ref_offsets[3 * p], pb.vals, sizeof(float) * pb.nvals,
+                                       hipMemcpyDeviceToDevice, s));
+        if (pb.ncols)
+            SRT_HIP_TRY(hipMemcpyAsync(rb + t->ref_offsets[3 * p + 1], pb.cols, sizeof(int32_t) * pb.ncols,
+                                       hipMemcpyDeviceToDevice, s));
+        if (NK)
+            SRT_HIP_TRY(hipMemcpyAsync(rb + t->ref_offsets[3 * p + 2], pb.counts, sizeof(int32_t) * NK,
+                                       hipMemcpyDeviceToDevice, s));
+    }
     SRT_HIP_TRY(hipStreamSynchronize(s));
     DeviceLut *lut = nullptr;
     SRT_TRY(t->map->device_lut(&lut));
@@ -1364,6 +1594,25 @@ int srt_tile_pack_tile(srt_tile *t, srt_stream_t stream) {
         pb.vals = nullptr;
     }
     t->tile_packed = true;
+    return SRT_OK;
+}
+
+int srt_tile_buffer(const srt_tile *t, const void **buffer, size_t *required_bytes, size_t *used_bytes,
+                    int64_t *panel_offsets) {
+    if (!t) return fail(SRT_ERR_INVALID, "null tile");
+    if (!t->tile_packed) return fail(SRT_ERR_INVALID, "pack_tile has not been called");
+    if (buffer) *buffer = t->ref_buffer;
+    if (required_bytes) *required_bytes = t->ref_required;
+    if (used_bytes) *used_bytes = t->ref_used;
+    if (panel_offsets) std::copy(t->ref_offsets.begin(), t->ref_offsets.end(), panel_offsets);
+    return SRT_OK;
+}
+
+int srt_tile_copy_buffer(const srt_tile *t, void *host_out, size_t bytes) {
+    if (!t || (!host_out && bytes)) return fail(SRT_ERR_INVALID, "null argument");
+    if (!t->tile_packed) return fail(SRT_ERR_INVALID, "pack_tile has not been called");
+    if (bytes > t->ref_required) return fail(SRT_ERR_INVALID, "more bytes than the buffer holds");
+    if (bytes) SRT_HIP_TRY(hipMemcpy(host_out, t->ref_buffer, bytes, hipMemcpyDeviceToHost));
     return SRT_OK;
 }
 
@@ -1383,10 +1632,7 @@ int srt_tile_execute(const srt_tile *t, int N_c, const float *B, int64_t ldb, fl
     realise_config(c, t->pk.M_r, t->pk.K, t->pk.K_c ? t->pk.K_c : t->pk.K, t->pk.total_cols, std::max(N_c, 1));
     std::lock_guard<std::mutex> g(t->mu);
     const int tiles = config_tiles(c, std::max(N_c, 1));
-    if (!t->sch.desc || !same_schedule(c, tiles, t->sch_cfg, t->sch)) {
-        SRT_TRY(build_schedule(t->pk, c, t->exec, tiles, t->sch, (hipStream_t)stream));
-        t->sch_cfg = c;
-    }
+    SRT_TRY(switch_schedule(t->pk, c, tiles, t->exec, t->sch, t->sch_cfg, t->retired, (hipStream_t)stream));
     return run_executor(t->pk, t->sch, t->exec, c, B, ldb, C, ldc, N_c, load_c ? 1 : 0,
                         (hipStream_t)stream);
 }
@@ -1417,6 +1663,8 @@ void srt_tile_destroy(srt_tile *t) {
     }
     t->pk.release();
     t->sch.release();
+    for (RetiredSchedule &r : t->retired) r.sch.release();
+    hipFree(t->ref_buffer);
     delete t;
 }
 

@@ -476,13 +476,12 @@ hipError_t launch_exec(const ExecLaunch &a, hipStream_t s) {
         return hipErrorInvalidConfiguration;
     const size_t lds = exec_lds_bytes(MR, V, a.waves_per_block, a.reduce, STAGE, a.K);
     if (lds > 65536) {
-        static bool raised = false;  // once per instantiation
-        if (!raised) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&exec_kernel<MR, V, U, STAGE, D>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
-            if (e != hipSuccess) return e;
-            raised = true;
-        }
+        // once per instantiation; a function-local static is initialised
+        // exactly once even when several threads launch at the same time
+        static const hipError_t raised =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(&exec_kernel<MR, V, U, STAGE, D>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
+        if (raised != hipSuccess) return raised;
     }
     hipLaunchKernelGGL((exec_kernel<MR, V, U, STAGE, D>), dim3((unsigned)nblocks),
                        dim3(64 * a.waves_per_block), lds, s, a.prog, reinterpret_cast<const int4 *>(a.desc),
@@ -497,13 +496,10 @@ hipError_t launch_exec_grouped(const ExecProblem *d_probs, int nprob, int total_
     if (total_blocks <= 0 || nprob <= 0) return hipSuccess;
     if (waves < 1 || waves > 16) return hipErrorInvalidConfiguration;
     if (lds > 65536) {
-        static bool raised = false;
-        if (!raised) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&exec_grouped_kernel<MR, V, U, STAGE, D>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
-            if (e != hipSuccess) return e;
-            raised = true;
-        }
+        static const hipError_t raised =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(&exec_grouped_kernel<MR, V, U, STAGE, D>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
+        if (raised != hipSuccess) return raised;
     }
     hipLaunchKernelGGL((exec_grouped_kernel<MR, V, U, STAGE, D>), dim3((unsigned)total_blocks), dim3(64 * waves),
                        lds, s, (ConstProblems)d_probs, nprob, xcd_remap);

@@ -1,14 +1,21 @@
 // Device-side packer: the reference's MicroKernelPacker / SOPTile::pack_panel
-// moved onto the GPU.
+// moved onto the GPU.  CSR input is packed in memory proportional to its
+// nonzeros (no dense staging):
 //
-//  scatter_csr     per (panel, column) nonzero code (bit r = row r of the
-//                  panel, sbench/SOP/sop_utils.py:25-30) and a dense copy of
-//                  each panel's values laid out [panel][column][row]
-//  count_codes     per panel: split every code into kernel ids through the
+//  cell_keys       one key per CSR entry: its cell (panel, column) as
+//                  panel * k + column, and its row inside the panel
+//  (radix sort)    entries sorted by cell, stably: a cell's entries stay in
+//                  row order
+//  cell_heads      first entry of every cell; a second entry of one (row,
+//                  column) is flagged as a duplicate
+//  cell_fill       per cell: key, first entry, nonzero code (bit r = row r of
+//                  the panel, sbench/SOP/sop_utils.py:25-30)
+//  segment_bounds  first cell of every (panel, K tile) segment
+//  count_cells     per segment: split every code into kernel ids through the
 //                  mapping LUT (sbench/SOP/sop_driver.py:58-65) and count
 //                  packed columns per id (nkern_counts, template:163-164)
-//  exclusive_scan  per-panel packed column / value offsets
-//  fill_codes      stable placement by id (the std::stable_sort of
+//  exclusive_scan  per-segment packed column / value offsets
+//  fill_cells      stable placement by id (the std::stable_sort of
 //                  template:63-73) and value compression (template:158-205)
 //
 // Stability: columns are visited in chunks of 64 in ascending order.  Inside
@@ -18,6 +25,8 @@
 // one step, lanes in column order, so within an id the packed order is
 // ascending column order - the same permutation a stable sort by id gives.
 #include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include "srt_internal.h"
 
@@ -30,56 +39,35 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-template <typename I>
-__global__ void scatter_csr_kernel(int m, int k, int64_t ld, int M_r, const float *__restrict__ Lx,
-                                   const I *__restrict__ Lp, const I *__restrict__ Li,
-                                   uint32_t *__restrict__ codes, float *__restrict__ dense,
-                                   int *__restrict__ bad) {
-    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= m) return;
-    const int p = row / M_r, r = row - p * M_r;
-    const I beg = Lp[row], end = Lp[row + 1];
-    for (I q = beg + lane; q < end; q += 64) {
-        const I col = Li[q];
-        if (col < 0 || col >= (I)k) {
-            atomicOr(bad, 1);
-            continue;
-        }
-        const int64_t pc = (int64_t)p * ld + (int64_t)col;
-        atomicOr(&codes[pc], 1u << r);
-        dense[pc * M_r + r] = Lx[q];
-    }
-}
-
 __device__ __forceinline__ void block_add_i64(unsigned long long *dst, int64_t v) {
     atomicAdd(dst, (unsigned long long)v);
 }
 
-// One workgroup per panel: LDS histogram over kernel ids.
-__global__ void count_codes_kernel(int K, int NK, const int32_t *__restrict__ lut_ptr,
+// One workgroup per segment (panel, K tile): LDS histogram over kernel ids
+// of the segment's cells.
+__global__ void count_cells_kernel(int NK, const int32_t *__restrict__ lut_ptr,
                                    const int32_t *__restrict__ lut_ids,
                                    const int32_t *__restrict__ id_pattern,
-                                   const uint32_t *__restrict__ codes,
+                                   const int64_t *__restrict__ seg_cells,
+                                   const uint32_t *__restrict__ cell_code,
                                    int32_t *__restrict__ counts, int64_t *__restrict__ ncols,
                                    int64_t *__restrict__ nvals) {
     extern __shared__ int hist[];
     __shared__ unsigned long long tot_c, tot_v;
-    const int p = blockIdx.x;
+    const int q = blockIdx.x;
     for (int i = threadIdx.x; i < NK; i += blockDim.x) hist[i] = 0;
     if (threadIdx.x == 0) tot_c = tot_v = 0;
     __syncthreads();
-    const uint32_t *pc = codes + (int64_t)p * K;
-    for (int c = threadIdx.x; c < K; c += blockDim.x) {
-        const uint32_t code = pc[c];
-        if (!code) continue;
+    const int64_t c0 = seg_cells[q], c1 = seg_cells[q + 1];
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const uint32_t code = cell_code[c];
         for (int e = lut_ptr[code]; e < lut_ptr[code + 1]; ++e) atomicAdd(&hist[lut_ids[e]], 1);
     }
     __syncthreads();
     int64_t lc = 0, lv = 0;
     for (int i = threadIdx.x; i < NK; i += blockDim.x) {
         const int h = hist[i];
-        counts[(int64_t)p * NK + i] = h;
+        counts[(int64_t)q * NK + i] = h;
         lc += h;
         lv += (int64_t)h * __popc((unsigned)id_pattern[i]);
     }
@@ -87,8 +75,8 @@ __global__ void count_codes_kernel(int K, int NK, const int32_t *__restrict__ lu
     block_add_i64(&tot_v, lv);
     __syncthreads();
     if (threadIdx.x == 0) {
-        ncols[p] = (int64_t)tot_c;
-        nvals[p] = (int64_t)tot_v;
+        ncols[q] = (int64_t)tot_c;
+        nvals[q] = (int64_t)tot_v;
     }
 }
 
@@ -189,51 +177,133 @@ __device__ __forceinline__ int64_t wave_sum(int64_t v) {
     return v;
 }
 
-// One 64-lane workgroup per panel.
+// One 64-lane workgroup per segment (panel p = q / KT): its cells in column
+// order, 64 at a time.  A cell's entries are in row order, so the value of
+// row r of a kernel pattern is the next entry if that entry is row r, else a
+// stored zero (padding).
 __global__ void __launch_bounds__(64)
-    fill_codes_kernel(int K, int col_offset, int seg, int NK, int M_r, const int32_t *__restrict__ lut_ptr,
+    fill_cells_kernel(int64_t k, int KT, int NK, int M_r, const int32_t *__restrict__ lut_ptr,
                       const int32_t *__restrict__ lut_ids, const int32_t *__restrict__ id_pattern,
-                      const uint32_t *__restrict__ codes, const float *__restrict__ dense,
-                      const int64_t *__restrict__ col_ptr, const int64_t *__restrict__ val_ptr,
-                      const int32_t *__restrict__ counts, int32_t *__restrict__ cols,
-                      float *__restrict__ vals, int64_t *__restrict__ padding) {
+                      const int64_t *__restrict__ seg_cells, const uint64_t *__restrict__ cell_key,
+                      const uint32_t *__restrict__ cell_code, const int64_t *__restrict__ cell_first,
+                      const uint32_t *__restrict__ ent_idx, const uint8_t *__restrict__ ent_row,
+                      const float *__restrict__ Lx, const int64_t *__restrict__ col_ptr,
+                      const int64_t *__restrict__ val_ptr, const int32_t *__restrict__ counts,
+                      int32_t *__restrict__ cols, float *__restrict__ vals, int64_t *__restrict__ padding) {
     extern __shared__ int cur[];
     int *cur_c = cur, *cur_v = cur + NK;
-    const int p = blockIdx.x;
+    const int q = blockIdx.x;
     const int lane = threadIdx.x & 63;
-    id_cursors(NK, counts + (int64_t)p * NK, id_pattern, cur_c, cur_v);
+    id_cursors(NK, counts + (int64_t)q * NK, id_pattern, cur_c, cur_v);
     __syncthreads();
-    const int64_t cbase = col_ptr[p], vbase = val_ptr[p];
-    const uint32_t *pcodes = codes + (int64_t)p * K;
-    // Segment p covers columns [(p % seg) * K, (p % seg + 1) * K) of its row
-    // panel (seg K tiles per panel; seg = 1: the whole row).
-    col_offset += (p % seg) * K;
+    const int64_t cbase = col_ptr[q], vbase = val_ptr[q];
+    const uint64_t panel_base = (uint64_t)(q / KT) * (uint64_t)k;
+    const int64_t s0 = seg_cells[q], s1 = seg_cells[q + 1];
     int64_t zeros = 0;
-    for (int chunk = 0; chunk < K; chunk += 64) {
-        const int c = chunk + lane;
-        const uint32_t code = c < K ? pcodes[c] : 0u;
+    for (int64_t chunk = s0; chunk < s1; chunk += 64) {
+        const int64_t c = chunk + lane;
+        const bool live = c < s1;
+        const uint32_t code = live ? cell_code[c] : 0u;
         const int e0 = code ? lut_ptr[code] : 0;
         const int ne = code ? lut_ptr[code + 1] - e0 : 0;
         const uint32_t pending = ne >= 32 ? 0xffffffffu : ((1u << ne) - 1u);
-        const float *d = dense + ((int64_t)p * K + c) * M_r;
+        const int col = live ? (int)(cell_key[c] - panel_base) : 0;
+        const int64_t f0 = live ? cell_first[c] : 0, f1 = live ? cell_first[c + 1] : 0;
         place_chunk(
             pending, [&](int j) { return lut_ids[e0 + j]; }, cur_c, cur_v, id_pattern,
             [&](int, int id, int pos_c, int pos_v) {
-                cols[cbase + pos_c] = c + col_offset;
+                cols[cbase + pos_c] = col;
                 const unsigned pat = (unsigned)id_pattern[id];
                 float *out = vals + vbase + pos_v;
-                int k = 0;
+                int kk = 0;
+                int64_t f = f0;
                 for (int r = 0; r < M_r; ++r) {
+                    while (f < f1 && (int)ent_row[ent_idx[f]] < r) ++f;
                     if ((pat >> r) & 1u) {
-                        const float v = d[r];
+                        const float v = (f < f1 && (int)ent_row[ent_idx[f]] == r) ? Lx[ent_idx[f]] : 0.0f;
                         zeros += (v == 0.0f);
-                        out[k++] = v;
+                        out[kk++] = v;
                     }
                 }
             });
     }
     zeros = wave_sum(zeros);
-    if (lane == 0) padding[p] = zeros;
+    if (lane == 0) padding[q] = zeros;
+}
+
+// One wave per CSR row: key = panel * k + column of every entry (entry e =
+// position in Li minus Lp[0]), its row inside the panel; columns outside
+// [0, k) set *bad.
+template <typename I>
+__global__ void cell_keys_kernel(int m, int64_t k, int M_r, const I *__restrict__ Lp, const I *__restrict__ Li,
+                                 uint64_t *__restrict__ keys, uint32_t *__restrict__ idx,
+                                 uint8_t *__restrict__ ent_row, int *__restrict__ bad) {
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= m) return;
+    const int p = row / M_r, r = row - p * M_r;
+    const I base = Lp[0];
+    const I beg = Lp[row], end = Lp[row + 1];
+    for (I q = beg + lane; q < end; q += 64) {
+        const I col = Li[q];
+        const int64_t e = (int64_t)(q - base);
+        if (col < 0 || (int64_t)col >= k) {
+            atomicOr(bad, 1);
+            keys[e] = 0;
+        } else {
+            keys[e] = (uint64_t)p * (uint64_t)k + (uint64_t)col;
+        }
+        idx[e] = (uint32_t)e;
+        ent_row[e] = (uint8_t)r;
+    }
+}
+
+// head[i] = 1 where a new cell starts in the sorted keys; two entries of one
+// (row, column) set *bad = 2 (a cell's entries are adjacent, in row order).
+__global__ void cell_heads_kernel(int64_t n, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx,
+                                  const uint8_t *__restrict__ ent_row, uint32_t *__restrict__ head,
+                                  int *__restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool h = i == 0 || keys[i] != keys[i - 1];
+    head[i] = h ? 1u : 0u;
+    if (!h && ent_row[idx[i]] == ent_row[idx[i - 1]]) atomicOr(bad, 2);
+}
+
+// Per cell: key, first entry, code.  cell_first[ncells] = n (sentinel).
+__global__ void cell_fill_kernel(int64_t n, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ idx,
+                                 const uint8_t *__restrict__ ent_row, const uint32_t *__restrict__ head,
+                                 const uint32_t *__restrict__ cell_of, uint64_t *__restrict__ cell_key,
+                                 uint32_t *__restrict__ cell_code, int64_t *__restrict__ cell_first) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == n - 1) cell_first[cell_of[i] + head[i]] = n;
+    if (!head[i]) return;
+    const uint32_t c = cell_of[i];
+    uint32_t code = 0;
+    for (int64_t j = i; j < n && (j == i || !head[j]); ++j) code |= 1u << ent_row[idx[j]];
+    cell_key[c] = keys[i];
+    cell_code[c] = code;
+    cell_first[c] = i;
+}
+
+// seg_cells[q] = first cell whose key >= the start of segment q = (p, t):
+// p * k + t * Kc; seg_cells[Q] = ncells.
+__global__ void segment_bounds_kernel(int Q, int KT, int64_t k, int Kc, int64_t ncells,
+                                      const uint64_t *__restrict__ cell_key, int64_t *__restrict__ seg_cells) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > Q) return;
+    if (q == Q) {
+        seg_cells[q] = ncells;
+        return;
+    }
+    const uint64_t want = (uint64_t)(q / KT) * (uint64_t)k + (uint64_t)(q % KT) * (uint64_t)Kc;
+    int64_t lo = 0, hi = ncells;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cell_key[mid] < want) lo = mid + 1; else hi = mid;
+    }
+    seg_cells[q] = lo;
 }
 
 // One thread per panel: number of ids with a nonzero count.
@@ -509,38 +579,76 @@ __global__ void __launch_bounds__(64)
 
 }  // namespace
 
-hipError_t launch_scatter_csr(int m, int k, int64_t ld, int M_r, const float *Lx, const void *Lp,
-                              const void *Li, int index_bytes, uint32_t *codes, float *dense,
-                              int *bad, hipStream_t s) {
+hipError_t launch_cell_keys(int m, int64_t k, int M_r, const void *Lp, const void *Li, int index_bytes,
+                            uint64_t *keys, uint32_t *idx, uint8_t *ent_row, int *bad, hipStream_t s) {
     if (m <= 0) return hipSuccess;
     const int waves = 4;
     const dim3 grid((m + waves - 1) / waves), block(64 * waves);
     if (index_bytes == 8)
-        hipLaunchKernelGGL(scatter_csr_kernel<int64_t>, grid, block, 0, s, m, k, ld, M_r, Lx,
-                           (const int64_t *)Lp, (const int64_t *)Li, codes, dense, bad);
+        hipLaunchKernelGGL(cell_keys_kernel<int64_t>, grid, block, 0, s, m, k, M_r, (const int64_t *)Lp,
+                           (const int64_t *)Li, keys, idx, ent_row, bad);
     else
-        hipLaunchKernelGGL(scatter_csr_kernel<int32_t>, grid, block, 0, s, m, k, ld, M_r, Lx,
-                           (const int32_t *)Lp, (const int32_t *)Li, codes, dense, bad);
+        hipLaunchKernelGGL(cell_keys_kernel<int32_t>, grid, block, 0, s, m, k, M_r, (const int32_t *)Lp,
+                           (const int32_t *)Li, keys, idx, ent_row, bad);
     return hipGetLastError();
 }
 
-hipError_t launch_count_codes(int P, int K, const DeviceLut &lut, const uint32_t *codes,
+hipError_t sort_cell_keys(int64_t n, int end_bit, uint64_t *keys, uint64_t *keys_alt, uint32_t *idx,
+                          uint32_t *idx_alt, void *temp, size_t *temp_bytes, bool *sorted_in_alt, hipStream_t s) {
+    hipcub::DoubleBuffer<uint64_t> kb(keys, keys_alt);
+    hipcub::DoubleBuffer<uint32_t> vb(idx, idx_alt);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, kb, vb, (int)n, 0, end_bit, s);
+    if (temp && e == hipSuccess) *sorted_in_alt = kb.Current() == keys_alt;
+    return e;
+}
+
+hipError_t launch_cell_heads(int64_t n, const uint64_t *keys, const uint32_t *idx, const uint8_t *ent_row,
+                             uint32_t *head, int *bad, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cell_heads_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, keys, idx,
+                       ent_row, head, bad);
+    return hipGetLastError();
+}
+
+hipError_t scan_cell_heads(int64_t n, const uint32_t *head, uint32_t *cell_of, void *temp, size_t *temp_bytes,
+                           hipStream_t s) {
+    return hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, head, cell_of, (int)n, s);
+}
+
+hipError_t launch_cell_fill(int64_t n, const uint64_t *keys, const uint32_t *idx, const uint8_t *ent_row,
+                            const uint32_t *head, const uint32_t *cell_of, uint64_t *cell_key, uint32_t *cell_code,
+                            int64_t *cell_first, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cell_fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, keys, idx, ent_row,
+                       head, cell_of, cell_key, cell_code, cell_first);
+    return hipGetLastError();
+}
+
+hipError_t launch_segment_bounds(int Q, int KT, int64_t k, int Kc, int64_t ncells, const uint64_t *cell_key,
+                                 int64_t *seg_cells, hipStream_t s) {
+    hipLaunchKernelGGL(segment_bounds_kernel, dim3((Q + 256) / 256), dim3(256), 0, s, Q, KT, k, Kc, ncells,
+                       cell_key, seg_cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_cells(int Q, const DeviceLut &lut, const int64_t *seg_cells, const uint32_t *cell_code,
                               int32_t *counts, int64_t *ncols, int64_t *nvals, hipStream_t s) {
-    if (P <= 0) return hipSuccess;
-    hipLaunchKernelGGL(count_codes_kernel, dim3(P), dim3(256), sizeof(int) * lut.num_patterns, s,
-                       K, lut.num_patterns, lut.lut_ptr, lut.lut_ids, lut.id_pattern, codes,
+    if (Q <= 0) return hipSuccess;
+    hipLaunchKernelGGL(count_cells_kernel, dim3(Q), dim3(256), sizeof(int) * lut.num_patterns, s,
+                       lut.num_patterns, lut.lut_ptr, lut.lut_ids, lut.id_pattern, seg_cells, cell_code,
                        counts, ncols, nvals);
     return hipGetLastError();
 }
 
-hipError_t launch_fill_codes(int P, int K, int col_offset, int seg, const DeviceLut &lut,
-                             const uint32_t *codes, const float *dense, const int64_t *col_ptr,
-                             const int64_t *val_ptr, const int32_t *counts, int32_t *cols,
-                             float *vals, int64_t *padding, hipStream_t s) {
-    if (P <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_codes_kernel, dim3(P), dim3(64), 2 * sizeof(int) * lut.num_patterns,
-                       s, K, col_offset, seg, lut.num_patterns, lut.M_r, lut.lut_ptr, lut.lut_ids,
-                       lut.id_pattern, codes, dense, col_ptr, val_ptr, counts, cols, vals,
+hipError_t launch_fill_cells(int Q, int64_t k, int KT, const DeviceLut &lut, const int64_t *seg_cells,
+                             const uint64_t *cell_key, const uint32_t *cell_code, const int64_t *cell_first,
+                             const uint32_t *ent_idx, const uint8_t *ent_row, const float *Lx,
+                             const int64_t *col_ptr, const int64_t *val_ptr, const int32_t *counts,
+                             int32_t *cols, float *vals, int64_t *padding, hipStream_t s) {
+    if (Q <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_cells_kernel, dim3(Q), dim3(64), 2 * sizeof(int) * lut.num_patterns, s, k, KT,
+                       lut.num_patterns, lut.M_r, lut.lut_ptr, lut.lut_ids, lut.id_pattern, seg_cells, cell_key,
+                       cell_code, cell_first, ent_idx, ent_row, Lx, col_ptr, val_ptr, counts, cols, vals,
                        padding);
     return hipGetLastError();
 }

@@ -146,16 +146,37 @@ struct Schedule {
     void release();
 };
 
+// A schedule no longer current, kept with the realised config it was built for.
+struct RetiredSchedule {
+    srt_tile_config cfg;
+    Schedule sch;
+};
+
 // Kernel launchers (packer.hip).
-hipError_t launch_scatter_csr(int m, int k, int64_t ld, int M_r, const float *Lx, const void *Lp,
-                              const void *Li, int index_bytes, uint32_t *codes, float *dense,
-                              int *bad, hipStream_t s);
-hipError_t launch_count_codes(int P, int K, const DeviceLut &lut, const uint32_t *codes,
+// CSR packing in O(nnz) memory (packer.hip): cell keys, radix sort, cells,
+// segment bounds, per-segment counts and fill.  The two hipcub-backed steps
+// follow hipcub's protocol: called with temp == nullptr they only store the
+// temporary storage they need in *temp_bytes.
+hipError_t launch_cell_keys(int m, int64_t k, int M_r, const void *Lp, const void *Li, int index_bytes,
+                            uint64_t *keys, uint32_t *idx, uint8_t *ent_row, int *bad, hipStream_t s);
+hipError_t sort_cell_keys(int64_t n, int end_bit, uint64_t *keys, uint64_t *keys_alt, uint32_t *idx,
+                          uint32_t *idx_alt, void *temp, size_t *temp_bytes, bool *sorted_in_alt, hipStream_t s);
+hipError_t launch_cell_heads(int64_t n, const uint64_t *keys, const uint32_t *idx, const uint8_t *ent_row,
+                             uint32_t *head, int *bad, hipStream_t s);
+hipError_t scan_cell_heads(int64_t n, const uint32_t *head, uint32_t *cell_of, void *temp, size_t *temp_bytes,
+                           hipStream_t s);
+hipError_t launch_cell_fill(int64_t n, const uint64_t *keys, const uint32_t *idx, const uint8_t *ent_row,
+                            const uint32_t *head, const uint32_t *cell_of, uint64_t *cell_key, uint32_t *cell_code,
+                            int64_t *cell_first, hipStream_t s);
+hipError_t launch_segment_bounds(int Q, int KT, int64_t k, int Kc, int64_t ncells, const uint64_t *cell_key,
+                                 int64_t *seg_cells, hipStream_t s);
+hipError_t launch_count_cells(int Q, const DeviceLut &lut, const int64_t *seg_cells, const uint32_t *cell_code,
                               int32_t *counts, int64_t *ncols, int64_t *nvals, hipStream_t s);
-hipError_t launch_fill_codes(int P, int K, int col_offset, int seg, const DeviceLut &lut,
-                             const uint32_t *codes, const float *dense, const int64_t *col_ptr,
-                             const int64_t *val_ptr, const int32_t *counts, int32_t *cols,
-                             float *vals, int64_t *padding, hipStream_t s);
+hipError_t launch_fill_cells(int Q, int64_t k, int KT, const DeviceLut &lut, const int64_t *seg_cells,
+                             const uint64_t *cell_key, const uint32_t *cell_code, const int64_t *cell_first,
+                             const uint32_t *ent_idx, const uint8_t *ent_row, const float *Lx,
+                             const int64_t *col_ptr, const int64_t *val_ptr, const int32_t *counts,
+                             int32_t *cols, float *vals, int64_t *padding, hipStream_t s);
 hipError_t launch_exclusive_scan(int n, const int64_t *in, int64_t *out, hipStream_t s);
 // Non-empty group lists from counts: ngroups[p], then (after a scan into
 // grp_ptr) the (id | popcount(pattern) << 16, count) pairs.

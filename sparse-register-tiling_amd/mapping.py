@@ -125,20 +125,68 @@ def save_mapping(M_r: int, mapping: Callable[[int], Iterable[int]], directory: s
     return mid
 
 
+def kernel_id_order(kernel_list: Iterable[int]) -> List[int]:
+    """Kernel patterns in id order: the list the reference hands its executor
+    generator (``nanokernels=list(patterns)``, sbench/SOP/sop_driver.py:93),
+    stable-sorted by popcount (tools/codegen/gen_vector_specialized.py:113;
+    id = index in the sorted list, :130-137).  Repeated patterns keep their
+    first position."""
+    seen, order = set(), []
+    for p in kernel_list:
+        p = int(p)
+        if p and p not in seen:
+            seen.add(p)
+            order.append(p)
+    return sorted(order, key=popcount)
+
+
+def file_kernel_list(table: Dict[int, Sequence[int]]) -> List[int]:
+    """The kernel list the reference builds for a mapping read from a file:
+    ``load_mapping`` adds every kernel pattern to a Python ``set`` line by
+    line (sbench/SOP/sop_bench.py:72-95) and ``make_sop_module`` passes
+    ``list(patterns)`` on (sop_driver.py:93).  The same construct is used
+    here, so the order is the interpreter's iteration order of that set."""
+    pats = set()
+    for v in table.values():
+        for p in v:
+            if int(p):
+                pats.add(int(p))
+    return list(pats)
+
+
 class Mapping:
     """A loaded mapping: ``M_r``, the code table, its id and kernel patterns.
 
-    Kernel ids follow (popcount, bitmask) order of the pattern set.
+    ``kernel_list`` is the pattern list the reference's generator receives:
+    a strategy's ``all_patterns()`` (merge then split patterns,
+    sbench/SOP/sop_hand_configs.py:16-17, passed at sop_bench.py:105,154), or,
+    when None, the ``list(set)`` of a mapping file (``file_kernel_list``).
+    ``patterns`` is the id order derived from it (``kernel_id_order``); every
+    pattern the table uses must be in the list, and listed patterns the table
+    never uses still get an id, as in the reference.
     """
 
-    def __init__(self, M_r: int, table: Dict[int, List[int]], mid: Optional[str] = None):
+    def __init__(self, M_r: int, table: Dict[int, List[int]], mid: Optional[str] = None,
+                 kernel_list: Optional[Sequence[int]] = None):
         self.M_r = int(M_r)
         self.table = OrderedDict((int(k), [int(x) for x in v]) for k, v in table.items())
         self.id = mid or mapping_id(self.table)
-        pats = set()
-        for v in self.table.values():
-            pats.update(int(x) for x in v if x)
-        self.patterns = sorted(pats, key=lambda p: (popcount(p), p))
+        self.kernel_list = [int(p) for p in (file_kernel_list(self.table) if kernel_list is None
+                                             else kernel_list)]
+        self.patterns = kernel_id_order(self.kernel_list)
+        listed = set(self.patterns)
+        for code, v in self.table.items():
+            for p in v:
+                if p and p not in listed:
+                    raise ValueError("code %d maps to pattern %d, which the kernel list lacks" % (code, p))
+                if p >= (1 << self.M_r):
+                    raise ValueError("pattern %d does not fit M_r=%d" % (p, self.M_r))
+
+    @classmethod
+    def from_strategy(cls, M_r: int, strategy: "SOPOrig") -> "Mapping":
+        """The mapping a strategy generates, with the strategy's pattern list
+        (what sop_bench.py:105,154 hands the generator)."""
+        return cls(M_r, mapping_table(M_r, strategy.map_pattern), kernel_list=strategy.all_patterns())
 
     def __call__(self, code: int) -> List[int]:
         code = int(code)
@@ -232,7 +280,39 @@ def generate_shipped_mappings(directory: str = MAPPINGS_DIR) -> Dict[str, str]:
     return out
 
 
+def strategy_for(M_r: int, name_or_id: str) -> Optional[str]:
+    """The shipped strategy behind a config name or a mapping id, if any."""
+    mid = NAMED_IDS.get(M_r, {}).get(name_or_id, name_or_id)
+    for sname, make in STRATEGIES.items():
+        if STRATEGY_M_R[sname] == M_r and _strategy_id(sname) == mid:
+            return sname
+    return None
+
+
+_STRATEGY_IDS: Dict[str, str] = {}
+
+
+def _strategy_id(name: str) -> str:
+    if name not in _STRATEGY_IDS:
+        _STRATEGY_IDS[name] = mapping_id(mapping_table(STRATEGY_M_R[name], STRATEGIES[name]().map_pattern))
+    return _STRATEGY_IDS[name]
+
+
+def strategy_mapping(name: str) -> Mapping:
+    """Mapping of a shipped strategy, ids from the strategy's pattern list."""
+    m = Mapping.from_strategy(STRATEGY_M_R[name], STRATEGIES[name]())
+    return m
+
+
 def named_mapping(M_r: int, name: str, search_dirs: Optional[Sequence[str]] = None) -> Mapping:
-    """Mapping by the reference's config name ("identity", "orig", "alt") or by id."""
+    """Mapping by the reference's config name ("identity", "orig", "alt") or by id.
+
+    A name or id that a shipped strategy generates gets that strategy's
+    kernel list (the path of sbench/SOP/sop_bench.py:105,154, which wrote the
+    mapping files); any other id is read from its file and gets the file's
+    kernel list (the load_mapping path, sop_bench.py:72-95)."""
+    sname = strategy_for(M_r, name)
+    if sname is not None:
+        return strategy_mapping(sname)
     mid = NAMED_IDS.get(M_r, {}).get(name, name)
     return read_mapping_file(find_mapping(mid, search_dirs))

@@ -8,7 +8,8 @@ from typing import Optional, Sequence, Union
 import torch
 
 from . import _lib
-from .mapping import MAPPINGS_DIR, Mapping, named_mapping, read_mapping_file, find_mapping
+from .mapping import (MAPPINGS_DIR, STRATEGY_M_R, Mapping, find_mapping, named_mapping, read_mapping_file,
+                      strategy_for)
 
 
 def stream_handle(stream: Optional[torch.cuda.Stream] = None) -> ct.c_void_p:
@@ -35,7 +36,8 @@ def ptr(t: Optional[torch.Tensor]) -> ct.c_void_p:
 
 
 class NativeMapping:
-    """An srt_mapping handle.  Kernel ids follow (popcount, bitmask) order."""
+    """An srt_mapping handle.  Kernel ids are the stable popcount sort of the
+    mapping's kernel list (``Mapping.kernel_list``)."""
 
     def __init__(self, handle: ct.c_void_p, mapping: Optional[Mapping] = None):
         self._h = handle
@@ -52,13 +54,17 @@ class NativeMapping:
     def from_mapping(cls, m: Mapping) -> "NativeMapping":
         code_ptr, flat = m.code_arrays()
         h = ct.c_void_p()
-        _lib.call("srt_mapping_create", m.M_r, _lib.int32_array(code_ptr), _lib.int32_array(flat),
-                  ct.byref(h))
-        return cls(h, m)
+        _lib.call("srt_mapping_create_ordered", m.M_r, _lib.int32_array(code_ptr), _lib.int32_array(flat),
+                  _lib.int32_array(m.kernel_list or [0]), len(m.kernel_list), ct.byref(h))
+        native = cls(h, m)
+        if native.patterns != m.patterns:
+            raise RuntimeError("kernel id order differs between the C library and the mapping")
+        return native
 
     @classmethod
     def from_id(cls, mapping_id: str, search_dirs: Optional[Sequence[str]] = None) -> "NativeMapping":
-        """Read mapping_<id>.txt through the C reader (srt_mapping_read)."""
+        """Read mapping_<id>.txt through the C reader (srt_mapping_read): the
+        file's kernel list, as the reference's load_mapping builds it."""
         dirs = [os.fsencode(d) for d in list(search_dirs or []) + [MAPPINGS_DIR]]
         arr = (ct.c_char_p * len(dirs))(*dirs)
         h = ct.c_void_p()
@@ -87,13 +93,18 @@ class NativeMapping:
 def resolve_mapping(mapping: Union[str, Mapping, NativeMapping], M_r: Optional[int] = None,
                     search_dirs: Optional[Sequence[str]] = None) -> NativeMapping:
     """A NativeMapping from an id ("400fa"), a config name with M_r ("orig"),
-    a Mapping, or a NativeMapping."""
+    a Mapping, or a NativeMapping.  Names and ids of the shipped strategies
+    carry the strategy's kernel list (``named_mapping``); other ids are read
+    by the C reader."""
     if isinstance(mapping, NativeMapping):
         return mapping
     if isinstance(mapping, Mapping):
         return NativeMapping.from_mapping(mapping)
     if M_r is not None:
         return NativeMapping.from_mapping(named_mapping(M_r, mapping, search_dirs))
+    for mr in sorted(set(STRATEGY_M_R.values())):
+        if strategy_for(mr, mapping) is not None:
+            return NativeMapping.from_mapping(named_mapping(mr, mapping, search_dirs))
     return NativeMapping.from_id(mapping, search_dirs)
 
 

@@ -228,7 +228,12 @@ def groups_to_mapping(groups: Sequence[Sequence[int]], M_r: int) -> Mapping:
         for q in g:
             owner[q] = k
     full = (1 << M_r) - 1
-    return Mapping(M_r, mapping_table(M_r, lambda q: [owner.get(q, full)]))
+    # kernel list in group order, as the ILP's output file lists its kernels
+    # one per line (read by sbench/SOP/sop_bench_ilp.py:98-111)
+    kernels = [_or(g) for g in groups]
+    if any(q not in owner for q in range(1, full + 1)) and full not in kernels:
+        kernels.append(full)
+    return Mapping(M_r, mapping_table(M_r, lambda q: [owner.get(q, full)]), kernel_list=kernels)
 
 
 def specialized_mapping(A: np.ndarray, M_r: int, max_groups: int = 16) -> Tuple[Mapping, float]:

@@ -107,6 +107,21 @@ class SOPTile:
         _lib.call("srt_tile_padding", self._h, ct.byref(v))
         return v.value
 
+    def buffer(self) -> Dict[str, object]:
+        """The packed tile in the reference's buffer layout (template:219-261):
+        ``required_bytes`` (template:222-229), ``used_bytes``, per panel the
+        byte offsets of its values / col_indices / nkern_counts segments
+        (``offsets`` [P, 3]) and a host copy of the buffer (``data``, uint8)."""
+        dev = ct.c_void_p()
+        req, used = ct.c_size_t(), ct.c_size_t()
+        offs = np.zeros(max(1, 3 * self.num_panels), dtype=np.int64)
+        _lib.call("srt_tile_buffer", self._h, ct.byref(dev), ct.byref(req), ct.byref(used),
+                  offs.ctypes.data_as(ct.POINTER(ct.c_int64)))
+        data = np.zeros(max(1, req.value), dtype=np.uint8)
+        _lib.call("srt_tile_copy_buffer", self._h, data.ctypes.data_as(ct.c_void_p), req.value)
+        return {"required_bytes": req.value, "used_bytes": used.value,
+                "offsets": offs[:3 * self.num_panels].reshape(self.num_panels, 3), "data": data[:req.value]}
+
     def execute(self, N_c: int, B: torch.Tensor, C: torch.Tensor, load_c: bool,
                 tile_config: Optional[TileConfig] = None, stream: Optional[torch.cuda.Stream] = None) -> None:
         cfg = (tile_config or TileConfig()).to_c()
@@ -244,13 +259,16 @@ def make_sop_module(acc: Acc, patterns: Iterable[int], pattern_mapping: Union[Ca
     patterns = list(patterns)
     if 0 in patterns:
         raise ValueError("Zero is not a valid pattern")
+    # the generator receives list(patterns) (sop_driver.py:93): that list,
+    # stable-sorted by popcount, fixes the kernel ids
     if isinstance(pattern_mapping, Mapping):
-        m = pattern_mapping
+        table = pattern_mapping.table
     else:
-        m = Mapping(acc.M, mapping_table(acc.M, lambda c: [p for p in pattern_mapping(c) if p]))
-    extra = set(m.patterns) - set(patterns)
+        table = mapping_table(acc.M, lambda c: [p for p in pattern_mapping(c) if p])
+    extra = {p for v in table.values() for p in v if p} - set(patterns)
     if extra:
         raise ValueError("mapping uses patterns %s that are not in the pattern set" % sorted(extra))
+    m = Mapping(acc.M, table, kernel_list=patterns)
     native = NativeMapping.from_mapping(m)
     from . import jit
     kernel_id = jit.ensure_executor(native, regen=regen)

@@ -119,3 +119,37 @@ def test_balanced_layout_tiny(srt, oracle):
         B = srt.workloads.dense_b(K, N, seed=K)
         plan, _ = _check(srt, oracle, A, B, 4, "orig", waves_per_block=16, nsplit=16)
         plan.close()
+
+
+def test_graph_replays_after_allocate_executor(srt, oracle):
+    """A graph captured at one width keeps replaying exactly after
+    allocate_executor switched the plan to another schedule (the earlier
+    schedule stays alive), and switching back reuses it."""
+    import torch
+    so, cb = oracle
+    A = _uneven(600, 256, 0.1, seed=19)
+    plan = _plan(srt, torch, A, 64, 4, "orig", layout=2, waves_per_block=8, vec_width=1)
+    pk = cb.pack_dense(A, 4, _oracle_table(so, 4, "orig"))
+    plan.allocate_executor(64)
+    kp64 = plan.split_counts().copy()
+    B = torch.from_numpy(srt.workloads.dense_b(256, 64, seed=3)).cuda()
+    ref = cb.exec_packed(pk, B.cpu().numpy(), nsplit=kp64)
+    C = torch.full((600, 64), float("nan"), device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan(C, B, stream=s)
+    for n in (448, 64, 448):
+        plan.allocate_executor(n)          # another width: another schedule becomes current
+        B2 = srt.workloads.dense_b(256, n, seed=n)
+        C2 = _run(srt, torch, plan, B2)
+        _bitexact(C2, cb.exec_packed(pk, B2, nsplit=plan.split_counts()))
+        for _ in range(2):
+            C.fill_(float("nan"))
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            _bitexact(C.cpu().numpy(), ref)
+    del g
+    plan.close()

@@ -32,11 +32,9 @@ def torch_cuda():
 
 
 def _oracle_table(so, M_r, name):
-    if name == "orig" and M_r == 4:
-        return so.mapping_table(4, so.orig4_strategy().map_pattern)
-    mid = {(4, "identity"): "61fee", (8, "orig"): "400fa", (8, "alt"): "747f9",
-           (8, "identity"): "161fb", (2, "identity"): "4af51", (6, "identity"): "001e3"}[(M_r, name)]
-    return so.read_mapping(os.path.join(GOLDEN_MAPPINGS, "mapping_%s.txt" % mid))[1]
+    """The oracle's own restatement of a named strategy: its table and the
+    strategy's kernel list (the ids the product must reproduce)."""
+    return so.named_table(M_r, name)
 
 
 def _plan(srt, torch, A, n, M_r, name, index_dtype=np.int64, **cfg):
@@ -284,6 +282,18 @@ def test_soptile_matches_reference_packing(srt, oracle, torch_cuda, M_r, name, s
         assert np.array_equal(got["cols"][got["col_ptr"][p]:got["col_ptr"][p + 1]], pd["col_indices"])
         assert np.array_equal(got["vals"][got["val_ptr"][p]:got["val_ptr"][p + 1]], pd["values"])
     assert st.padding() == sum(pd["padding"] for pd in panels)
+    # the reference's single buffer (template:219-261): offsets, required
+    # bytes and every segment's content
+    buf = st.buffer()
+    offs, end = so.pack_tile_offsets(panels)
+    assert buf["required_bytes"] == so.pack_tile_required_bytes(panels)
+    assert buf["used_bytes"] == end and np.array_equal(buf["offsets"], np.array(offs, dtype=np.int64))
+    data = buf["data"]
+    for (v, c, n), pd in zip(offs, panels):
+        assert v % 64 == 0 and c % 64 == 0 and n % 64 == 0
+        assert np.array_equal(data[v:v + 4 * pd["num_nnz"]].view(np.float32), pd["values"])
+        assert np.array_equal(data[c:c + 4 * pd["num_col_indices"]].view(np.int32), pd["col_indices"])
+        assert np.array_equal(data[n:n + 4 * pd["num_nkern"]].view(np.int32), pd["nkern_counts"])
     pk = cb.pack_dense(tile, M_r, table)
     for nsplit in (1, 4):
         ns, C = module.execute_tile(st, torch.from_numpy(B).cuda(), num_runs=2, measured_iterations=3,

@@ -92,3 +92,50 @@ def test_admission_by_budget(env):
     with pytest.raises(MemoryError):
         big = srt.serving.Request(rid=99, weight=3, B=torch.zeros((520, 512), dtype=torch.bfloat16, device="cuda"))
         server.run_batch([big])
+
+
+@pytest.fixture(scope="module")
+def full_env():
+    """The serving workload at its real shapes: 48 weights, 12 layers x
+    {1024x1024 x2, 4096x1024, 1024x4096}, 90% magnitude-pruned, BF16."""
+    import importlib
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    Ws = srt.serving.serving_weights()
+    plans = [srt.precision.PanelPlan.from_dense(torch.from_numpy(W), "bf16") for W in Ws]
+    return srt, torch, Ws, plans
+
+
+def test_full_serving_workload(full_env):
+    """48 resident weights, Poisson requests with log-uniform N up to 512:
+    every served result within the bound of the FP64 product of the BF16
+    operands and rel <= 1e-2 of the FP64 product of the unquantised inputs,
+    and each result bit-identical alone and in the mix."""
+    srt, torch, Ws, plans = full_env
+    shapes = [W.shape for W in Ws]
+    assert len(Ws) == 48 and sorted(set(shapes)) == [(1024, 1024), (1024, 4096), (4096, 1024)]
+    _log("serving 48 weights at full shapes")
+    server = srt.serving.Server(plans, budget_bytes=256 << 20, max_batch=32, slots=3, grid=1024)
+    reqs = srt.serving.poisson_requests(48, 50000.0, shapes, seed=11, n_max=512)
+    # the sample spans every weight shape and N up to 512
+    assert max(r.N for r in reqs) > 256 and {shapes[r.weight] for r in reqs} == set(shapes)
+    big = srt.serving.Request(rid=999, weight=2, B=(torch.rand((1024, 512), device="cuda") * 2 - 1).to(torch.bfloat16))
+    reqs.append(big)
+    st = server.serve(reqs, keep_results=True)
+    _log("  ", st)
+    assert st["requests"] == len(reqs)
+    served = {}
+    for r in reqs:
+        _check(r, Ws[r.weight])
+        served[r.rid] = r.C.cpu().numpy()
+    # alone == mixed at these shapes (a batch holding every request, reversed)
+    sample = reqs[:24] + [big]
+    mixed = [srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B) for r in sample]
+    server.run_batch(list(reversed(mixed)), slot_index=1)
+    for r in mixed:
+        alone = srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B)
+        server.run_batch([alone], slot_index=2)
+        a = alone.C.cpu().numpy()
+        assert np.array_equal(r.C.cpu().numpy().view(np.uint32), a.view(np.uint32)), "request %d" % r.rid
+        assert np.array_equal(a.view(np.uint32), served[r.rid].view(np.uint32)), "request %d" % r.rid
+    assert server.pool.used == 0

@@ -28,7 +28,7 @@ def test_specialized_mapping_executor(srt, oracle):
     plan(C, torch.from_numpy(B).cuda())
     torch.cuda.synchronize()
     C = C.cpu().numpy()
-    pk = cb.pack_dense(A, 4, m.table)
+    pk = cb.pack_dense(A, 4, m.table, kernel_list=m.kernel_list)
     assert plan.stats()["padding"] == pk["padding"]
     ref = cb.exec_packed(pk, B, nsplit=plan.split_counts())
     assert np.array_equal(C.view(np.uint32), ref.view(np.uint32))

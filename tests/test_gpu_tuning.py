@@ -35,7 +35,8 @@ def test_tuned_plan_exact(srt, oracle, case):
     real = plan.get_config()
     assert plan.mapping.M_r == rec["M_r"] and real.vec_width == rec["vec_width"]
     assert real.K_c == (rec.get("K_c") or c.K)
-    pk = cb.pack_dense_ktiled(A, rec["M_r"], plan.mapping.mapping.table, real.K_c)
+    pk = cb.pack_dense_ktiled(A, rec["M_r"], plan.mapping.mapping.table, real.K_c,
+                              kernel_list=plan.mapping.mapping.kernel_list)
     _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=plan.split_counts()))
     plan.close()
 

@@ -33,8 +33,84 @@ def test_named_mappings(srt):
     assert srt.named_mapping(8, "orig").id == "400fa"
     assert srt.named_mapping(8, "alt").id == "747f9"
     m = srt.named_mapping(8, "orig")
-    assert m.patterns == [1, 2, 4, 8, 16, 32, 64, 128, 15, 60, 85, 170, 195, 240, 63, 207, 243, 252, 255]
     assert m(0) == [] and m(3) == [195]
+
+
+# Kernel id tables, written out: the reference generator's stable popcount
+# sort (tools/codegen/gen_vector_specialized.py:113) of the list its benchmark
+# passes, config.all_patterns() = merge then split patterns
+# (sbench/SOP/sop_hand_configs.py:16-17,99-103,160-169,209-219).
+KAT_400FA = [1, 2, 4, 8, 16, 32, 64, 128, 85, 170, 195, 60, 15, 240, 252, 243, 207, 63, 255]
+KAT_DA01E = [1, 2, 4, 8, 14, 13, 11, 7, 15]
+KAT_61FEE = [1, 2, 4, 8, 3, 5, 6, 9, 10, 12, 7, 11, 13, 14, 15]
+# the same 400fa / 747f9 pattern set read from its file: list(set) of the
+# file's patterns (load_mapping, sbench/SOP/sop_bench.py:72-95) -- 128 sits in
+# slot 0 of the interpreter's 64-slot table -- then the same sort
+KAT_400FA_FILE = [128, 1, 2, 4, 8, 16, 32, 64, 15, 170, 60, 195, 85, 240, 63, 207, 243, 252, 255]
+
+
+def test_kernel_id_kats(srt):
+    assert srt.named_mapping(8, "orig").patterns == KAT_400FA
+    assert srt.named_mapping(8, "alt").patterns == KAT_400FA
+    assert srt.named_mapping(4, "orig").patterns == KAT_DA01E
+    assert srt.named_mapping(4, "identity").patterns == KAT_61FEE
+    assert srt.named_mapping(8, "400fa").patterns == KAT_400FA   # by id: the strategy's list
+    for M_r, name, kat in [(8, "orig", KAT_400FA), (8, "alt", KAT_400FA), (4, "orig", KAT_DA01E),
+                           (4, "identity", KAT_61FEE)]:
+        assert srt.resolve_mapping(name, M_r=M_r).patterns == kat    # through srt_mapping_create_ordered
+    assert srt.resolve_mapping("400fa").patterns == KAT_400FA
+    # the file path, in Python and through the C reader
+    for mid in ("400fa", "747f9"):
+        assert srt.read_mapping_file(os.path.join(GOLDEN_MAPPINGS, "mapping_%s.txt" % mid)).patterns == KAT_400FA_FILE
+        assert srt.NativeMapping.from_id(mid, [GOLDEN_MAPPINGS]).patterns == KAT_400FA_FILE
+
+
+def test_make_sop_module_ids_follow_the_pattern_list(srt):
+    """make_sop_module(acc, patterns, mapping): the ids come from the
+    ``patterns`` argument (sop_driver.py:93), unused patterns included."""
+    strat = srt.SOPOrig([14, 13, 11, 7, 15], [1, 2, 4, 8], split_cost=10000)
+    pats = strat.all_patterns()
+    m = srt.Mapping(4, srt.mapping.mapping_table(4, strat.map_pattern), kernel_list=pats + [3])
+    assert m.patterns == [1, 2, 4, 8, 3, 14, 13, 11, 7, 15]
+    with pytest.raises(ValueError):
+        srt.Mapping(4, srt.mapping.mapping_table(4, strat.map_pattern), kernel_list=[1, 2, 4, 8])
+
+
+def _shuffled_mapping_file(path, M_r, rng, keep_frac):
+    """A mapping file whose lines come in random order: a kept code maps to
+    itself, the others to the full pattern.  Returns the kernel patterns in
+    the order the lines add them."""
+    full = (1 << M_r) - 1
+    codes = list(range(1, full + 1))
+    rng.shuffle(codes)
+    keep = set(c for c in codes if rng.random() < keep_frac)
+    seq = []
+    with open(path, "w") as f:
+        f.write("%d\n" % M_r)
+        for c in codes:
+            k = c if c in keep else full
+            f.write("%d: [%d]\n" % (c, k))
+            seq.append(k)
+    return seq
+
+
+def test_file_id_order_is_the_interpreters_set_order(srt, tmp_path):
+    """The C reader's ids equal the stable popcount sort of ``list(set)``
+    built by this interpreter in file-line order, for random line orders and
+    pattern subsets (the construct of the reference's load_mapping)."""
+    import random
+    rng = random.Random(5)
+    for trial in range(24):
+        M_r = [3, 5, 7, 9, 11][trial % 5]
+        d = tmp_path / ("t%d" % trial)
+        d.mkdir()
+        seq = _shuffled_mapping_file(str(d / "mapping_zzzzz.txt"), M_r, rng, rng.choice([0.02, 0.2, 0.7, 1.0]))
+        pyset = set()
+        for k in seq:
+            pyset.add(k)
+        want = sorted(list(pyset), key=lambda p: bin(p).count("1"))
+        assert srt.NativeMapping.from_id("zzzzz", [str(d)]).patterns == want, trial
+        assert srt.read_mapping_file(str(d / "mapping_zzzzz.txt")).patterns == want
 
 
 def test_load_mapping_like_reference(srt):
@@ -87,7 +163,13 @@ def test_codegen_rejects_bad_sets():
         codegen.gen_executor_sources(4, [])
     with pytest.raises(ValueError):
         codegen.gen_executor_sources(4, [16])
-    assert codegen.canonical_patterns([15, 1, 3, 2]) == [1, 2, 3, 15]
+    assert codegen.id_ordered_patterns([2, 1, 15]) == [2, 1, 15]
+    with pytest.raises(ValueError):
+        codegen.id_ordered_patterns([1, 3, 2])     # not in stable popcount order
+    with pytest.raises(ValueError):
+        codegen.id_ordered_patterns([1, 1])
+    # the same set in another id order is another executor
+    assert codegen.executor_id(4, [1, 2, 4, 8, 14, 13, 11, 7, 15]) != codegen.executor_id(4, [1, 2, 4, 8, 7, 11, 13, 14, 15])
 
 
 # ------------------------------------------------------------------ C ABI
@@ -119,6 +201,9 @@ def test_shipped_executors_cover_named_mappings(srt):
     ids = {e[0] for e in srt.executors()}
     for M_r, name in [(4, "identity"), (4, "orig"), (8, "orig"), (8, "alt")]:
         nm = srt.resolve_mapping(name, M_r=M_r)
+        assert nm.executor_id() in ids
+    for mid in ("61fee", "da01e", "400fa", "747f9"):      # the file path (load_mapping's ids)
+        nm = srt.NativeMapping.from_id(mid)
         assert nm.executor_id() in ids
 
 

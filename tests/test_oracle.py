@@ -51,6 +51,44 @@ def test_da01e_id(oracle):
     assert so.mapping_id(t2) != "da01e"
 
 
+# Kernel id tables written out (tools/codegen/gen_vector_specialized.py:113
+# stable popcount sort of the strategy's all_patterns(), merge then split:
+# sbench/SOP/sop_hand_configs.py:16-17,99-103,160-169,209-219).
+ID_KATS = {(8, "orig"): [1, 2, 4, 8, 16, 32, 64, 128, 85, 170, 195, 60, 15, 240, 252, 243, 207, 63, 255],
+           (8, "alt"): [1, 2, 4, 8, 16, 32, 64, 128, 85, 170, 195, 60, 15, 240, 252, 243, 207, 63, 255],
+           (4, "orig"): [1, 2, 4, 8, 14, 13, 11, 7, 15],
+           (4, "identity"): [1, 2, 4, 8, 3, 5, 6, 9, 10, 12, 7, 11, 13, 14, 15],
+           (2, "identity"): [1, 2, 3]}
+
+
+@pytest.mark.parametrize("key", sorted(ID_KATS))
+def test_kernel_id_kats(oracle, key):
+    so, _ = oracle
+    table = so.named_table(*key)
+    assert so.kernel_pattern_table(table) == ID_KATS[key]
+    # the table itself is the committed / published mapping
+    mid = {(8, "orig"): "400fa", (8, "alt"): "747f9", (4, "identity"): "61fee", (2, "identity"): "4af51"}.get(key)
+    if mid:
+        assert so.mapping_file_text(key[0], table) == _golden(mid)
+    else:
+        assert so.mapping_id(table) == "da01e"
+
+
+def test_file_kernel_ids(oracle):
+    """A mapping read from a file: list(set) of its patterns in line order
+    (load_mapping), then the stable popcount sort."""
+    so, _ = oracle
+    M_r, table = so.read_mapping(os.path.join(GOLDEN_MAPPINGS, "mapping_400fa.txt"))
+    assert so.kernel_pattern_table(table) == [128, 1, 2, 4, 8, 16, 32, 64, 15, 170, 60, 195, 85, 240,
+                                              63, 207, 243, 252, 255]
+    # identity files: the set of 1..2^M_r-1 iterates ascending, so ids are (popcount, value)
+    M_r, table = so.read_mapping(os.path.join(GOLDEN_MAPPINGS, "mapping_161fb.txt"))
+    assert so.kernel_pattern_table(table) == sorted(range(1, 256), key=lambda p: (bin(p).count("1"), p))
+    # a pattern the table uses must be listed
+    with pytest.raises(ValueError):
+        so.kernel_pattern_table(table, kernel_list=[1, 2, 3])
+
+
 def test_every_golden_file_id(oracle):
     so, _ = oracle
     files = sorted(glob.glob(os.path.join(GOLDEN_MAPPINGS, "mapping_*.txt")))
