@@ -64,7 +64,7 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--form", default="plain", choices=["plain", "precision", "serving", "cnn", "sweep"],
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "serving", "cnn", "sweep", "floor"],
                     help="plain = the headline FP32 workload (default); the others print their own line")
     ap.add_argument("--qps", default="5000,10000,20000,40000,80000,160000",
                     help="serving: offered loads (requests/s)")
@@ -367,25 +367,90 @@ def bench_precision(args):
                       "unit": "GFLOP/s", "n_gpus": 1, "data": "synthetic", "results": rows}), flush=True)
 
 
-def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample=16):
+def bench_floor(args):
+    """Launch floor of the headline executor's launch geometry (same grid and
+    workgroup size, cold-rotated like the headline): an empty kernel, and
+    kernels whose waves walk a chain of 1..4 dependent cold scalar loads (the
+    chain a cold executor wave walks before its first B load).  Graph-timed
+    like the headline; rocprofv3 --kernel-trace --stats of this command gives
+    the per-kernel durations."""
+    import ctypes as ct
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    wl = srt.workloads
+    case = wl.PRUNED_2048_512_S90_N512
+    A, B = wl.case_inputs(case, seed=0)
+    csr = torch.from_numpy(A).to_sparse_csr()
+    Bd = torch.from_numpy(B).cuda()
+    M_r, name, cfg, _ = choose_config(args, srt, csr, case.N, Bd)
+    plan = srt.MatMulSpecialized.from_csr(csr, case.N, mapping_id=srt.named_mapping(M_r, name), tile_config=cfg)
+    grid, block = ct.c_int(), ct.c_int()
+    srt._lib.call("srt_plan_launch_shape", plan._h, case.N, ct.byref(grid), ct.byref(block))
+    g, b = grid.value, block.value
+    waves = g * b // 64
+    depth_max = 4
+    region = waves * depth_max * 128 + 128        # bytes one chain launch touches
+    R = max(2, math.ceil(2 * MALL_BYTES / region))
+    pool = torch.zeros(R * region // 4 + 64, dtype=torch.int32, device="cuda")
+    sink = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def probe(kind, depth, i):
+        off = (i % R) * region
+        # the stream current at launch time: the capture stream inside capture()
+        return lambda: srt._lib.call("srt_probe_launch_floor", kind, g, b, ct.c_void_p(pool.data_ptr() + off),
+                                     region, depth, ct.c_void_p(sink.data_ptr()),
+                                     ct.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    rows = []
+    for kind, depth in [(0, 0), (1, 1), (1, 2), (1, 3), (1, 4)]:
+        gr = capture([probe(kind, depth, i) for i in range(args.steps)])
+        gr.replay()
+        torch.cuda.synchronize()
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        gr.replay()
+        e.record(stream)
+        torch.cuda.synchronize()
+        us = s.elapsed_time(e) * 1e3 / args.steps
+        rows.append({"kernel": "empty" if kind == 0 else "chain", "dependent_cold_loads": depth,
+                     "us_per_launch": round(us, 3)})
+        log(rows[-1])
+        del gr
+    assert int(sink.item()) == 0
+    out = {"form": "floor", "metric": "launch floor of the executor's geometry", "unit": "us", "n_gpus": 1,
+           "grid": g, "block": b, "waves": waves, "steps": args.steps, "replicas": R,
+           "config": {"workload": case.name, "M_r": M_r, "mapping": name}, "results": rows}
+    print(json.dumps(out), flush=True)
+
+
+def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample=16, form="regtile"):
     """48 resident BF16 weights (12 x {1024x1024 x2, 4096x1024, 1024x4096},
     90 % magnitude-pruned), Poisson requests with log-uniform N in [1, 512]:
     the continuous-batching server against one plain FP32 launch per request,
-    same arrivals, at the given offered loads and at saturation."""
+    same arrivals, at the given offered loads and at saturation.  form
+    "regtile": register-tiled sparse panels (BF16 values); "mfma": 16-row
+    dense panels on matrix cores."""
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")
     sv = srt.serving
     Ws = sv.serving_weights()
     shapes = [W.shape for W in Ws]
     t0 = time.perf_counter()
-    panels = [srt.precision.PanelPlan.from_dense(torch.from_numpy(W), "bf16") for W in Ws]
+    if form == "regtile":
+        panels = sv.RegTileWeights(Ws)
+        grid = 1024
+    else:
+        panels = [srt.precision.PanelPlan.from_dense(torch.from_numpy(W), "bf16") for W in Ws]
+        grid = 1024
     plans32 = []
     for W in Ws:
         csr = torch.from_numpy(W).to_sparse_csr()
         plans32.append(srt.MatMulSpecialized.from_csr(csr, 512, mapping_id=srt.named_mapping(4, "orig"),
                                                       tile_config=srt.TileConfig(xcd_remap=1)))
     log("packed 48 weights in %.1f s" % (time.perf_counter() - t0))
-    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, shape=mfma_shape, grid=1024)
+    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, shape=mfma_shape, grid=grid)
     levels = []
     for qps in qps_levels:
         reqs = sv.poisson_requests(requests, qps, shapes, seed=int(qps) % 1000 + 1)
@@ -411,10 +476,13 @@ def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample
     for r in sample:
         server.run_batch([r])
     errs = [po.rel_error_vs_fp64(r.C.cpu().numpy(), Ws[r.weight], r.B.float().cpu().numpy()) for r in sample]
-    # resident weight bytes: stored 16x32 blocks (BF16) against the nonzero values
-    resident = sum(p.stats()["stored_blocks"] * 512 * 2 for p in panels)
+    # resident weight bytes (what the served launches read) against the nonzero values
+    resident = server.resident_bytes()
     nnz = sum(int((W != 0).sum()) for W in Ws)
-    out = {"weights": "48 x {1024x1024 x2, 4096x1024, 1024x4096} per 12 layers, 90% magnitude-pruned",
+    out = {"served_form": {"regtile": "register-tiled sparse panels, BF16 values held as FP32, BF16 B widened "
+                                      "on the device, FP32 accumulation",
+                           "mfma": "16-row dense BF16 panels on MFMA"}[form],
+           "weights": "48 x {1024x1024 x2, 4096x1024, 1024x4096} per 12 layers, 90% magnitude-pruned",
            "requests_per_level": requests, "slo_ms": slo_ms,
            "max_qps_within_slo": {"served": max(ok_s) if ok_s else None, "plain_fp32": max(ok_p) if ok_p else None},
            "throughput_ratio_at_equal_p99": ratio,
@@ -422,7 +490,7 @@ def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample
            "saturation_ratio": sat["req_per_s"] / psat["req_per_s"],
            "p99_ms_at_slo_load": None, "served_rel_err_vs_fp64_max": max(errs),
            "weight_nnz": nnz, "weight_nnz_bytes_bf16": 2 * nnz, "resident_weight_bytes": resident,
-           "levels": levels}
+           "dense_weight_bytes_bf16": sum(W.size * 2 for W in Ws), "levels": levels}
     if ok_s and ok_p:
         at = min(max(ok_s), max(ok_p))
         lv = [l for l in levels if l["offered_qps"] == at][0]
@@ -432,11 +500,16 @@ def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample
 
 
 def bench_serving(args):
-    """The serving form's own JSON line (see serving_comparison)."""
-    res = serving_comparison([float(q) for q in args.qps.split(",")], args.requests, args.slo_ms, args.mfma_shape)
-    out = {"form": "serving", "metric": "served p99 latency vs plain; requests/s at p99 <= %.1f ms" % args.slo_ms,
-           "unit": "req/s", "n_gpus": 1, "data": "synthetic", "dtype": "bf16 (served) / f32 (plain)", **res}
-    print(json.dumps(out), flush=True)
+    """The serving form's own JSON lines (see serving_comparison): the
+    register-tiled sparse form, then the dense-panel MFMA form."""
+    for form in ("regtile", "mfma"):
+        res = serving_comparison([float(q) for q in args.qps.split(",")], args.requests, args.slo_ms,
+                                 args.mfma_shape, form=form)
+        out = {"form": "serving", "metric": "served p99 latency vs plain; requests/s at p99 <= %.1f ms" % args.slo_ms,
+               "unit": "req/s", "n_gpus": 1, "data": "synthetic", "dtype": "bf16 (served) / f32 (plain)", **res}
+        print(json.dumps(out), flush=True)
+        torch_mod = importlib.import_module("torch")
+        torch_mod.cuda.empty_cache()
 
 
 def bench_sweep(args):
@@ -669,6 +742,8 @@ def main():
         return bench_cnn(args)
     if args.form == "sweep":
         return bench_sweep(args)
+    if args.form == "floor":
+        return bench_floor(args)
     import numpy as np
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")
@@ -829,11 +904,11 @@ def main():
         torch.cuda.empty_cache()
         sv = serving_comparison([float(q) for q in args.serving_qps.split(",")], args.serving_requests,
                                 args.slo_ms, args.mfma_shape, accuracy_sample=8)
-        out["serving"] = {k: sv[k] for k in ("weights", "requests_per_level", "slo_ms", "max_qps_within_slo",
-                                             "throughput_ratio_at_equal_p99", "p99_ms_at_slo_load",
-                                             "saturation_req_per_s", "saturation_ratio",
+        out["serving"] = {k: sv[k] for k in ("served_form", "weights", "requests_per_level", "slo_ms",
+                                             "max_qps_within_slo", "throughput_ratio_at_equal_p99",
+                                             "p99_ms_at_slo_load", "saturation_req_per_s", "saturation_ratio",
                                              "served_rel_err_vs_fp64_max", "weight_nnz_bytes_bf16",
-                                             "resident_weight_bytes")}
+                                             "resident_weight_bytes", "dense_weight_bytes_bf16")}
         out["serving"]["p50_p99_ms"] = [{"offered_qps": l["offered_qps"],
                                          "served": [l["served"]["p50_ms"], l["served"]["p99_ms"]],
                                          "plain_fp32": [l["plain_fp32"]["p50_ms"], l["plain_fp32"]["p99_ms"]]}

@@ -68,11 +68,15 @@ typedef struct srt_tile_config {
                               the chain runs tile by tile as the reference's K-tiled
                               execution with load_c does; otherwise one tile of K      */
     int N_c;               /* columns per wavefront (64 * vec_width)                  */
-    int tiling_strategy;   /* reference enum: 0 manual, 1 CAKE, 2 CAKE+TLB (recorded) */
+    int tiling_strategy;   /* reference enum: 0 manual (K_c as given), 1 CAKE (K_c sized so
+                              a workgroup's B tile, K_c x 64*vec_width fp32, fits its LDS),
+                              2 CAKE + TLB compensation (then K_c = max_tlb_entries *
+                              tlb_page_size / (n * 4) when n x K_c fp32 needs more pages,
+                              SpMM_GECSB.h:339-348); realised at plan creation          */
     float beta;            /* CAKE tiling beta (recorded; not a BLAS beta)            */
     int sparse_a;          /* recorded                                                */
-    int max_tlb_entries;   /* recorded                                                */
-    int tlb_page_size;     /* recorded                                                */
+    int max_tlb_entries;   /* TLB compensation (tiling_strategy 2)                    */
+    int tlb_page_size;     /* TLB compensation (tiling_strategy 2), bytes             */
     int runtime_schedule;  /* reference outer schedule id (KNM=0 ...) (recorded)      */
     int vec_width;         /* V in {1,2,4}: fp32 columns per lane; 0 = choose        */
     int waves_per_block;   /* wave slots per workgroup, 1..16; a panel's waves share
@@ -177,10 +181,40 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
                     const char *executor_id, const srt_mapping *mapping, srt_stream_t stream,
                     srt_plan **out);
 
+/* A hybrid plan: the 16-row panels whose density (nonzeros / (rows * k))
+ * reaches dense_threshold are stored dense and run on FP32 matrix cores
+ * (v_mfma_f32_16x16x4_f32) after the register-tiled launch over the other
+ * panels; the rest is as srt_plan_create (K tiles, schedules, execute).  The
+ * reference marks a tile dense at 0.30 (sbench/padding_reordering.py:36) and
+ * pads patterns by its ILP cost model (sbench/ilp_pad/nano_solver.py:76-116);
+ * a threshold above 1 gives a plain plan.  Dense panels accumulate in the
+ * matrix core's order (k in steps of 4), within the FP32 summation bound,
+ * not bit-identical to the register-tiled chain.  Hybrid plans execute
+ * alone (not in groups or persistent tables). */
+int srt_plan_create_hybrid(int m, int k, int n, const float *Lx, const void *Lp, const void *Li, int index_bytes,
+                           const srt_tile_config *cfg, float dense_threshold, const srt_mapping *mapping,
+                           srt_stream_t stream, srt_plan **out);
+/* Dense panels of a hybrid plan: their count, first rows (num_dense entries,
+ * may be NULL) and device bytes. */
+int srt_plan_dense_panels(const srt_plan *p, int *num_dense, int32_t *panel_rows, size_t *bytes);
+
 int srt_plan_get_config(const srt_plan *p, srt_tile_config *out);
 
 /* Bytes of packed A held on the device (MatMulSpecialized::require_storage). */
 int srt_plan_required_storage(const srt_plan *p, size_t *bytes);
+
+/* Device bytes of the current launch schedule (wave descriptors and batch
+ * records), beside the packed data.  No counterpart in the reference. */
+int srt_plan_schedule_bytes(const srt_plan *p, size_t *bytes);
+
+/* Free the packed column indices: the batch records already carry every
+ * packed column's B row, so execution no longer reads them, and a resident
+ * weight (the serving form) keeps only values, records and descriptors.
+ * Afterwards the schedule cannot be rebuilt (srt_plan_allocate_executor with
+ * a width that needs another schedule fails) and srt_plan_copy_packed cannot
+ * return the columns.  Synchronises the device.  No counterpart in the
+ * reference. */
+int srt_plan_release_columns(srt_plan *p);
 
 /* Fix the launch shape for B widths up to n (MatMulSpecialized::allocate_executor). */
 int srt_plan_allocate_executor(srt_plan *p, int n);
@@ -206,8 +240,8 @@ int srt_plan_split_counts(const srt_plan *p, int32_t *kp);
 int srt_plan_launch_shape(const srt_plan *p, int n, int *grid, int *block);
 
 /* Launch-floor probe with the executor's geometry (measurement only): kind 0
- * an empty kernel; kind 1 every wave walking `depth` dependent scalar loads
- * through buf (buf_bytes >= 128, zero-filled; sink: one writable int). */
+ * an empty kernel; kind 1 every wave walking `depth` (1..4) dependent scalar
+ * loads through buf (buf_bytes >= 128, zero-filled; sink: one writable int). */
 int srt_probe_launch_floor(int kind, int grid, int block, const void *buf, int64_t buf_bytes, int depth,
                            void *sink, srt_stream_t stream);
 
@@ -401,6 +435,48 @@ typedef struct srt_bprep_problem {
 int srt_bprep_problem_tiles(const srt_bprep_problem *pr);
 int srt_mfma_grouped_b_prep(int dtype, const srt_bprep_problem *d_probs, const int *d_hdr, int grid,
                             srt_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Register-tile serving: many plans' executions in one persistent launch
+ * over a device problem table.  No counterpart in the reference (one
+ * MatMulSpecialized per call, SpMM_SOP.h:178).  The host fills one
+ * srt_exec_problem per SpMM (srt_exec_problem_fill), sets block_begin to the
+ * running sum of n_wblocks * ceil(n / (64 * vec_width)) in table order, and
+ * copies the table plus a header {number of problems, total workgroups} to
+ * device memory; the launch reads both from there, so a captured graph
+ * replays any batch written into the same buffers.  Every problem's result is
+ * bit-identical to srt_plan_execute of its plan at that lane width.  All
+ * plans of one table share the executor (mapping) and waves_per_block of
+ * `proto` and were built with unroll 4.
+ * ---------------------------------------------------------------------- */
+typedef struct srt_exec_problem {
+    const int32_t *prog;   /* plan's batch records   */
+    const int32_t *desc;   /* plan's wave descriptors */
+    const float *vals;     /* plan's packed values    */
+    const float *B;        /* K x ldb fp32            */
+    float *C;              /* M x ldc fp32            */
+    int64_t ldc;
+    uint32_t b_bytes, ldb;
+    int M, N, K, n_wblocks, reduce, load_c;
+    int block_begin;
+    int pad;
+} srt_exec_problem;
+
+/* One table record for plan p (B, C may be NULL with n = 0 for a template
+ * whose operand fields the caller fills in later; then it must keep
+ * b_bytes = K * ldb * 4 and the vec_width alignment rules: vec_width | n, ldb,
+ * ldc and 4 * vec_width-byte aligned B and C). */
+int srt_exec_problem_fill(const srt_plan *p, const float *B, int64_t ldb, float *C, int64_t ldc, int n,
+                          int vec_width, int block_begin, srt_exec_problem *out);
+int srt_exec_persistent_execute(const srt_plan *proto, int vec_width, const srt_exec_problem *d_probs,
+                                const int *d_hdr, int grid, srt_stream_t stream);
+
+/* Grouped BF16 -> FP32 widening of the requests' B operands (same table and
+ * header convention, srt_bprep_problem with ldk = the FP32 row stride,
+ * columns [N, ldk) written as zero; src_bf16 must be 1).  A problem's tiles
+ * are 1024-element pieces of its K x ldk output: ceil(K * ldk / 1024). */
+int srt_widen_problem_tiles(const srt_bprep_problem *pr);
+int srt_exec_grouped_b_widen(const srt_bprep_problem *d_probs, const int *d_hdr, int grid, srt_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * CSR baseline (sbench/SOP/_C/pytorch_csr_wrapper.cpp:243-312 _csr_kernel,

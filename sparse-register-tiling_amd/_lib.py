@@ -85,6 +85,9 @@ SIGNATURES = {
     "srt_executor_describe": (_i, [_i, ct.POINTER(_i), ct.POINTER(_i), _p_i32, ct.c_char_p, ct.c_size_t]),
     "srt_plan_create": (_i, [_i, _i, _i, _vp, _vp, _vp, _i, ct.POINTER(TileConfigC), _i,
                              ct.c_char_p, _vp, _vp, ct.POINTER(_vp)]),
+    "srt_plan_create_hybrid": (_i, [_i, _i, _i, _vp, _vp, _vp, _i, ct.POINTER(TileConfigC), ct.c_float, _vp, _vp,
+                                    ct.POINTER(_vp)]),
+    "srt_plan_dense_panels": (_i, [_vp, ct.POINTER(_i), _vp, ct.POINTER(ct.c_size_t)]),
     "srt_plan_get_config": (_i, [_vp, ct.POINTER(TileConfigC)]),
     "srt_plan_required_storage": (_i, [_vp, ct.POINTER(ct.c_size_t)]),
     "srt_plan_allocate_executor": (_i, [_vp, _i]),
@@ -97,6 +100,8 @@ SIGNATURES = {
     "srt_group_execute": (_i, [_vp, _vp]),
     "srt_group_info": (_i, [_vp, _vp, _vp, _vp]),
     "srt_group_destroy": (None, [_vp]),
+    "srt_plan_release_columns": (_i, [_vp]),
+    "srt_plan_schedule_bytes": (_i, [_vp, ct.POINTER(ct.c_size_t)]),
     "srt_plan_launch_shape": (_i, [_vp, _i, ct.POINTER(_i), ct.POINTER(_i)]),
     "srt_probe_launch_floor": (_i, [_i, _i, _i, _vp, _i64, _i, _vp, _vp]),
     "srt_plan_split_counts": (_i, [_vp, _vp]),
@@ -111,6 +116,10 @@ SIGNATURES = {
     "srt_tile_execute": (_i, [_vp, _i, _vp, _i64, _vp, _i64, _i, ct.POINTER(TileConfigC), _vp]),
     "srt_tile_copy_packed": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _p_i64, _p_i64]),
     "srt_tile_destroy": (None, [_vp]),
+    "srt_exec_problem_fill": (_i, [_vp, _vp, _i64, _vp, _i64, _i, _i, _i, _vp]),
+    "srt_exec_persistent_execute": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
+    "srt_widen_problem_tiles": (_i, [_vp]),
+    "srt_exec_grouped_b_widen": (_i, [_vp, _vp, _i, _vp]),
     "srt_csr_spmm": (_i, [_i, _i, _i, _i64, _vp, _vp, _i, _vp, _vp, _i64, _vp, _i64, _i, _vp]),
     "srt_mfma_plan_create": (_i, [_i, _i, _vp, _vp, _vp, _i, _i, _vp, ct.POINTER(_vp)]),
     "srt_mfma_plan_reserve": (_i, [_vp, _i, _i]),

@@ -98,6 +98,10 @@ def _grouped_name(eid: str, v: int) -> str:
     return "srt_grouped_%s_v%d" % (eid, v)
 
 
+def _persistent_name(eid: str, v: int) -> str:
+    return "srt_persistent_%s_v%d" % (eid, v)
+
+
 def _launcher_name(eid: str, stage: int, v: int, u: int) -> str:
     return "srt_launch_%s_s%d_v%d_u%d" % (eid, stage, v, u)
 
@@ -127,6 +131,7 @@ def gen_executor_sources(M_r: int, patterns: Sequence[int], lean: bool = False) 
     decls = []
     rows = []
     grouped = {}
+    persistent = {}
     for stage in ((0,) if lean else (0, 1)):
         vrows = []
         for v in VEC_WIDTHS:
@@ -141,13 +146,21 @@ def gen_executor_sources(M_r: int, patterns: Sequence[int], lean: bool = False) 
                             % (name, v, u, "true" if stage else "false"))
                     if stage == 0 and u == GROUPED_UNROLL:
                         gname = _grouped_name(eid, v)
+                        pname = _persistent_name(eid, v)
                         decls.append("hipError_t %s(const srt::ExecProblem *, int, int, int, size_t, int, "
                                      "hipStream_t);" % gname)
+                        decls.append("hipError_t %s(const srt::ExecProblem *, const int *, int, int, size_t, "
+                                     "hipStream_t);" % pname)
                         grouped[v] = "&" + gname
+                        persistent[v] = "&" + pname
                         body += ("\n\n__attribute__((visibility(\"hidden\"))) hipError_t %s(const srt::ExecProblem *p, "
                                  "int np, int nb, int w, size_t lds, int x, hipStream_t s) {\n    return "
                                  "srt::launch_exec_grouped<MR, %d, %d, false, Dispatch>(p, np, nb, w, lds, x, s);\n}"
                                  % (gname, v, u))
+                        body += ("\n\n__attribute__((visibility(\"hidden\"))) hipError_t %s(const srt::ExecProblem *p, "
+                                 "const int *h, int g, int w, size_t lds, hipStream_t s) {\n    return "
+                                 "srt::launch_exec_persistent<MR, %d, %d, false, Dispatch>(p, h, g, w, lds, s);\n}"
+                                 % (pname, v, u))
                     files["exec_%s__s%dv%du%d.hip" % (eid, stage, v, u)] = (
                         head + _dispatch_text(M_r, pats, u) + "\n" + body + "\n")
                 else:
@@ -167,7 +180,8 @@ const srt::ExecutorDesc k_desc = {{
     {{
 {rows}
     }},
-    {{{grouped}}}}};
+    {{{grouped}}},
+    {{{persistent}}}}};
 
 struct Registrar {{
     Registrar() {{ srt::register_executor(&k_desc); }}
@@ -176,7 +190,8 @@ struct Registrar {{
 }}  // namespace
 """.format(decls="\n".join('__attribute__((visibility("hidden"))) ' + d for d in decls), eid=eid, M_r=M_r,
            n=len(pats), pat_list=", ".join(str(p) for p in pats), rows=",\n".join(rows),
-           grouped=", ".join(grouped.get(v, "nullptr") for v in VEC_WIDTHS))
+           grouped=", ".join(grouped.get(v, "nullptr") for v in VEC_WIDTHS),
+           persistent=", ".join(persistent.get(v, "nullptr") for v in VEC_WIDTHS))
     return files
 
 

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -72,8 +73,8 @@ void Packed::release() {
 }
 
 size_t Packed::bytes() const {
-    return sizeof(int32_t) * (size_t)P * NK + 3 * sizeof(int64_t) * (size_t)(P + 1) +
-           sizeof(int32_t) * (size_t)total_cols + sizeof(float) * (size_t)total_vals +
+    return sizeof(int32_t) * (size_t)P * KT * NK + 3 * sizeof(int64_t) * (size_t)(P + 1) +
+           (cols ? sizeof(int32_t) * (size_t)total_cols : 0) + sizeof(float) * (size_t)total_vals +
            2 * sizeof(int32_t) * (size_t)total_groups + sizeof(int32_t) * (size_t)NK;
 }
 
@@ -336,6 +337,23 @@ struct srt_mapping {
 };
 
 // ------------------------------------------------------------------- plans
+// Dense 16-row panels of a hybrid plan (srt_plan_create_hybrid), run on FP32
+// matrix cores after the register-tiled part (dense_panels.hip).
+struct DensePart {
+    int ndense = 0, G = 0;          // dense panels, 16-column K groups
+    void *apack = nullptr;          // [ndense][G][64] float4, MFMA A-operand order
+    int32_t *panel_row = nullptr;   // first row of each dense panel
+    std::vector<int32_t> h_panel_row;
+    size_t bytes = 0;
+    void release() {
+        (void)hipFree(apack);
+        (void)hipFree(panel_row);
+        apack = nullptr;
+        panel_row = nullptr;
+        ndense = 0;
+    }
+};
+
 struct srt_plan {
     Packed pk;
     Schedule sch;
@@ -346,6 +364,7 @@ struct srt_plan {
     int n_alloc = 0;
     int device = 0;
     std::vector<RetiredSchedule> retired;  // schedules of earlier widths, alive until destroy
+    DensePart dense;                       // hybrid plans: panels on matrix cores
 };
 
 struct PanelBuf {
@@ -821,6 +840,8 @@ static void balanced_slots(const Packed &pk, Schedule &sch, int n_tiles, std::ve
 static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const ExecutorDesc *exec, int n_tiles,
                           Schedule &out, std::vector<RetiredSchedule> &retired, const srt_tile_config &out_cfg,
                           hipStream_t s) {
+    if (!pk.cols && pk.total_cols)
+        return fail(SRT_ERR_INVALID, "the plan's column indices were released: its schedule cannot change");
     Schedule sch;
     sch.unroll = schedule_unroll(exec, pow2_upto8(cfg.unroll) ? cfg.unroll : 4);
     sch.waves_per_block = cfg.waves_per_block;
@@ -860,8 +881,16 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
     }
     const int nslots = (int)(slots.size() / 3);
     sch.n_wblocks = nslots / std::max(1, sch.waves_per_block);
-    // Record run of each slot: its chunk's columns plus a terminator, runs
-    // laid out in slot order.
+    // Record run of each slot: one record per batch of its chunk plus a
+    // terminator, runs laid out in slot order.  A chunk's batches are its
+    // overlap with each (K tile, id) group cut into U-column pieces
+    // (wave_prog_kernel), counted here from the per-panel counts.
+    std::vector<int32_t> hc((size_t)pk.P * pk.KT * pk.NK);
+    if (!hc.empty()) {
+        SRT_HIP_TRY(hipMemcpyAsync(hc.data(), pk.counts, sizeof(int32_t) * hc.size(), hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipStreamSynchronize(s));
+    }
+    const int64_t U = sch.unroll;
     std::vector<int64_t> rec_start((size_t)std::max(nslots, 1), 0);
     int64_t nrun = 0;
     for (int i = 0; i < nslots; ++i) {
@@ -869,7 +898,14 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
         const int p = slots[3 * i], j = slots[3 * i + 1], kp = slots[3 * i + 2];
         if (p >= 0) {
             const int64_t nc = pk.h_col_ptr[p + 1] - pk.h_col_ptr[p];
-            nrun += (int64_t)(j + 1) * nc / kp - (int64_t)j * nc / kp;
+            const int64_t c0 = (int64_t)j * nc / kp, c1 = (int64_t)(j + 1) * nc / kp;
+            const int32_t *cp = hc.data() + (size_t)p * pk.KT * pk.NK;
+            int64_t pos = 0;
+            for (int64_t g = 0; g < (int64_t)pk.KT * pk.NK && pos < c1; ++g) {
+                const int64_t lo = std::max(pos, c0), hi = std::min(pos + cp[g], c1);
+                if (hi > lo) nrun += (hi - lo + U - 1) / U;
+                pos += cp[g];
+            }
         }
         nrun += 1;
     }
@@ -885,8 +921,8 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
         SRT_HIP_TRY(launch_wave_desc(nslots, d_slots, pk.groups, pk.grp_ptr, pk.col_ptr,
                                      pk.id_pattern, sch.desc, s));
     }
-    // Batch records: at most one per packed column of each chunk plus a
-    // terminator per slot (nrun in all), and two records of look-ahead padding.
+    // Batch records: one per batch of each chunk plus a terminator per slot
+    // (nrun in all), and two records of look-ahead padding.
     if (pk.total_vals + SRT_VAL_PAD > (int64_t)0x7fffffffLL)
         return fail(SRT_ERR_UNSUPPORTED, "more than 2^31 packed values");
     const size_t nrec = (size_t)nrun + 2;
@@ -894,6 +930,7 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
     if (nrec * RS * sizeof(int32_t) > (size_t)0xffffffffULL)
         return fail(SRT_ERR_UNSUPPORTED, "batch records exceed 4 GiB");
     SRT_TRY(dmalloc(&sch.prog, nrec * RS));
+    sch.bytes = sizeof(int32_t) * (nrec * RS + 16 * (size_t)std::max(nslots, 1));
     SRT_HIP_TRY(hipMemsetAsync(sch.prog, 0, sizeof(int32_t) * nrec * RS, s));
     if (nslots) {
         SRT_HIP_TRY(hipMemcpyAsync(d_start, rec_start.data(), sizeof(int64_t) * nslots, hipMemcpyHostToDevice, s));
@@ -943,6 +980,32 @@ static int alloc_packed_arrays(Packed &pk, hipStream_t s) {
     SRT_HIP_TRY(hipMemsetAsync(pk.cols + pk.total_cols, 0, sizeof(int32_t) * SRT_COL_PAD, s));
     SRT_HIP_TRY(hipMemsetAsync(pk.vals + pk.total_vals, 0, sizeof(float) * SRT_VAL_PAD, s));
     return SRT_OK;
+}
+
+// K tile of the packing for the tiling strategy (the reference's enum,
+// cpp_testbed/src/spmm/GECSB/SpMM_GECSB.h:21-25):
+//   0 manual  - K_c as given (0 = one tile);
+//   1 CAKE    - CAKE sizes the K block so one core's B block stays in its
+//               cache (get_cache_dims_4, SpMM_GECSB.h:316-335; the CAKE
+//               library itself is external); here the block is a
+//               workgroup's B tile, K_c rows x 64 * vec_width columns, held
+//               to the LDS of one workgroup (a multiple of 32 rows);
+//   2 CAKE + TLB compensation - as 1, then K_c = max_tlb_entries *
+//               tlb_page_size / (n * 4) whenever the B block n x K_c fp32
+//               needs more than max_tlb_entries pages (SpMM_GECSB.h:339-348).
+static int tiled_k(const srt_tile_config &c, int k, int n) {
+    int K_c = c.K_c;
+    if (c.tiling_strategy == 1 || c.tiling_strategy == 2) {
+        const int V = (c.vec_width == 1 || c.vec_width == 2 || c.vec_width == 4) ? c.vec_width : 4;
+        K_c = (int)std::max<size_t>(32, lds_limit() / ((size_t)64 * V * sizeof(float)) / 32 * 32);
+    }
+    if (c.tiling_strategy == 2 && n > 0 && c.tlb_page_size > 0 && c.max_tlb_entries > 0) {
+        const int kc = (K_c >= 1 && K_c < k) ? K_c : k;
+        const int64_t bc_bytes = (int64_t)n * kc * (int64_t)sizeof(float);
+        if (bc_bytes / c.tlb_page_size > c.max_tlb_entries)
+            K_c = (int)std::max<int64_t>(1, (int64_t)c.max_tlb_entries * c.tlb_page_size / ((int64_t)n * sizeof(float)));
+    }
+    return (K_c >= 1 && K_c < k) ? K_c : 0;
 }
 
 // Packs an m x k CSR matrix (device arrays) into pk.  Mirrors the packing
@@ -1115,7 +1178,7 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
         return fail(SRT_ERR_INVALID, "executor and mapping pattern sets differ");
     auto plan = std::make_unique<srt_plan>();
     SRT_HIP_TRY(hipGetDevice(&plan->device));
-    const int K_c = cfg ? cfg->K_c : 0;
+    const int K_c = cfg ? tiled_k(*cfg, k, n) : 0;
     int rc = pack_csr(plan->pk, map, m, k, K_c, Lx, Lp, Li, index_bytes, (hipStream_t)stream);
     if (rc) {
         plan->pk.release();
@@ -1134,6 +1197,126 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
         return rc;
     }
     *out = plan.release();
+    return SRT_OK;
+}
+
+int srt_plan_create_hybrid(int m, int k, int n, const float *Lx, const void *Lp, const void *Li, int index_bytes,
+                           const srt_tile_config *cfg, float dense_threshold, const srt_mapping *mapping,
+                           srt_stream_t stream, srt_plan **out) {
+    if (!out || !mapping || !Lp) return fail(SRT_ERR_INVALID, "srt_plan_create_hybrid: null argument");
+    *out = nullptr;
+    if (m < 0 || k < 0 || n < 0) return fail(SRT_ERR_INVALID, "negative dimension");
+    if (index_bytes != 4 && index_bytes != 8) return fail(SRT_ERR_INVALID, "index_bytes must be 4 or 8");
+    if (!(dense_threshold >= 0.0f)) return fail(SRT_ERR_INVALID, "dense_threshold must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    // The CSR on the host: row pointer (checked as pack_csr checks it), then
+    // the entries.
+    std::vector<int64_t> rp((size_t)m + 1);
+    if (index_bytes == 8) {
+        SRT_HIP_TRY(hipMemcpyAsync(rp.data(), Lp, 8 * ((size_t)m + 1), hipMemcpyDeviceToHost, s));
+    } else {
+        std::vector<int32_t> r32((size_t)m + 1);
+        SRT_HIP_TRY(hipMemcpyAsync(r32.data(), Lp, 4 * ((size_t)m + 1), hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipStreamSynchronize(s));
+        std::copy(r32.begin(), r32.end(), rp.begin());
+    }
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    if (rp[0] < 0) return fail(SRT_ERR_INVALID, "CSR row pointer starts below zero");
+    for (int i = 0; i < m; ++i)
+        if (rp[i + 1] < rp[i]) return fail(SRT_ERR_INVALID, "CSR row pointer decreases at row " + std::to_string(i));
+    const int64_t base = rp[0], nnz = rp[m] - rp[0];
+    if (nnz > 0 && (!Lx || !Li)) return fail(SRT_ERR_INVALID, "null CSR value or index array");
+    // Dense panels: 16-row panels whose density reaches the threshold.
+    const int P16 = (m + 15) / 16;
+    std::vector<int32_t> dense_rows;
+    std::vector<char> is_dense((size_t)P16, 0);
+    for (int p = 0; p < P16 && k > 0; ++p) {
+        const int r0 = 16 * p, r1 = std::min(m, r0 + 16);
+        const double density = (double)(rp[r1] - rp[r0]) / ((double)(r1 - r0) * k);
+        if (density >= dense_threshold && rp[r1] > rp[r0]) {
+            is_dense[p] = 1;
+            dense_rows.push_back(r0);
+        }
+    }
+    if (dense_rows.empty())
+        return srt_plan_create(m, k, n, Lx, Lp, Li, index_bytes, cfg, 1, nullptr, mapping, stream, out);
+    std::vector<int64_t> ci((size_t)std::max<int64_t>(nnz, 1));
+    std::vector<float> cv((size_t)std::max<int64_t>(nnz, 1));
+    if (nnz > 0) {
+        if (index_bytes == 8) {
+            SRT_HIP_TRY(hipMemcpyAsync(ci.data(), (const int64_t *)Li + base, 8 * (size_t)nnz, hipMemcpyDeviceToHost, s));
+        } else {
+            std::vector<int32_t> c32((size_t)nnz);
+            SRT_HIP_TRY(hipMemcpyAsync(c32.data(), (const int32_t *)Li + base, 4 * (size_t)nnz, hipMemcpyDeviceToHost, s));
+            SRT_HIP_TRY(hipStreamSynchronize(s));
+            std::copy(c32.begin(), c32.end(), ci.begin());
+        }
+        SRT_HIP_TRY(hipMemcpyAsync(cv.data(), Lx + base, 4 * (size_t)nnz, hipMemcpyDeviceToHost, s));
+        SRT_HIP_TRY(hipStreamSynchronize(s));
+    }
+    // The register-tiled part: the CSR with the dense panels' rows emptied.
+    std::vector<int64_t> rp2((size_t)m + 1, 0), ci2;
+    std::vector<float> cv2;
+    for (int i = 0; i < m; ++i) {
+        if (!is_dense[i / 16])
+            for (int64_t q = rp[i] - base; q < rp[i + 1] - base; ++q) {
+                ci2.push_back(ci[q]);
+                cv2.push_back(cv[q]);
+            }
+        rp2[i + 1] = (int64_t)ci2.size();
+    }
+    // The dense part in MFMA A-operand order: A[row = l % 16][k = 16 g + 4 j + l / 16]
+    // at ((d * G + g) * 64 + l) * 4 + j.
+    const int nd = (int)dense_rows.size(), G = (k + 15) / 16;
+    std::vector<float> ap((size_t)nd * G * 64 * 4, 0.0f);
+    std::vector<char> seen(ap.size(), 0);
+    for (int d = 0; d < nd; ++d)
+        for (int r = 0; r < 16 && dense_rows[d] + r < m; ++r) {
+            const int row = dense_rows[d] + r;
+            for (int64_t q = rp[row] - base; q < rp[row + 1] - base; ++q) {
+                const int64_t col = ci[q];
+                if (col < 0 || col >= k) return fail(SRT_ERR_INVALID, "CSR column index out of range [0, k)");
+                const int g = (int)(col / 16), w = (int)(col % 16);
+                const size_t at = (((size_t)d * G + g) * 64 + (size_t)(r + 16 * (w % 4))) * 4 + (size_t)(w / 4);
+                if (seen[at]) return fail(SRT_ERR_INVALID, "CSR holds an entry twice (same row and column)");
+                seen[at] = 1;
+                ap[at] = cv[q];
+            }
+        }
+    int64_t *d_rp = nullptr, *d_ci = nullptr;
+    float *d_cv = nullptr;
+    Scratch sc;
+    SRT_TRY(sc.get(&d_rp, rp2.size()));
+    SRT_TRY(sc.get(&d_ci, std::max<size_t>(ci2.size(), 1)));
+    SRT_TRY(sc.get(&d_cv, std::max<size_t>(cv2.size(), 1)));
+    SRT_HIP_TRY(hipMemcpyAsync(d_rp, rp2.data(), 8 * rp2.size(), hipMemcpyHostToDevice, s));
+    if (!ci2.empty()) {
+        SRT_HIP_TRY(hipMemcpyAsync(d_ci, ci2.data(), 8 * ci2.size(), hipMemcpyHostToDevice, s));
+        SRT_HIP_TRY(hipMemcpyAsync(d_cv, cv2.data(), 4 * cv2.size(), hipMemcpyHostToDevice, s));
+    }
+    srt_plan *plan = nullptr;
+    SRT_TRY(srt_plan_create(m, k, n, d_cv, d_rp, d_ci, 8, cfg, 1, nullptr, mapping, stream, &plan));
+    std::unique_ptr<srt_plan, void (*)(srt_plan *)> guard(plan, srt_plan_destroy);
+    DensePart &dp = plan->dense;
+    dp.ndense = nd;
+    dp.G = G;
+    dp.h_panel_row = dense_rows;
+    dp.bytes = sizeof(float) * ap.size() + sizeof(int32_t) * (size_t)nd;
+    SRT_HIP_TRY(hipMalloc(&dp.apack, sizeof(float) * ap.size()));
+    SRT_HIP_TRY(hipMalloc((void **)&dp.panel_row, sizeof(int32_t) * (size_t)nd));
+    SRT_HIP_TRY(hipMemcpyAsync(dp.apack, ap.data(), sizeof(float) * ap.size(), hipMemcpyHostToDevice, s));
+    SRT_HIP_TRY(hipMemcpyAsync(dp.panel_row, dense_rows.data(), sizeof(int32_t) * (size_t)nd, hipMemcpyHostToDevice, s));
+    SRT_HIP_TRY(hipStreamSynchronize(s));
+    plan->pk.nnz = nnz;  // the whole matrix's entries (the register-tiled part holds the others)
+    *out = guard.release();
+    return SRT_OK;
+}
+
+int srt_plan_dense_panels(const srt_plan *p, int *num_dense, int32_t *panel_rows, size_t *bytes) {
+    if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    if (num_dense) *num_dense = p->dense.ndense;
+    if (panel_rows) std::copy(p->dense.h_panel_row.begin(), p->dense.h_panel_row.end(), panel_rows);
+    if (bytes) *bytes = p->dense.bytes;
     return SRT_OK;
 }
 
@@ -1172,7 +1355,25 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
     int dev = -1;
     SRT_HIP_TRY(hipGetDevice(&dev));
     if (dev != p->device) return fail(SRT_ERR_INVALID, "plan belongs to another device");
-    return run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream);
+    SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream));
+    if (p->dense.ndense && n > 0)  // the dense panels' rows, zero from the register-tiled part, overwritten
+        SRT_HIP_TRY(launch_dense_panels(p->dense.ndense, p->dense.G, p->dense.apack, p->dense.panel_row, B, ldb, C,
+                                        ldc, p->pk.M, n, p->pk.K, (hipStream_t)stream));
+    return SRT_OK;
+}
+
+int srt_plan_release_columns(srt_plan *p) {
+    if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    SRT_HIP_TRY(hipDeviceSynchronize());  // no launch that builds a schedule may still read them
+    (void)hipFree(p->pk.cols);
+    p->pk.cols = nullptr;
+    return SRT_OK;
+}
+
+int srt_plan_schedule_bytes(const srt_plan *p, size_t *bytes) {
+    if (!p || !bytes) return fail(SRT_ERR_INVALID, "null argument");
+    *bytes = p->sch.bytes;
+    return SRT_OK;
 }
 
 int srt_plan_launch_shape(const srt_plan *p, int n, int *grid, int *block) {
@@ -1212,8 +1413,10 @@ static int copy_packed(const Packed &pk, int32_t *counts, int64_t *col_ptr, int6
         SRT_HIP_TRY(hipMemcpy(counts, pk.counts, sizeof(int32_t) * pk.P * pk.KT * pk.NK, hipMemcpyDeviceToHost));
     if (col_ptr) SRT_HIP_TRY(hipMemcpy(col_ptr, pk.col_ptr, sizeof(int64_t) * (pk.P + 1), hipMemcpyDeviceToHost));
     if (val_ptr) SRT_HIP_TRY(hipMemcpy(val_ptr, pk.val_ptr, sizeof(int64_t) * (pk.P + 1), hipMemcpyDeviceToHost));
-    if (cols && pk.total_cols)
+    if (cols && pk.total_cols) {
+        if (!pk.cols) return fail(SRT_ERR_INVALID, "the plan's column indices were released");
         SRT_HIP_TRY(hipMemcpy(cols, pk.cols, sizeof(int32_t) * pk.total_cols, hipMemcpyDeviceToHost));
+    }
     if (vals && pk.total_vals)
         SRT_HIP_TRY(hipMemcpy(vals, pk.vals, sizeof(float) * pk.total_vals, hipMemcpyDeviceToHost));
     return SRT_OK;
@@ -1230,6 +1433,7 @@ void srt_plan_destroy(srt_plan *p) {
     p->pk.release();
     p->sch.release();
     for (RetiredSchedule &r : p->retired) r.sch.release();
+    p->dense.release();
     delete p;
 }
 
@@ -1327,6 +1531,7 @@ int srt_group_create(const srt_plan *const *plans, int count, const float *const
     for (int i = 0; i < count; ++i) {
         const srt_plan *p = plans[i];
         if (p->device != g->device) return fail(SRT_ERR_INVALID, "srt_group_create: plan belongs to another device");
+        if (p->dense.ndense) return fail(SRT_ERR_UNSUPPORTED, "srt_group_create: hybrid plans execute alone");
         if (i == 0) {
             g->exec = p->exec;
             g->waves = p->sch.waves_per_block;
@@ -1366,6 +1571,71 @@ int srt_group_execute(const srt_group *g, srt_stream_t stream) {
     if (!fn) return fail(SRT_ERR_UNSUPPORTED, "grouped executor variant not generated");
     SRT_HIP_TRY(fn(g->d_probs, (int)g->h_probs.size(), g->total_blocks, g->waves, g->lds, g->xcd_remap,
                    (hipStream_t)stream));
+    return SRT_OK;
+}
+
+// ---------------------------------------------------- persistent problem table
+static_assert(sizeof(srt_exec_problem) == sizeof(ExecProblem), "srt_exec_problem mirrors ExecProblem");
+static_assert(offsetof(srt_exec_problem, ldc) == offsetof(ExecProblem, ldc), "srt_exec_problem layout");
+static_assert(offsetof(srt_exec_problem, block_begin) == offsetof(ExecProblem, block_begin),
+              "srt_exec_problem layout");
+
+// The persistent launchers are generated at this unroll (codegen GROUPED_UNROLL).
+static constexpr int kPersistentUnroll = 4;
+
+int srt_exec_problem_fill(const srt_plan *p, const float *B, int64_t ldb, float *C, int64_t ldc, int n,
+                          int vec_width, int block_begin, srt_exec_problem *out) {
+    if (!p || !out) return fail(SRT_ERR_INVALID, "srt_exec_problem_fill: null argument");
+    const int V = vec_width;
+    if (V != 1 && V != 2 && V != 4) return fail(SRT_ERR_INVALID, "vec_width must be 1, 2 or 4");
+    if (n < 0 || ldb < n || ldc < n || block_begin < 0) return fail(SRT_ERR_INVALID, "bad n / ldb / ldc / block_begin");
+    if (p->sch.unroll != kPersistentUnroll)
+        return fail(SRT_ERR_UNSUPPORTED, "the plan's schedule is not cut for unroll 4 (TileConfig.unroll = 4)");
+    if (p->dense.ndense) return fail(SRT_ERR_UNSUPPORTED, "hybrid plans execute alone");
+    if (n > 0 && (!B || !C)) return fail(SRT_ERR_INVALID, "null B or C");
+    if (n % V || ldb % V || ldc % V || !aligned(B, 4 * V) || !aligned(C, 4 * V))
+        return fail(SRT_ERR_INVALID, "vec_width must divide n, ldb, ldc and the operands be 4 * vec_width aligned");
+    if ((int64_t)p->pk.K * ldb * (int64_t)sizeof(float) > (int64_t)0xffffffffLL)
+        return fail(SRT_ERR_UNSUPPORTED, "B larger than 4 GiB (K * ldb * 4 bytes)");
+    if (p->sch.reduce && !lds_fits(false, p->pk.K, p->sch.waves_per_block, p->pk.M_r, V, 1))
+        return fail(SRT_ERR_UNSUPPORTED, "chunk reduction needs more LDS than a workgroup has");
+    ExecProblem q{};
+    q.prog = p->sch.prog;
+    q.desc = p->sch.desc;
+    q.vals = p->pk.vals;
+    q.B = B;
+    q.C = C;
+    q.ldc = ldc;
+    q.b_bytes = (uint32_t)std::max<int64_t>((int64_t)p->pk.K * ldb * (int64_t)sizeof(float), 4);
+    q.ldb = (uint32_t)ldb;
+    q.M = p->pk.M;
+    q.N = n;
+    q.K = p->pk.K;
+    q.n_wblocks = p->sch.n_wblocks;
+    q.reduce = p->sch.reduce;
+    q.load_c = 0;
+    q.block_begin = block_begin;
+    std::memcpy(out, &q, sizeof(q));
+    return SRT_OK;
+}
+
+int srt_exec_persistent_execute(const srt_plan *proto, int vec_width, const srt_exec_problem *d_probs,
+                                const int *d_hdr, int grid, srt_stream_t stream) {
+    if (!proto || !d_probs || !d_hdr) return fail(SRT_ERR_INVALID, "srt_exec_persistent_execute: null argument");
+    const int V = vec_width;
+    if (V != 1 && V != 2 && V != 4) return fail(SRT_ERR_INVALID, "vec_width must be 1, 2 or 4");
+    if (grid < 1) return fail(SRT_ERR_INVALID, "grid must be >= 1");
+    if (proto->sch.unroll != kPersistentUnroll)
+        return fail(SRT_ERR_UNSUPPORTED, "the plan's schedule is not cut for unroll 4 (TileConfig.unroll = 4)");
+    int dev = -1;
+    SRT_HIP_TRY(hipGetDevice(&dev));
+    if (dev != proto->device) return fail(SRT_ERR_INVALID, "plan belongs to another device");
+    ExecPersistentFn fn = proto->exec->persistent[log2_small(V)];
+    if (!fn) return fail(SRT_ERR_UNSUPPORTED, "persistent executor variant not generated");
+    const int waves = proto->sch.waves_per_block;
+    const size_t lds = reduce_lds(waves, proto->pk.M_r, V);
+    if (lds > lds_limit()) return fail(SRT_ERR_UNSUPPORTED, "chunk reduction needs more LDS than a workgroup has");
+    SRT_HIP_TRY(fn(reinterpret_cast<const ExecProblem *>(d_probs), d_hdr, grid, waves, lds, (hipStream_t)stream));
     return SRT_OK;
 }
 

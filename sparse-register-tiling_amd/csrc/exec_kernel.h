@@ -456,6 +456,29 @@ __global__ void __launch_bounds__(1024)
                                   probs[lo].load_c, b - probs[lo].block_begin);
 }
 
+// Grouped and persistent: the problem count and the logical workgroup total
+// are read from device memory (hdr[0], hdr[1]) next to the problem table, and
+// a fixed grid walks logical workgroups blockIdx.x, blockIdx.x + gridDim.x, ...
+// so one captured graph replays any batch written into the same buffers (the
+// serving form).  Each logical workgroup runs exec_body exactly as the
+// one-launch-per-problem kernel does, so every problem's result is the same
+// bits alone or in any batch.
+template <int MR, int V, int U, bool STAGE, class D>
+__global__ void __launch_bounds__(1024)
+    exec_persistent_kernel(ConstProblems probs, const int *__restrict__ hdr) {
+    extern __shared__ float lds[];
+    const int nprob = hdr[0], total = hdr[1];
+    int q = 0;
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        while (q + 1 < nprob && probs[q + 1].block_begin <= b) ++q;  // b ascends per workgroup
+        exec_body<MR, V, U, STAGE, D>(lds, probs[q].prog, reinterpret_cast<const int4 *>(probs[q].desc),
+                                      probs[q].vals, probs[q].B, probs[q].b_bytes, probs[q].ldb, probs[q].C,
+                                      probs[q].ldc, probs[q].M, probs[q].N, probs[q].K, probs[q].n_wblocks,
+                                      probs[q].reduce, probs[q].load_c, b - probs[q].block_begin);
+        __syncthreads();  // the next logical workgroup reuses the LDS
+    }
+}
+
 // LDS bytes of the chunk reduction: one partial tile per wave slot.
 inline size_t reduce_lds_bytes(int MR, int V, int waves, int reduce) {
     return reduce ? (size_t)waves * MR * V * 64 * sizeof(float) : 0;
@@ -503,6 +526,22 @@ hipError_t launch_exec_grouped(const ExecProblem *d_probs, int nprob, int total_
     }
     hipLaunchKernelGGL((exec_grouped_kernel<MR, V, U, STAGE, D>), dim3((unsigned)total_blocks), dim3(64 * waves),
                        lds, s, (ConstProblems)d_probs, nprob, xcd_remap);
+    return hipGetLastError();
+}
+
+template <int MR, int V, int U, bool STAGE, class D>
+hipError_t launch_exec_persistent(const ExecProblem *d_probs, const int *d_hdr, int grid, int waves, size_t lds,
+                                  hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
+    if (waves < 1 || waves > 16) return hipErrorInvalidConfiguration;
+    if (lds > 65536) {
+        static const hipError_t raised =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(&exec_persistent_kernel<MR, V, U, STAGE, D>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit());
+        if (raised != hipSuccess) return raised;
+    }
+    hipLaunchKernelGGL((exec_persistent_kernel<MR, V, U, STAGE, D>), dim3((unsigned)grid), dim3(64 * waves), lds, s,
+                       (ConstProblems)d_probs, d_hdr);
     return hipGetLastError();
 }
 

@@ -163,8 +163,11 @@ int srt_mfma_convert_b(int dtype, int64_t rows, int cols, const float *B, int64_
     return SRT_OK;
 }
 
-int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
-                          const float *col_scales, int n, int shape, int ksplit, srt_stream_t stream) {
+// Shared body of the two execute entry points: `work` holds the K-split
+// partial tiles (work_floats of them).
+static int mfma_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
+                        const float *col_scales, int n, int shape, int ksplit, float *work, size_t work_floats,
+                        const char *short_work, hipStream_t stream) {
     if (!p) return fail(SRT_ERR_INVALID, "null plan");
     if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
     if (n == 0 || p->M == 0) return SRT_OK;
@@ -180,8 +183,7 @@ int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const v
     if ((uintptr_t)B % 16 != 0 || (ldb * elt_bytes(p->dtype)) % 16 != 0)
         return fail(SRT_ERR_INVALID, "B (transposed) rows must be 16-byte aligned (as srt_mfma_convert_b writes them)");
     const int ks = std::max(1, std::min(ksplit, std::max(p->KB, 1)));
-    if (ks > 1 && (size_t)ks * p->M * n > p->work_floats)
-        return fail(SRT_ERR_INVALID, "K split needs a larger reservation (srt_mfma_plan_reserve)");
+    if (ks > 1 && (size_t)ks * p->M * n > work_floats) return fail(SRT_ERR_INVALID, short_work);
     MfmaLaunch a;
     a.fp8 = p->dtype == SRT_DT_FP8;
     a.shape = shape;
@@ -199,9 +201,16 @@ int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const v
     a.C = C;
     a.ldc = ldc;
     a.ks = ks;
-    a.work = p->work;
-    SRT_HIP_TRY(launch_mfma_exec(a, (hipStream_t)stream));
+    a.work = work;
+    SRT_HIP_TRY(launch_mfma_exec(a, stream));
     return SRT_OK;
+}
+
+int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
+                          const float *col_scales, int n, int shape, int ksplit, srt_stream_t stream) {
+    if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    return mfma_execute(p, C, ldc, B, ldb, col_scales, n, shape, ksplit, p->work, p->work_floats,
+                        "K split needs a larger reservation (srt_mfma_plan_reserve)", (hipStream_t)stream);
 }
 
 int srt_mfma_plan_stats_get(const srt_mfma_plan *p, srt_mfma_stats *o) {
@@ -286,6 +295,20 @@ int srt_bprep_problem_tiles(const srt_bprep_problem *pr) {
     if (!pr) return -1;
     if (pr->N <= 0 || pr->ldk <= 0) return 0;
     return (int)(((pr->ldk + 31) / 32) * ((pr->N + 31) / 32));
+}
+
+int srt_widen_problem_tiles(const srt_bprep_problem *pr) {
+    if (!pr) return -1;
+    if (pr->K <= 0 || pr->ldk <= 0) return 0;
+    return (int)(((int64_t)pr->K * pr->ldk + 1023) / 1024);
+}
+
+int srt_exec_grouped_b_widen(const srt_bprep_problem *d_probs, const int *d_hdr, int grid, srt_stream_t stream) {
+    if (grid < 1) return fail(SRT_ERR_INVALID, "grid must be >= 1");
+    if (!d_probs || !d_hdr) return fail(SRT_ERR_INVALID, "null table");
+    SRT_HIP_TRY(launch_grouped_b_widen(reinterpret_cast<const BPrepProblem *>(d_probs), d_hdr, grid,
+                                       (hipStream_t)stream));
+    return SRT_OK;
 }
 
 int srt_mfma_grouped_b_prep(int dtype, const srt_bprep_problem *d_probs, const int *d_hdr, int grid,

@@ -2,7 +2,7 @@
 // kernels with the executor's grid and block shape that do (almost) nothing,
 // so their rocprofv3 duration is the floor every executor launch pays.
 //   kind 0: empty body
-//   kind 1: every wave issues `depth` dependent scalar loads from `buf`
+//   kind 1: every wave issues `depth` (1..4) dependent scalar loads from `buf`
 //           (the first from a per-wave cache line, each next address taken
 //           from the loaded word), the latency chain a cold executor wave
 //           walks before its first B load (descriptor -> record -> values);
@@ -17,14 +17,16 @@ namespace {
 
 __global__ void floor_empty_kernel() {}
 
-__global__ void floor_chain_kernel(const int32_t *__restrict__ buf, int64_t words, int depth,
-                                   int32_t *__restrict__ sink) {
+template <int DEPTH>
+__global__ void floor_chain_kernel(const int32_t *__restrict__ buf, int64_t words, int32_t *__restrict__ sink) {
+    constexpr int depth = DEPTH;
     const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-    int64_t at = ((int64_t)wave * 32) % words;  // one 128-byte line per wave
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 32;  // one 128-byte line per wave and step
+    int64_t at = ((int64_t)wave * 32) % words;
     int32_t v = 0;
     for (int d = 0; d < depth; ++d) {
         v = __builtin_amdgcn_readfirstlane(buf[at]);
-        at = (at + 32 + (int64_t)(uint32_t)v) % words;
+        at = (at + stride + (int64_t)(uint32_t)v) % words;
     }
     if (v == 0x7fffffff && threadIdx.x == 0) sink[0] = v;  // keeps the loads; never true for a zero buffer
 }
@@ -40,10 +42,17 @@ extern "C" int srt_probe_launch_floor(int kind, int grid, int block, const void 
     if (kind == 0) {
         hipLaunchKernelGGL(floor_empty_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream);
     } else if (kind == 1) {
-        if (!buf || !sink || buf_bytes < 128 || depth < 0 || depth > 64)
-            return fail(SRT_ERR_INVALID, "probe: chain needs a buffer of >= 128 bytes, a sink and depth <= 64");
-        hipLaunchKernelGGL(floor_chain_kernel, dim3(grid), dim3(block), 0, (hipStream_t)stream,
-                           (const int32_t *)buf, (int64_t)(buf_bytes / 4), depth, (int32_t *)sink);
+        if (!buf || !sink || buf_bytes < 128 || depth < 1 || depth > 4)
+            return fail(SRT_ERR_INVALID, "probe: chain needs a buffer of >= 128 bytes, a sink and depth in [1, 4]");
+        const int32_t *b = (const int32_t *)buf;
+        const int64_t w = buf_bytes / 4;
+        int32_t *k = (int32_t *)sink;
+        hipStream_t st = (hipStream_t)stream;
+        // one instantiation per depth: rocprofv3 reports each under its own name
+        if (depth == 1) hipLaunchKernelGGL(floor_chain_kernel<1>, dim3(grid), dim3(block), 0, st, b, w, k);
+        if (depth == 2) hipLaunchKernelGGL(floor_chain_kernel<2>, dim3(grid), dim3(block), 0, st, b, w, k);
+        if (depth == 3) hipLaunchKernelGGL(floor_chain_kernel<3>, dim3(grid), dim3(block), 0, st, b, w, k);
+        if (depth == 4) hipLaunchKernelGGL(floor_chain_kernel<4>, dim3(grid), dim3(block), 0, st, b, w, k);
     } else {
         return fail(SRT_ERR_INVALID, "probe: kind must be 0 or 1");
     }

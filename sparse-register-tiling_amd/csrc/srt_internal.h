@@ -68,6 +68,9 @@ struct ExecProblem {
 };
 typedef hipError_t (*ExecGroupedFn)(const ExecProblem *d_probs, int nprob, int total_blocks, int waves,
                                     size_t lds, int xcd_remap, hipStream_t s);
+// Persistent grouped launch: problem count and workgroup total read from d_hdr.
+typedef hipError_t (*ExecPersistentFn)(const ExecProblem *d_probs, const int *d_hdr, int grid, int waves,
+                                       size_t lds, hipStream_t s);
 
 // One generated executor (one pattern set).  launch[s][v][u]: s = 0 reads B
 // rows from global memory, s = 1 stages the B tile in LDS; V = 1,2,4 and
@@ -79,6 +82,7 @@ struct ExecutorDesc {
     const int32_t *patterns;  // id order
     ExecLaunchFn launch[2][3][4];
     ExecGroupedFn grouped[3];  // global-memory B path, unroll 4, V = 1,2,4 (null: not generated)
+    ExecPersistentFn persistent[3];  // the same variants, persistent over a device problem table
 };
 
 // Largest dynamic LDS allocation of one workgroup on the current device.
@@ -142,6 +146,7 @@ struct Schedule {
     int layout = 1;           // realised tile config layout (1 panel order, 2 balanced)
     int n_tiles = 1;          // N tiles the balanced layout was sized for
     int reduce = 0;
+    size_t bytes = 0;         // device bytes of desc + prog
     std::vector<int32_t> kp;  // per panel
     void release();
 };
@@ -260,7 +265,12 @@ struct BPrepProblem {
 hipError_t launch_mfma_grouped(bool fp8, int shape, const MfmaProblem *probs, const int *hdr, int grid,
                                hipStream_t s);
 hipError_t launch_grouped_b_prep(bool fp8, const BPrepProblem *probs, const int *hdr, int grid, hipStream_t s);
+hipError_t launch_grouped_b_widen(const BPrepProblem *probs, const int *hdr, int grid, hipStream_t s);
 int mfma_shape_tile(int shape, int *nt, int *wp, int *pw);
+
+// Dense 16-row panels of hybrid plans on FP32 matrix cores (dense_panels.hip).
+hipError_t launch_dense_panels(int ndense, int G, const void *apack, const int32_t *panel_row, const float *B,
+                               int64_t ldb, float *C, int64_t ldc, int M, int N, int K, hipStream_t s);
 
 // CSR baseline (csr_spmm.hip).
 hipError_t launch_csr(int m, int k, int n, int64_t nnz, const void *rowptr, const void *colind,

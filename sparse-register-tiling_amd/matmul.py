@@ -104,7 +104,10 @@ class MatMulSpecialized:
                  executor_id: Optional[str] = None,
                  mapping_id: Union[str, Mapping, NativeMapping] = "400fa",
                  datatransform: bool = True, mapping_search_dirs: Optional[Sequence[str]] = None,
-                 stream: Optional[torch.cuda.Stream] = None):
+                 stream: Optional[torch.cuda.Stream] = None, dense_threshold: Optional[float] = None):
+        """``dense_threshold``: 16-row panels at least this dense run on FP32
+        matrix cores (a hybrid plan, srt_plan_create_hybrid); None = every
+        panel register-tiled."""
         device = torch.device("cuda", torch.cuda.current_device())
         self.mapping = resolve_mapping(mapping_id, search_dirs=mapping_search_dirs)
         Lp_t = _index_tensor(Lp, device)
@@ -122,9 +125,17 @@ class MatMulSpecialized:
         cfg = (tile_config or TileConfig()).to_c()
         h = ct.c_void_p()
         eid = executor_id.encode() if executor_id else None
-        _lib.call("srt_plan_create", self.m, self.k, self.n, ptr(Lx_t), ptr(Lp_t), ptr(Li_t),
-                  Lp_t.element_size(), ct.byref(cfg), int(nThreads), eid, self.mapping.handle,
-                  stream_handle(stream), ct.byref(h))
+        if dense_threshold is None:
+            _lib.call("srt_plan_create", self.m, self.k, self.n, ptr(Lx_t), ptr(Lp_t), ptr(Li_t),
+                      Lp_t.element_size(), ct.byref(cfg), int(nThreads), eid, self.mapping.handle,
+                      stream_handle(stream), ct.byref(h))
+        else:
+            if executor_id:
+                raise ValueError("a hybrid plan derives its executor from the mapping")
+            _lib.call("srt_plan_create_hybrid", self.m, self.k, self.n, ptr(Lx_t), ptr(Lp_t), ptr(Li_t),
+                      Lp_t.element_size(), ct.byref(cfg), float(dense_threshold), self.mapping.handle,
+                      stream_handle(stream), ct.byref(h))
+        self.dense_threshold = dense_threshold
         self._h = h
         self.device = device
 
@@ -141,6 +152,27 @@ class MatMulSpecialized:
     def require_storage(self) -> int:
         v = ct.c_size_t()
         _lib.call("srt_plan_required_storage", self._h, ct.byref(v))
+        return v.value
+
+    def dense_panels(self) -> Dict[str, object]:
+        """Dense 16-row panels of a hybrid plan: count, first rows, device bytes."""
+        n = ct.c_int()
+        b = ct.c_size_t()
+        _lib.call("srt_plan_dense_panels", self._h, ct.byref(n), None, ct.byref(b))
+        rows = np.zeros(max(1, n.value), dtype=np.int32)
+        _lib.call("srt_plan_dense_panels", self._h, None, rows.ctypes.data_as(ct.c_void_p), None)
+        return {"count": n.value, "rows": rows[:n.value], "bytes": b.value}
+
+    def release_columns(self) -> None:
+        """Free the packed column indices (execution reads the batch records);
+        the schedule then cannot be rebuilt for another width."""
+        _lib.call("srt_plan_release_columns", self._h)
+
+    def schedule_bytes(self) -> int:
+        """Device bytes of the current launch schedule (wave descriptors and
+        batch records) beside the packed data."""
+        v = ct.c_size_t()
+        _lib.call("srt_plan_schedule_bytes", self._h, ct.byref(v))
         return v.value
 
     def __call__(self, C: torch.Tensor, B: torch.Tensor, n: Optional[int] = None,

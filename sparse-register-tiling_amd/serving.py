@@ -1,12 +1,26 @@
 """Serving form: resident pre-packed sparse weights, continuous batching.
 
 No counterpart in the reference (its bindings run one SpMM per call,
-SpMM_SOP.h:178, template:307-392).  A ``Server`` keeps packed BF16 weights
-(``PanelPlan``, 16-row MFMA panels) resident in HBM and serves requests
-C = W . B (B: K x N BF16, any N) as they arrive:
+SpMM_SOP.h:178, template:307-392).  A ``Server`` keeps packed weights
+resident in HBM and serves requests C = W . B (B: K x N BF16, any N) as they
+arrive.  Two weight forms (backends):
 
-* admission by HBM budget: each admitted request holds its transposed B
-  operand and its FP32 C in one block of a reserved slab (power-of-two size
+* ``RegTileWeights`` (the default served form): the register-tiled sparse
+  panels of the plain path (``MatMulSpecialized``, 4-row "orig" pattern set)
+  over BF16-valued weights (each value rounded to BF16, held as FP32 for the
+  scalar unit), so resident bytes follow the nonzeros; the requests' BF16 B
+  is widened to FP32 on the device and every SpMM of a batch runs in one
+  persistent grouped executor launch.  FP32 accumulation, the same fused
+  multiply-add chain as a lone launch: results equal the oracle's chain over
+  the BF16 operands bit for bit;
+* ``PanelPlan`` weights: 16-row BF16 / FP8 panels on matrix cores (every
+  16 x 32 block holding a nonzero stored dense), B transposed on the device,
+  one persistent grouped MFMA launch.
+
+Common to both:
+
+* admission by HBM budget: each admitted request holds its device operand
+  and its FP32 C in one block of a reserved slab (power-of-two size
   classes); requests wait while the budget is full;
 * continuous batching: whenever one of the launch slots is free, every
   admitted request not yet running (up to ``max_batch``) goes into one batch;
@@ -14,9 +28,8 @@ C = W . B (B: K x N BF16, any N) as they arrive:
   the GPU overlap on the device;
 * one batch = one replay of a HIP graph captured per slot: upload of the
   problem tables (pinned host -> device), one grouped launch preparing every
-  request's B operand, one persistent grouped MFMA launch computing every
-  request's tiles.  Problem counts live in device memory, so the same graph
-  replays any batch;
+  request's B operand, one persistent grouped SpMM launch.  Problem counts
+  live in device memory, so the same graph replays any batch;
 * latency = completion (HIP event on the slot's stream, relative to a
   reference event) - arrival; p50 / p99 and throughput are reported.
 
@@ -123,19 +136,208 @@ class Request:
         return self.done - self.arrival
 
 
+EXEC_PROBLEM_DTYPE = np.dtype([("prog", "<u8"), ("desc", "<u8"), ("vals", "<u8"), ("B", "<u8"), ("C", "<u8"),
+                               ("ldc", "<i8"), ("b_bytes", "<u4"), ("ldb", "<u4"), ("M", "<i4"), ("N", "<i4"),
+                               ("K", "<i4"), ("n_wblocks", "<i4"), ("reduce", "<i4"), ("load_c", "<i4"),
+                               ("block_begin", "<i4"), ("pad", "<i4")])
+assert EXEC_PROBLEM_DTYPE.itemsize == 88
+
+
+def _align(x: int, a: int = 256) -> int:
+    return -(-int(x) // a) * a
+
+
+class MfmaBackend:
+    """16-row BF16 / FP8 panels on matrix cores (``PanelPlan`` weights)."""
+
+    kind = "mfma"
+    prob_dtype = PROBLEM_DTYPE
+
+    def __init__(self, weights: Sequence[PanelPlan], shape: int, grid: int):
+        dt = {w.dtype for w in weights}
+        if len(dt) != 1:
+            raise ValueError("all weights must share one dtype")
+        self.dtype = dt.pop()
+        self.code = DTYPES[self.dtype]
+        self.weights = list(weights)
+        self.shape, self.grid = int(shape), int(grid)
+        self.esz = 2 if self.dtype == "bf16" else 1
+        self.wM = np.array([w.m for w in weights], dtype=np.int64)
+        self.wK = np.array([w.k for w in weights], dtype=np.int64)
+        self.wldk = -(-self.wK // 32) * 32
+        nt, wp, pw = SHAPES[self.shape]
+        self.wrows = -(-self.wM // (16 * wp * pw))  # row tiles per weight
+
+    def weight_records(self, base: int) -> None:
+        n = len(self.weights)
+        self.rec = np.zeros(n, dtype=PROBLEM_DTYPE)
+        for i, w in enumerate(self.weights):
+            _lib.call("srt_mfma_problem_fill", w._h, ct.c_void_p(base), int(self.wldk[i]), None,
+                      ct.c_void_p(base), 1, 1, 0, self.rec[i:i + 1].ctypes.data_as(ct.c_void_p))
+
+    def operand_bytes(self, wi: int, N: int) -> int:
+        return _align(N * int(self.wldk[wi]) * self.esz)
+
+    def resident_bytes(self) -> int:
+        return sum(w.stats()["stored_blocks"] * 512 * self.esz for w in self.weights)
+
+    def fill(self, slot: "_Slot", wi, N, bt, cp, bp) -> None:
+        nt, wp, pw = SHAPES[self.shape]
+        n = len(wi)
+        ldk = self.wldk[wi]
+        tiles = self.wrows[wi] * -(-N // nt)
+        ptiles = (ldk // 32) * -(-N // 32)
+        pr = slot.prob[:n]
+        pr[:] = self.rec[wi]
+        pr["Bt"] = bt
+        pr["C"] = cp
+        pr["ldb"] = ldk
+        pr["ldc"] = N
+        pr["N"] = N
+        pr["tile_begin"] = np.cumsum(tiles) - tiles
+        pr["vec_ok"] = 1
+        pp = slot.prep[:n]
+        pp["src"] = bp
+        pp["dst"] = bt
+        pp["cscale"] = 0
+        pp["ld_src"] = N
+        pp["ldk"] = ldk
+        pp["K"] = self.wK[wi]
+        pp["N"] = N
+        pp["tile_begin"] = np.cumsum(ptiles) - ptiles
+        pp["src_bf16"] = 1
+        slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()))
+
+    def enqueue(self, slot: "_Slot", s: ct.c_void_p) -> None:
+        _lib.call("srt_mfma_grouped_b_prep", self.code, ct.c_void_p(slot.d_prep.data_ptr()),
+                  ct.c_void_p(slot.d_hdr.data_ptr()), self.grid, s)
+        _lib.call("srt_mfma_grouped_execute", self.code, self.shape, ct.c_void_p(slot.d_prob.data_ptr()),
+                  ct.c_void_p(slot.d_hdr.data_ptr() + 8), self.grid, s)
+
+
+class RegTileWeights:
+    """Register-tiled sparse weights for serving: each W rounded to BF16
+    values (held as FP32), packed with one mapping and one launch shape so
+    every weight runs in the same persistent grouped launch.  The launch
+    shape keeps one float per lane (any N), panel-order workgroups (a
+    schedule that serves every N) and unroll 4 (the persistent variant)."""
+
+    def __init__(self, weights, mapping=(4, "orig"), waves_per_block: int = 8, nsplit: int = 8,
+                 chunk_cols: int = 64, n_max: int = 512):
+        from .mapping import named_mapping
+        from .matmul import MatMulSpecialized, TileConfig
+        from .native import resolve_mapping
+        nm = resolve_mapping(named_mapping(*mapping))
+        cfg = TileConfig(vec_width=1, unroll=4, waves_per_block=waves_per_block, nsplit=nsplit,
+                         chunk_cols=chunk_cols, layout=1, b_staging=2, xcd_remap=1)
+        self.mapping = mapping
+        self.plans = []
+        self.dense_shapes = []
+        self.nnz = 0
+        for W in weights:
+            W = torch.as_tensor(W, dtype=torch.float32).cpu()
+            Wq = W.to(torch.bfloat16).to(torch.float32)    # BF16 values, exact in FP32
+            csr = Wq.to_sparse_csr()
+            self.nnz += int(csr.values().numel())
+            plan = MatMulSpecialized.from_csr(csr, n_max, mapping_id=nm, tile_config=cfg)
+            plan.release_columns()   # the batch records carry the B rows; the schedule never changes
+            self.plans.append(plan)
+            self.dense_shapes.append(tuple(W.shape))
+
+    def __len__(self) -> int:
+        return len(self.plans)
+
+    def __getitem__(self, i):
+        return self.plans[i]
+
+    def resident_bytes(self) -> int:
+        """Device bytes held per weight: packed values, batch records, wave
+        descriptors and the small per-panel arrays (counts, offsets, groups)."""
+        return sum(p.require_storage + p.schedule_bytes() for p in self.plans)
+
+    def resident_breakdown(self) -> Dict[str, int]:
+        st = [p.stats() for p in self.plans]
+        return {"values": sum(4 * s["packed_values"] for s in st),
+                "schedules": sum(p.schedule_bytes() for p in self.plans),
+                "other": sum(p.require_storage - 4 * s["packed_values"] for p, s in zip(self.plans, st))}
+
+
+class RegTileBackend:
+    """Register-tiled sparse panels (``RegTileWeights``): BF16 B widened to
+    FP32 on the device, one persistent grouped executor launch per batch."""
+
+    kind = "regtile"
+    prob_dtype = EXEC_PROBLEM_DTYPE
+    V = 1
+
+    def __init__(self, weights: RegTileWeights, grid: int):
+        self.weights = weights
+        self.plans = weights.plans
+        self.grid = int(grid)
+        self.wM = np.array([p.m for p in self.plans], dtype=np.int64)
+        self.wK = np.array([p.k for p in self.plans], dtype=np.int64)
+
+    def weight_records(self, base: int) -> None:
+        n = len(self.plans)
+        self.rec = np.zeros(n, dtype=EXEC_PROBLEM_DTYPE)
+        for i, p in enumerate(self.plans):
+            _lib.call("srt_exec_problem_fill", p._h, None, 1, None, 1, 0, self.V, 0,
+                      self.rec[i:i + 1].ctypes.data_as(ct.c_void_p))
+        self.wblocks = self.rec["n_wblocks"].astype(np.int64)
+
+    def operand_bytes(self, wi: int, N: int) -> int:
+        return _align(N * int(self.wK[wi]) * 4)
+
+    def resident_bytes(self) -> int:
+        return self.weights.resident_bytes()
+
+    def fill(self, slot: "_Slot", wi, N, bt, cp, bp) -> None:
+        n = len(wi)
+        K = self.wK[wi]
+        blocks = self.wblocks[wi] * -(-N // (64 * self.V))
+        wtiles = -(-(K * N) // 1024)   # srt_widen_problem_tiles: the K x N output walked flat
+        pr = slot.prob[:n]
+        pr[:] = self.rec[wi]
+        pr["B"] = bt
+        pr["C"] = cp
+        pr["ldb"] = N
+        pr["ldc"] = N
+        pr["N"] = N
+        pr["b_bytes"] = np.maximum(K * N * 4, 4)
+        pr["block_begin"] = np.cumsum(blocks) - blocks
+        pp = slot.prep[:n]
+        pp["src"] = bp
+        pp["dst"] = bt
+        pp["cscale"] = 0
+        pp["ld_src"] = N
+        pp["ldk"] = N
+        pp["K"] = K
+        pp["N"] = N
+        pp["tile_begin"] = np.cumsum(wtiles) - wtiles
+        pp["src_bf16"] = 1
+        slot.hdr[:] = (n, int(wtiles.sum()), n, int(blocks.sum()))
+
+    def enqueue(self, slot: "_Slot", s: ct.c_void_p) -> None:
+        _lib.call("srt_exec_grouped_b_widen", ct.c_void_p(slot.d_prep.data_ptr()),
+                  ct.c_void_p(slot.d_hdr.data_ptr()), self.grid, s)
+        _lib.call("srt_exec_persistent_execute", self.plans[0]._h, self.V, ct.c_void_p(slot.d_prob.data_ptr()),
+                  ct.c_void_p(slot.d_hdr.data_ptr() + 8), self.grid, s)
+
+
 class _Slot:
-    def __init__(self, max_batch: int, device, dtype_code: int, shape: int, grid: int):
+    def __init__(self, max_batch: int, device, backend):
         self.stream = torch.cuda.Stream(device=device)
-        self.h_prob = torch.zeros(max_batch * PROBLEM_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
+        self.backend = backend
+        pdt = backend.prob_dtype
+        self.h_prob = torch.zeros(max_batch * pdt.itemsize, dtype=torch.uint8).pin_memory()
         self.h_prep = torch.zeros(max_batch * BPREP_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
         self.h_hdr = torch.zeros(4, dtype=torch.int32).pin_memory()  # prep {n, tiles}, spmm {n, tiles}
         self.d_prob = torch.zeros_like(self.h_prob, device=device)
         self.d_prep = torch.zeros_like(self.h_prep, device=device)
         self.d_hdr = torch.zeros(4, dtype=torch.int32, device=device)
-        self.prob = np.frombuffer(self.h_prob.numpy(), dtype=PROBLEM_DTYPE)
+        self.prob = np.frombuffer(self.h_prob.numpy(), dtype=pdt)
         self.prep = np.frombuffer(self.h_prep.numpy(), dtype=BPREP_DTYPE)
         self.hdr = self.h_hdr.numpy()
-        self.dtype_code, self.shape, self.grid = dtype_code, shape, grid
         self.graph: Optional[torch.cuda.CUDAGraph] = None
         self.batch: List[Request] = []
         self.end = torch.cuda.Event(enable_timing=True)
@@ -145,10 +347,7 @@ class _Slot:
         self.d_prob.copy_(self.h_prob, non_blocking=True)
         self.d_prep.copy_(self.h_prep, non_blocking=True)
         self.d_hdr.copy_(self.h_hdr, non_blocking=True)
-        _lib.call("srt_mfma_grouped_b_prep", self.dtype_code, ct.c_void_p(self.d_prep.data_ptr()),
-                  ct.c_void_p(self.d_hdr.data_ptr()), self.grid, s)
-        _lib.call("srt_mfma_grouped_execute", self.dtype_code, self.shape, ct.c_void_p(self.d_prob.data_ptr()),
-                  ct.c_void_p(self.d_hdr.data_ptr() + 8), self.grid, s)
+        self.backend.enqueue(self, s)
 
     def capture(self) -> None:
         """Capture upload + grouped prep + grouped SpMM once; replays reuse it."""
@@ -171,70 +370,60 @@ class _Slot:
 
 
 class Server:
-    """Resident weights + admission + continuous batching (see module doc)."""
+    """Resident weights + admission + continuous batching (see module doc).
 
-    def __init__(self, weights: Sequence[PanelPlan], budget_bytes: int = 256 << 20, max_batch: int = 64,
+    ``weights``: a ``RegTileWeights`` (register-tiled sparse panels) or a
+    list of ``PanelPlan`` (MFMA panels).  ``grid``: workgroups of the
+    persistent launches; ``shape``: MFMA launch shape (MFMA weights only)."""
+
+    def __init__(self, weights, budget_bytes: int = 256 << 20, max_batch: int = 64,
                  slots: int = 3, shape: int = 6, grid: int = 1024, use_graphs: bool = True):
-        if not weights:
+        if weights is None or len(weights) == 0:
             raise ValueError("no weights")
-        dt = {w.dtype for w in weights}
-        if len(dt) != 1:
-            raise ValueError("all weights must share one dtype")
-        self.dtype = dt.pop()
-        self.weights = list(weights)
+        if isinstance(weights, RegTileWeights):
+            self.backend = RegTileBackend(weights, grid)
+            self.dtype = "bf16"
+        else:
+            self.backend = MfmaBackend(weights, shape, grid)
+            self.dtype = self.backend.dtype
+        self.weights = weights
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.pool = SlabPool(budget_bytes, self.device)
         self.max_batch = int(max_batch)
         self.shape = int(shape)
         self.grid = int(grid)
         self.use_graphs = use_graphs
-        code = DTYPES[self.dtype]
-        self.slots = [_Slot(self.max_batch, self.device, code, self.shape, self.grid) for _ in range(slots)]
+        self.slots = [_Slot(self.max_batch, self.device, self.backend) for _ in range(slots)]
         if use_graphs:
             for sl in self.slots:
                 sl.capture()
         torch.cuda.synchronize()
-        self._weight_tables()
+        self.backend.weight_records(self.pool.base)
         self.stats: Dict[str, float] = {}
 
-    # -- admission ------------------------------------------------------------
-    def _weight_tables(self) -> None:
-        """Per-weight constants of the problem records (one C call per weight)."""
-        nt, wp, pw = SHAPES[self.shape]
-        n = len(self.weights)
-        self._wrec = np.zeros(n, dtype=PROBLEM_DTYPE)
-        self._wM = np.array([w.m for w in self.weights], dtype=np.int64)
-        self._wK = np.array([w.k for w in self.weights], dtype=np.int64)
-        self._wldk = -(-self._wK // 32) * 32
-        self._wrows = -(-self._wM // (16 * wp * pw))  # row tiles per weight
-        for i, w in enumerate(self.weights):
-            _lib.call("srt_mfma_problem_fill", w._h, ct.c_void_p(self.pool.base), int(self._wldk[i]), None,
-                      ct.c_void_p(self.pool.base), 1, 1, 0, self._wrec[i:i + 1].ctypes.data_as(ct.c_void_p))
+    def resident_bytes(self) -> int:
+        return self.backend.resident_bytes()
 
+    # -- admission ------------------------------------------------------------
     def request_bytes(self, req: Request) -> int:
-        w = self.weights[req.weight]
-        ldk = -(-w.k // 32) * 32
-        esz = 2 if self.dtype == "bf16" else 1
-        return -(-req.N * ldk * esz // 256) * 256 + w.m * req.N * 4
+        return self.backend.operand_bytes(req.weight, req.N) + int(self.backend.wM[req.weight]) * req.N * 4
 
     def admit(self, req: Request) -> bool:
         """Reserve the request's operand and result in the budget (False = wait)."""
-        w = self.weights[req.weight]
-        if req.B.shape[0] != w.k or req.B.dtype != torch.bfloat16 or not req.B.is_contiguous():
+        K = int(self.backend.wK[req.weight])
+        if req.B.shape[0] != K or req.B.dtype != torch.bfloat16 or not req.B.is_contiguous():
             raise ValueError("request B must be a contiguous K x N BF16 tensor for weight %d" % req.weight)
         nbytes = self.request_bytes(req)
         off = self.pool.alloc(nbytes)
         if off is None:
             return False
-        ldk = -(-w.k // 32) * 32
         req._off, req._bytes = off, nbytes
         req._bt_off = off
-        req._c_off = off + -(-req.N * ldk * 2 // 256) * 256
+        req._c_off = off + self.backend.operand_bytes(req.weight, req.N)
         return True
 
     def result_view(self, req: Request) -> torch.Tensor:
-        w = self.weights[req.weight]
-        return self.pool.view(req._c_off, (w.m, req.N), torch.float32)
+        return self.pool.view(req._c_off, (int(self.backend.wM[req.weight]), req.N), torch.float32)
 
     def release(self, req: Request, keep_result: bool = False, stream: Optional[torch.cuda.Stream] = None) -> None:
         """Return the request's memory to the budget.  With keep_result the
@@ -249,36 +438,13 @@ class Server:
     # -- batches ------------------------------------------------------------
     def _fill(self, slot: _Slot, batch: List[Request]) -> None:
         """Problem tables of one batch, vectorised over its requests."""
-        nt, wp, pw = SHAPES[self.shape]
         n = len(batch)
         wi = np.fromiter((r.weight for r in batch), dtype=np.int64, count=n)
         N = np.fromiter((r.N for r in batch), dtype=np.int64, count=n)
         bt = self.pool.base + np.fromiter((r._bt_off for r in batch), dtype=np.int64, count=n)
         cp = self.pool.base + np.fromiter((r._c_off for r in batch), dtype=np.int64, count=n)
         bp = np.fromiter((r.B.data_ptr() for r in batch), dtype=np.int64, count=n)
-        ldk = self._wldk[wi]
-        tiles = self._wrows[wi] * -(-N // nt)
-        ptiles = (ldk // 32) * -(-N // 32)
-        pr = slot.prob[:n]
-        pr[:] = self._wrec[wi]
-        pr["Bt"] = bt
-        pr["C"] = cp
-        pr["ldb"] = ldk
-        pr["ldc"] = N
-        pr["N"] = N
-        pr["tile_begin"] = np.cumsum(tiles) - tiles
-        pr["vec_ok"] = 1
-        pp = slot.prep[:n]
-        pp["src"] = bp
-        pp["dst"] = bt
-        pp["cscale"] = 0
-        pp["ld_src"] = N
-        pp["ldk"] = ldk
-        pp["K"] = self._wK[wi]
-        pp["N"] = N
-        pp["tile_begin"] = np.cumsum(ptiles) - ptiles
-        pp["src_bf16"] = 1
-        slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()))
+        self.backend.fill(slot, wi, N, bt, cp, bp)
 
     def run_batch(self, batch: List[Request], slot_index: int = 0) -> None:
         """Admit, run and release one batch synchronously; results stay in

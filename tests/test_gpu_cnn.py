@@ -39,7 +39,7 @@ def _bits(x):
     return np.ascontiguousarray(x).view(np.uint32)
 
 
-@pytest.mark.parametrize("batch", [1, 4])
+@pytest.mark.parametrize("batch", [1, 4, 8])
 @pytest.mark.parametrize("mapping,shape", [((4, "orig"), dict(vec_width=4, waves_per_block=8, nsplit=1)),
                                            ((8, "orig"), dict(vec_width=4, waves_per_block=8, nsplit=8)),
                                            ((4, "identity"), dict(vec_width=2, waves_per_block=16, nsplit=16))])

@@ -49,3 +49,24 @@ def test_reference_default_k_tile_on_the_2048x512_workload(srt, oracle):
     pk = cb.pack_dense_ktiled(A, 4, _oracle_table(so, 4, "orig"), 256)
     _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=plan.split_counts()))
     plan.close()
+
+
+@pytest.mark.parametrize("strategy,n", [(1, 256), (2, 512), (2, 96)])
+def test_tiling_strategy_k_tile(srt, oracle, strategy, n):
+    """tiling_strategy 1 (CAKE) sizes K_c to a workgroup's LDS B tile;
+    2 adds the TLB compensation K_c = entries * page / (n * 4) whenever the
+    n x K_c B block needs more pages (SpMM_GECSB.h:339-348).  The packing is
+    the K-tiled one, bit-exact."""
+    import torch
+    so, cb = oracle
+    A = srt.workloads.magnitude_pruned(256, 2048, 0.9, seed=3)
+    B = srt.workloads.dense_b(2048, n, seed=4)
+    plan = _plan(srt, torch, A, n, 4, "orig", tiling_strategy=strategy, vec_width=4,
+                 max_tlb_entries=64, tlb_page_size=4096)
+    kc = plan.get_config().K_c
+    cake = 160 * 1024 // (64 * 4 * 4)          # 160 KiB of LDS / (64 lanes x 4 floats x 4 bytes) = 160 rows
+    tlb = 64 * 4096 // (n * 4) if n * cake * 4 // 4096 > 64 else cake
+    assert kc == (cake if strategy == 1 else tlb), (strategy, n, kc)
+    pk = cb.pack_dense_ktiled(A, 4, _oracle_table(so, 4, "orig"), kc)
+    _bitexact(_run(srt, torch, plan, B), cb.exec_packed(pk, B, nsplit=plan.split_counts()))
+    plan.close()

@@ -120,9 +120,9 @@ def test_unaligned_b_and_empty(srt, torch_cuda, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "fp8"])
-@pytest.mark.parametrize("mk,s", [((4096, 1024), 0.5), ((1024, 4096), 0.7)])
+@pytest.mark.parametrize("mk,s", [(mk, s) for mk in ((4096, 1024), (1024, 4096)) for s in (0.5, 0.6, 0.7)])
 def test_precision_workloads_error_vs_fp64(srt, torch_cuda, dtype, mk, s):
-    """The precision configurations (magnitude-pruned, N = 512)."""
+    """All six precision configurations (magnitude-pruned, N = 512)."""
     M, K = mk
     _log("precision workload", dtype, mk, s)
     case = srt.workloads.SpMMCase(M, K, s, 512)

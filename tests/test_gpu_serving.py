@@ -139,3 +139,98 @@ def test_full_serving_workload(full_env):
         assert np.array_equal(r.C.cpu().numpy().view(np.uint32), a.view(np.uint32)), "request %d" % r.rid
         assert np.array_equal(a.view(np.uint32), served[r.rid].view(np.uint32)), "request %d" % r.rid
     assert server.pool.used == 0
+
+
+# ------------------------------------------------------------------ register-tiled sparse weights
+def _oracle_served(so, cb, W, B_bf16, plan):
+    """The oracle's chain over the BF16-valued weight (its own packing with
+    the 4-row "orig" strategy) and the BF16 B widened to FP32."""
+    import torch
+    Wq = torch.from_numpy(W).to(torch.bfloat16).float().numpy()
+    pk = cb.pack_dense(Wq, 4, so.named_table(4, "orig"))
+    B = B_bf16.float().cpu().numpy()
+    return cb.exec_packed(pk, B, nsplit=plan.split_counts())
+
+
+@pytest.fixture(scope="module")
+def rt_env():
+    import importlib
+    import torch
+    torch.cuda.init()
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    shapes = [(256, 128), (128, 256), (300, 96), (64, 520)]
+    Ws = [srt.workloads.magnitude_pruned(M, K, 0.9, seed=i) for i, (M, K) in enumerate(shapes)]
+    weights = srt.serving.RegTileWeights(Ws)
+    return srt, torch, shapes, Ws, weights
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_regtile_served_results_exact(rt_env, oracle, graphs):
+    """Register-tiled sparse serving: every result equals the oracle's chain
+    over the BF16 operands bit for bit (and is within 1e-2 of FP64)."""
+    srt, torch, shapes, Ws, weights = rt_env
+    so, cb = oracle
+    _log("regtile serving graphs=%s" % graphs)
+    server = srt.serving.Server(weights, budget_bytes=64 << 20, max_batch=8, slots=3, use_graphs=graphs, grid=256)
+    reqs = _requests(srt, torch, shapes, 40, qps=20000.0, seed=13)
+    st = server.serve(reqs, keep_results=True)
+    _log("  ", st)
+    assert st["requests"] == 40 and st["batches"] >= 5
+    for r in reqs:
+        ref = _oracle_served(so, cb, Ws[r.weight], r.B, weights[r.weight])
+        got = r.C.cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "request %d" % r.rid
+        assert po.rel_error_vs_fp64(got, Ws[r.weight], r.B.float().cpu().numpy()) <= 1e-2
+    assert server.pool.used == 0
+
+
+def test_regtile_alone_equals_mixed_and_budget(rt_env):
+    srt, torch, shapes, Ws, weights = rt_env
+    server = srt.serving.Server(weights, budget_bytes=64 << 20, max_batch=16, slots=2, grid=64)
+    reqs = _requests(srt, torch, shapes, 12, qps=1e6, seed=15)
+    alone = []
+    for r in reqs:
+        server.run_batch([r])
+        alone.append(r.C.cpu().numpy())
+    mixed = [srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B) for r in reqs]
+    server.run_batch(list(reversed(mixed)), slot_index=1)
+    for r, a in zip(mixed, alone):
+        assert np.array_equal(r.C.cpu().numpy().view(np.uint32), a.view(np.uint32)), "request %d" % r.rid
+    small = srt.serving.Server(weights, budget_bytes=256 << 10, max_batch=64, slots=2, grid=64)
+    with pytest.raises(MemoryError):
+        big = srt.serving.Request(rid=99, weight=3, B=torch.zeros((520, 512), dtype=torch.bfloat16, device="cuda"))
+        small.run_batch([big])
+    st = small.serve(_requests(srt, torch, shapes, 24, qps=1e6, seed=17), keep_results=True)
+    assert st["batches"] > 1
+
+
+def test_regtile_full_serving_workload(oracle):
+    """The 48 weights at their real shapes on register-tiled sparse panels:
+    resident bytes well below the dense BF16 size, every result exact
+    against the oracle, alone == mixed."""
+    import importlib
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    so, cb = oracle
+    Ws = srt.serving.serving_weights()
+    weights = srt.serving.RegTileWeights(Ws)
+    dense_bf16 = sum(W.size * 2 for W in Ws)
+    nnz = sum(int((W != 0).sum()) for W in Ws)
+    server = srt.serving.Server(weights, budget_bytes=256 << 20, max_batch=32, slots=3, grid=1024)
+    resident = server.resident_bytes()
+    _log("regtile 48 weights: resident %.1f MB, dense BF16 %.1f MB, nnz BF16 %.1f MB"
+         % (resident / 1e6, dense_bf16 / 1e6, 2 * nnz / 1e6))
+    assert resident < 0.7 * dense_bf16
+    shapes = [W.shape for W in Ws]
+    reqs = srt.serving.poisson_requests(40, 50000.0, shapes, seed=21, n_max=512)
+    big = srt.serving.Request(rid=999, weight=3, B=(torch.rand((4096, 512), device="cuda") * 2 - 1).to(torch.bfloat16))
+    reqs.append(big)
+    st = server.serve(reqs, keep_results=True)
+    _log("  ", st)
+    for r in reqs:
+        ref = _oracle_served(so, cb, Ws[r.weight], r.B, weights[r.weight])
+        assert np.array_equal(r.C.cpu().numpy().view(np.uint32), ref.view(np.uint32)), "request %d" % r.rid
+    mixed = [srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B) for r in reqs[:16]]
+    server.run_batch(list(reversed(mixed)), slot_index=1)
+    for r, m in zip(reqs[:16], mixed):
+        assert np.array_equal(r.C.cpu().numpy().view(np.uint32), m.C.cpu().numpy().view(np.uint32))

@@ -22,8 +22,15 @@ def test_table_covers_every_workload(srt):
         assert rest == sorted(rest, key=lambda r: r["seconds"])
 
 
-@pytest.mark.parametrize("case", [(2048, 512, 0.9, 512), (512, 2048, 0.7, 128), (1024, 4096, 0.95, 256)])
+def _gpu_cases():
+    import importlib
+    wl = importlib.import_module("sparse-register-tiling_amd.workloads")
+    return [(c.M, c.K, c.sparsity, c.N) for c in wl.gpu_cases()]
+
+
+@pytest.mark.parametrize("case", _gpu_cases())
 def test_tuned_plan_exact(srt, oracle, case):
+    """Every one of the 48 pruned-FFN workloads, with its shipped configuration."""
     import torch
     so, cb = oracle
     at = srt.autotune

@@ -258,3 +258,12 @@ def test_group_abi_rejects_bad_arguments(srt):
     assert L.srt_group_info(None, None, None, None) == srt._lib.SRT_ERR_INVALID
     assert L.srt_group_rebind(None, None, None, None, None, None, None) == srt._lib.SRT_ERR_INVALID
     L.srt_group_destroy(None)
+
+
+def test_sparsity_bucket_rounds_pruning_counts(srt):
+    """A matrix pruned to a nominal sparsity keeps floor or round of the kept
+    count; both land in the nominal sparsity's bucket of the tuning table."""
+    at = srt.autotune
+    assert at.sparsity_bucket(1 - 78643 / 262144.0) == "(0,0.7]"        # 512 x 512 at 70 %
+    assert at.sparsity_bucket(1 - 104858 / 1048576.0) == "(0.8,0.9]"    # 2048 x 512 at 90 %
+    assert at.sparsity_bucket(0.95) == "(0.9,1]" and at.sparsity_bucket(0.75) == "(0.7,0.8]"
