@@ -385,6 +385,19 @@ int srt_mfma_convert_b(int dtype, int64_t rows, int cols, const float *B, int64_
  * ranges are added in order. */
 int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
                           const float *col_scales, int n, int shape, int ksplit, srt_stream_t stream);
+/* As srt_mfma_plan_execute, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * K-split partial tiles live in a library-owned static device array of
+ * SRT_NOALLOC_SCRATCH_FLOATS floats (one per GPU, held until the process
+ * ends), so ksplit * m * n must not exceed it (SRT_ERR_INVALID otherwise; no
+ * reservation is needed).  Every call on a GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race.  Nothing in it has
+ * to start at zero: each call writes every partial it reads. */
+#define SRT_NOALLOC_SCRATCH_FLOATS (8u << 20) /* 32 MiB */
+int srt_mfma_plan_execute_noalloc(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
+                                  const float *col_scales, int n, int shape, int ksplit, srt_stream_t stream);
+
 int srt_mfma_plan_stats_get(const srt_mfma_plan *p, srt_mfma_stats *out);
 int srt_mfma_plan_row_scales(const srt_mfma_plan *p, float *out);
 /* Host copies: slot[P*KB] (block slot), blk_kb[stored], blocks (512 elements each). */

@@ -125,6 +125,7 @@ SIGNATURES = {
     "srt_mfma_plan_reserve": (_i, [_vp, _i, _i]),
     "srt_mfma_convert_b": (_i, [_i, _i64, _i, _vp, _i64, _vp, _i64, _vp, _vp]),
     "srt_mfma_plan_execute": (_i, [_vp, _vp, _i64, _vp, _i64, _vp, _i, _i, _i, _vp]),
+    "srt_mfma_plan_execute_noalloc": (_i, [_vp, _vp, _i64, _vp, _i64, _vp, _i, _i, _i, _vp]),
     "srt_mfma_plan_stats_get": (_i, [_vp, ct.POINTER(MfmaStatsC)]),
     "srt_mfma_plan_row_scales": (_i, [_vp, _vp]),
     "srt_mfma_plan_copy_blocks": (_i, [_vp, _vp, _vp, _vp]),

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,34 @@ struct srt_mfma_plan {
 };
 
 static int elt_bytes(int dtype) { return dtype == SRT_DT_FP8 ? 1 : 2; }
+
+// Scratch of the allocation-free entry point (srt_mfma_plan_execute_noalloc):
+// the K-split partial tiles, ks * M * n floats at most SRT_NOALLOC_SCRATCH_FLOATS.
+// Every GPU the library runs on holds its own copy until the process ends.
+// Host code never passes `scratch` itself to a kernel: the address of this
+// device's copy comes from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) float scratch[SRT_NOALLOC_SCRATCH_FLOATS];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+static int noalloc_scratch(float **out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static float *ptrs[kMaxDevices] = {};
+    int dev = -1;
+    SRT_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SRT_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(mu);
+    if (!ptrs[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(SRT_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        ptrs[dev] = static_cast<float *>(ptr);
+    }
+    *out = ptrs[dev];
+    return SRT_OK;
+}
 
 extern "C" {
 
@@ -211,6 +240,21 @@ int srt_mfma_plan_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const v
     if (!p) return fail(SRT_ERR_INVALID, "null plan");
     return mfma_execute(p, C, ldc, B, ldb, col_scales, n, shape, ksplit, p->work, p->work_floats,
                         "K split needs a larger reservation (srt_mfma_plan_reserve)", (hipStream_t)stream);
+}
+
+int srt_mfma_plan_execute_noalloc(const srt_mfma_plan *p, float *C, int64_t ldc, const void *B, int64_t ldb,
+                                  const float *col_scales, int n, int shape, int ksplit, srt_stream_t stream) {
+    if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
+    const int ks = std::max(1, std::min(ksplit, std::max(p->KB, 1)));
+    if (ks > 1 && (size_t)ks * p->M * (size_t)n > (size_t)SRT_NOALLOC_SCRATCH_FLOATS)
+        return fail(SRT_ERR_INVALID, "ksplit * m * n = " + std::to_string((size_t)ks * p->M * (size_t)n) +
+                                         " exceeds the scratch maximum of " +
+                                         std::to_string((size_t)SRT_NOALLOC_SCRATCH_FLOATS) + " floats");
+    float *work = nullptr;
+    if (ks > 1) MF_TRY(noalloc_scratch(&work));
+    return mfma_execute(p, C, ldc, B, ldb, col_scales, n, shape, ksplit, work, SRT_NOALLOC_SCRATCH_FLOATS,
+                        "K split exceeds the scratch maximum", (hipStream_t)stream);
 }
 
 int srt_mfma_plan_stats_get(const srt_mfma_plan *p, srt_mfma_stats *o) {
