@@ -64,7 +64,8 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--form", default="plain", choices=["plain", "precision", "serving", "cnn", "sweep", "floor"],
+    ap.add_argument("--form", default="plain",
+                    choices=["plain", "precision", "serving", "cnn", "sweep", "floor", "hybrid"],
                     help="plain = the headline FP32 workload (default); the others print their own line")
     ap.add_argument("--qps", default="5000,10000,20000,40000,80000,160000",
                     help="serving: offered loads (requests/s)")
@@ -72,6 +73,8 @@ def parse():
     ap.add_argument("--slo-ms", type=float, default=2.0, help="serving: p99 target for the throughput ratio")
     ap.add_argument("--mfma-shape", type=int, default=6, help="serving: MFMA tile shape of the grouped launch (0-7)")
     ap.add_argument("--cnn-batches", default="1,4,8", help="cnn: images per batch (N = H*W*batch)")
+    ap.add_argument("--thresholds", default="0.2,0.25,0.3,0.35,0.4",
+                    help="hybrid: dense-panel density thresholds to time")
     ap.add_argument("--sweep-cpu-seconds", type=float, default=0.3, help="sweep: host time per case")
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -425,6 +428,66 @@ def bench_floor(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_hybrid(args):
+    """Hybrid plans (dense 16-row panels on FP32 matrix cores, the rest
+    register-tiled) against the plain register-tiled plan of the shipped
+    tuning table and the dense FP32 product (torch.matmul), FP32, N = 512, on
+    magnitude-pruned matrices from 50 to 90 % sparsity.  Graph-timed on one
+    resident operand set; every hybrid result is checked against FP64."""
+    import numpy as np
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    at = importlib.import_module("sparse-register-tiling_amd.autotune")
+    from oracle import cbind, sop_oracle as so
+    wl = srt.workloads
+    cases = [(1024, 4096, s) for s in (0.5, 0.6, 0.7, 0.8)] + [(4096, 1024, s) for s in (0.5, 0.7)] + \
+            [(2048, 512, 0.9)]
+    taus = [float(t) for t in args.thresholds.split(",")]
+    rows = []
+    for M, K, sp in cases:
+        case = wl.SpMMCase(M, K, sp, 512)
+        A, B = wl.case_inputs(case, seed=0)
+        nnz = int(np.count_nonzero(A))
+        flop = 2.0 * nnz * case.N
+        csr = torch.from_numpy(A).to_sparse_csr()
+        Bd = torch.from_numpy(B).cuda()
+        C = torch.empty((M, case.N), device="cuda")
+        plain = at.tuned_plan(csr, case.N)
+        t_plain = at.time_graph(lambda: plain(C, Bd), reps=30)
+        cfg = plain.get_config()
+        mp = plain.mapping
+        plain.close()
+        A_dev = torch.from_numpy(A).cuda()
+        t_dense = at.time_graph(lambda: torch.matmul(A_dev, Bd, out=C), reps=30)
+        C64 = cbind.dense_f64(A, B)
+        bound = so.fp64_error_bound(A, B)
+        res = []
+        for tau in taus:
+            hp = srt.MatMulSpecialized.from_csr(csr, case.N, mapping_id=mp, tile_config=cfg, dense_threshold=tau)
+            t = at.time_graph(lambda: hp(C, Bd), reps=30)
+            hp(C, Bd)
+            torch.cuda.synchronize()
+            got = C.cpu().numpy()
+            ok = bool(np.all(np.abs(got - C64) <= bound))
+            rel = float(np.linalg.norm(got - C64) / np.linalg.norm(C64))
+            dp = hp.dense_panels()
+            res.append({"threshold": tau, "dense_panels": dp["count"], "panels": -(-M // 16), "us": round(t * 1e6, 2),
+                        "gflops": round(flop / t / 1e9, 1), "within_fp64_bound": ok, "rel_err_vs_fp64": rel})
+            hp.close()
+            log(M, K, sp, res[-1])
+        best = min(res, key=lambda r: r["us"])
+        rows.append({"M": M, "K": K, "sparsity": sp, "N": case.N, "nnz": nnz,
+                     "plain_us": round(t_plain * 1e6, 2), "dense_matmul_us": round(t_dense * 1e6, 2),
+                     "hybrid": res, "best_hybrid_us": best["us"], "best_threshold": best["threshold"]})
+        log(rows[-1]["M"], rows[-1]["K"], sp, "plain", rows[-1]["plain_us"], "dense", rows[-1]["dense_matmul_us"],
+            "best hybrid", best["us"], "@", best["threshold"])
+        del A_dev
+        torch.cuda.empty_cache()
+    print(json.dumps({"form": "hybrid", "metric": "FP32 SpMM time, hybrid (dense panels on MFMA) vs plain vs dense",
+                      "unit": "us", "n_gpus": 1, "data": "synthetic", "timing": "graph, warm, one operand set",
+                      "results": rows}), flush=True)
+
+
 def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample=16, form="regtile"):
     """48 resident BF16 weights (12 x {1024x1024 x2, 4096x1024, 1024x4096},
     90 % magnitude-pruned), Poisson requests with log-uniform N in [1, 512]:
@@ -744,6 +807,8 @@ def main():
         return bench_sweep(args)
     if args.form == "floor":
         return bench_floor(args)
+    if args.form == "hybrid":
+        return bench_hybrid(args)
     import numpy as np
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")

@@ -132,6 +132,182 @@ def tune(csr: torch.Tensor, N: int, mappings: Sequence[Tuple[int, str]] = DEFAUL
     return out
 
 
+# ---------------------------------------------------------------- cold timing
+MALL_BYTES = 256 << 20
+
+
+def time_plan_cold(csr: torch.Tensor, N: int, mapping, cfg: TileConfig, reps: int = 60,
+                   B: Optional[torch.Tensor] = None) -> float:
+    """Seconds per launch with every launch cold in cache: launches rotate
+    over R replicas of (packed A, B, C) whose footprint exceeds twice the
+    256 MB last-level cache, captured in one graph (the headline protocol)."""
+    M, K = csr.shape
+    if B is None:
+        B = torch.rand((K, N), device="cuda") * 2 - 1
+    first = MatMulSpecialized.from_csr(csr, N, mapping_id=mapping, tile_config=cfg)
+    footprint = first.require_storage + first.schedule_bytes() + 4 * (K * N + M * N)
+    R = max(2, min(reps, -(-2 * MALL_BYTES // footprint)))
+    plans = [first] + [MatMulSpecialized.from_csr(csr, N, mapping_id=mapping, tile_config=cfg)
+                       for _ in range(R - 1)]
+    Bs = [B] + [B.clone() for _ in range(R - 1)]
+    Cs = [torch.empty((M, N), device="cuda") for _ in range(R)]
+    i = [0]
+
+    def launch():
+        j = i[0] % R
+        i[0] += 1
+        plans[j](Cs[j], Bs[j])
+
+    t = time_graph(launch, reps=max(reps, R), blocks=1)
+    for p in plans:
+        p.close()
+    return t
+
+
+# ----------------------------------------------------------- counter evidence
+def _probe_command(npz: str, N: int, rec: Dict, launches: int) -> List[str]:
+    code = ("import importlib; importlib.import_module('sparse-register-tiling_amd.autotune')"
+            ".probe_main(['--npz', %r, '--n', '%d', '--config', %r, '--launches', '%d'])"
+            % (npz, N, json.dumps({k: v for k, v in rec.items() if k != "realised"}), launches))
+    import sys
+    return [sys.executable, "-c", code]
+
+
+def probe_main(argv=None) -> None:
+    """Counter probe (run under rocprofv3 --pmc by counter_evidence): the
+    candidate's launches rotated over cold replicas, then one elementwise
+    copy of COPY_BYTES whose FETCH_SIZE / WRITE_SIZE readings calibrate the
+    counters."""
+    import argparse
+    import numpy as np
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--npz", required=True)
+    ap.add_argument("--n", type=int, required=True)
+    ap.add_argument("--config", required=True)
+    ap.add_argument("--launches", type=int, default=40)
+    a = ap.parse_args(argv)
+    z = np.load(a.npz)
+    M, K = int(z["shape"][0]), int(z["shape"][1])
+    csr = torch.sparse_csr_tensor(torch.from_numpy(z["rowptr"]), torch.from_numpy(z["col"]),
+                                  torch.from_numpy(z["val"]), size=(M, K))
+    rec = json.loads(a.config)
+    shape = {k: v for k, v in rec.items() if k in TileConfig.__dataclass_fields__}
+    mapping = named_mapping(rec["M_r"], rec["mapping"])
+    cfg = TileConfig(xcd_remap=1, **shape)
+    B = torch.rand((K, a.n), device="cuda") * 2 - 1
+    first = MatMulSpecialized.from_csr(csr, a.n, mapping_id=mapping, tile_config=cfg)
+    footprint = first.require_storage + first.schedule_bytes() + 4 * (K * a.n + M * a.n)
+    R = max(2, -(-2 * MALL_BYTES // footprint))
+    plans = [first] + [MatMulSpecialized.from_csr(csr, a.n, mapping_id=mapping, tile_config=cfg)
+                       for _ in range(R - 1)]
+    Bs = [B] + [B.clone() for _ in range(R - 1)]
+    Cs = [torch.empty((M, a.n), device="cuda") for _ in range(R)]
+    torch.cuda.synchronize()
+    for i in range(a.launches):
+        plans[i % R](Cs[i % R], Bs[i % R])
+    src = torch.ones(COPY_BYTES // 4, device="cuda")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+
+
+COPY_BYTES = 256 << 20
+EXEC_KERNEL = "exec_kernel"
+
+
+def _counter_means(outdir: str) -> Dict[str, Dict[str, float]]:
+    """Mean per dispatch of every counter, per kernel name, from rocprofv3 CSV."""
+    import csv
+    import glob
+    acc: Dict[str, Dict[str, List[float]]] = {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(
+                    float(r["Counter_Value"]))
+    return {k: {c: statistics.mean(v) for c, v in d.items()} for k, d in acc.items()}
+
+
+def counter_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 4, launches: int = 40,
+                     rocprof: str = "rocprofv3", timeout: int = 240) -> List[Dict]:
+    """rocprofv3 counter evidence for the ``top`` leading records: each runs
+    in child processes under ``rocprofv3 --pmc FETCH_SIZE``, then
+    ``--pmc WRITE_SIZE`` (cold rotated launches plus a calibration copy),
+    and gets ``hbm_bytes`` (HBM
+    bytes per launch, each counter scaled by what it reads for the copy of
+    known size) and ``hbm_gbps`` (those bytes over its cold time).  Records
+    the counters could not be read for are left without them."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    csr = csr.to_sparse_csr() if csr.layout != torch.sparse_csr else csr
+    with tempfile.TemporaryDirectory(prefix="srt_counters_") as tmp:
+        npz = os.path.join(tmp, "a.npz")
+        np.savez(npz, shape=np.array(csr.shape), rowptr=csr.crow_indices().cpu().numpy(),
+                 col=csr.col_indices().cpu().numpy(), val=csr.values().cpu().float().numpy())
+        for i, rec in enumerate(records[:top]):
+            scaled = {}
+            # One pass per counter: FETCH_SIZE takes 3 of the 4 TCC counter
+            # slots and WRITE_SIZE 2, so they cannot share a pass.
+            for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+                out = os.path.join(tmp, "p%d_%s" % (i, counter))
+                cmd = [rocprof, "--pmc", counter, "-d", out, "-o", "run", "--output-format", "csv",
+                       "--"] + _probe_command(npz, N, rec, launches)
+                try:
+                    subprocess.run(cmd, check=True, capture_output=True, timeout=timeout)
+                except (OSError, subprocess.SubprocessError):
+                    break
+                means = _counter_means(out)
+                ex = [d[counter] for k, d in means.items() if EXEC_KERNEL in k and counter in d]
+                cp = [d[counter] for k, d in means.items() if "elementwise" in k and counter in d]
+                if not ex or not cp or max(cp) <= 0:
+                    break
+                scaled[counter] = ex[0] * COPY_BYTES / max(cp)
+            if len(scaled) != 2:
+                continue
+            rec["hbm_bytes"] = int(round(scaled["FETCH_SIZE"] + scaled["WRITE_SIZE"]))
+            t = rec.get("seconds_cold") or rec["seconds"]
+            rec["hbm_gbps"] = round(rec["hbm_bytes"] / t / 1e9, 1)
+            rec["mfma_busy"] = 0.0   # register-tiled executors issue no MFMA
+    return records
+
+
+def rank_with_evidence(records: List[Dict], slack: float = 0.03) -> List[Dict]:
+    """Order records by cold time; among those within ``slack`` of the
+    fastest, the one moving the fewest HBM bytes leads (the same speed with
+    less traffic leaves more of the memory system to concurrent work)."""
+    key = lambda r: r.get("seconds_cold", r["seconds"])
+    recs = sorted(records, key=key)
+    if not recs:
+        return recs
+    best = key(recs[0])
+    near = [r for r in recs if key(r) <= best * (1 + slack) and "hbm_bytes" in r]
+    if near:
+        lead = min(near, key=lambda r: (r["hbm_bytes"], key(r)))
+        recs.remove(lead)
+        recs.insert(0, lead)
+    return recs
+
+
+def tune_with_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 4,
+                       B: Optional[torch.Tensor] = None, cold_reps: int = 60, counters: bool = True) -> List[Dict]:
+    """Second stage of tuning: the ``top`` records of a warm search re-timed
+    on the cold-rotated protocol (``seconds_cold``) and, with ``counters``,
+    given rocprofv3 HBM-traffic evidence; returned ranked by
+    ``rank_with_evidence``, each marked ``evidence``."""
+    head = [dict(r) for r in records[:top]]
+    for rec in head:
+        shape = {k: v for k, v in rec.items() if k in TileConfig.__dataclass_fields__}
+        rec["seconds_cold"] = time_plan_cold(csr, N, named_mapping(rec["M_r"], rec["mapping"]),
+                                             TileConfig(xcd_remap=1, **shape), reps=cold_reps, B=B)
+    if counters:
+        counter_evidence(csr, N, head, top=top)
+    head = rank_with_evidence(head)
+    for rec in head:
+        rec["evidence"] = "cold" + ("+counters" if "hbm_bytes" in rec else "")
+    return head + [dict(r) for r in records[top:]]
+
+
 # Measured choices shipped with the package (same layout as the cache).
 SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "heuristics", "mi355x.json")
 
@@ -204,6 +380,28 @@ def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--out", required=True, help="JSON table to write (the shipped one is %s)" % SHIPPED)
     ap.add_argument("--reps", type=int, default=20, help="launches per timing (the shortlist gets 3x)")
+    ap.add_argument("--evidence", default="",
+                    help="comma-separated MxK shapes (e.g. 2048x512) whose records get the second stage: "
+                         "cold-rotated timing and rocprofv3 counter evidence (from the table --out holds)")
+    ap.add_argument("--no-counters", action="store_true", help="evidence stage without rocprofv3")
+    ap.add_argument("--top", type=int, default=4)
     a = ap.parse_args(argv)
+    if a.evidence:
+        shapes = set(a.evidence.split(","))
+        cache = TuningCache(a.out)
+        for case in wl.gpu_cases():
+            if "%dx%d" % (case.M, case.K) not in shapes:
+                continue
+            recs = cache.get(case.M, case.K, case.sparsity, case.N)
+            if not recs:
+                continue
+            A, B = wl.case_inputs(case, seed=0)
+            csr = torch.from_numpy(A).to_sparse_csr()
+            recs = tune_with_evidence(csr, case.N, recs, top=a.top, B=torch.from_numpy(B).cuda(),
+                                      counters=not a.no_counters)
+            cache.put(case.M, case.K, case.sparsity, case.N, recs)
+            r = recs[0]
+            print(case, "cold %.2f us" % (r["seconds_cold"] * 1e6), "hbm %s B" % r.get("hbm_bytes"), flush=True)
+        return
     tune_workloads(wl.gpu_cases(), a.out, reps=a.reps,
                    log=lambda c, r: print(c, "%.2f us" % (r["seconds"] * 1e6), flush=True))

@@ -267,3 +267,35 @@ def test_sparsity_bucket_rounds_pruning_counts(srt):
     assert at.sparsity_bucket(1 - 78643 / 262144.0) == "(0,0.7]"        # 512 x 512 at 70 %
     assert at.sparsity_bucket(1 - 104858 / 1048576.0) == "(0.8,0.9]"    # 2048 x 512 at 90 %
     assert at.sparsity_bucket(0.95) == "(0.9,1]" and at.sparsity_bucket(0.75) == "(0.7,0.8]"
+
+
+def test_rank_with_evidence_prefers_less_traffic_within_slack(srt):
+    """Cold time orders the records; inside 3 % of the fastest the one with
+    the fewest HBM bytes leads; records without counters never jump."""
+    at = srt.autotune
+    recs = [dict(seconds=1.0, seconds_cold=10.0e-6, hbm_bytes=9_000_000, id="a"),
+            dict(seconds=1.0, seconds_cold=10.2e-6, hbm_bytes=6_000_000, id="b"),
+            dict(seconds=1.0, seconds_cold=10.25e-6, id="c"),
+            dict(seconds=1.0, seconds_cold=12.0e-6, hbm_bytes=1_000_000, id="d")]
+    assert [r["id"] for r in at.rank_with_evidence(recs)] == ["b", "a", "c", "d"]
+    recs[1]["seconds_cold"] = 10.4e-6
+    assert [r["id"] for r in at.rank_with_evidence(recs)] == ["a", "c", "b", "d"]
+
+
+def test_counter_means_reads_rocprof_csv(srt, tmp_path):
+    """Per-kernel, per-counter means over dispatches from rocprofv3's
+    counter_collection CSV (the layout rocprofv3 --output-format csv writes)."""
+    at = srt.autotune
+    d = tmp_path / "host" / "1234"
+    d.mkdir(parents=True)
+    (d / "run_counter_collection.csv").write_text(
+        "Correlation_Id,Dispatch_Id,Agent_Id,Queue_Id,Process_Id,Thread_Id,Grid_Size,Kernel_Id,Kernel_Name,"
+        "Workgroup_Size,LDS_Block_Size,Scratch_Size,VGPR_Count,Accum_VGPR_Count,SGPR_Count,Counter_Name,Counter_Value,"
+        "Start_Timestamp,End_Timestamp\n"
+        "1,1,1,1,1,1,262144,3,exec_kernel<4>,1024,0,0,64,0,32,FETCH_SIZE,4000,0,1\n"
+        "2,2,1,1,1,1,262144,3,exec_kernel<4>,1024,0,0,64,0,32,FETCH_SIZE,6000,0,1\n"
+        "2,2,1,1,1,1,262144,3,exec_kernel<4>,1024,0,0,64,0,32,WRITE_SIZE,2048,0,1\n"
+        "3,3,1,1,1,1,1024,4,elementwise_kernel,256,0,0,16,0,16,FETCH_SIZE,262144,0,1\n")
+    m = at._counter_means(str(tmp_path))
+    assert m["exec_kernel<4>"] == {"FETCH_SIZE": 5000.0, "WRITE_SIZE": 2048.0}
+    assert m["elementwise_kernel"] == {"FETCH_SIZE": 262144.0}
