@@ -174,10 +174,11 @@ def _probe_command(npz: str, N: int, rec: Dict, launches: int) -> List[str]:
 
 
 def probe_main(argv=None) -> None:
-    """Counter probe (run under rocprofv3 --pmc by counter_evidence): the
-    candidate's launches rotated over cold replicas, then one elementwise
-    copy of COPY_BYTES whose FETCH_SIZE / WRITE_SIZE readings calibrate the
-    counters."""
+    """Counter probe (run under rocprofv3 --pmc by counter_evidence): one
+    launch per replica (first touch of their pages), then ``--launches``
+    launches rotated over the cold replicas (the ones measured), then one
+    elementwise copy of COPY_BYTES whose FETCH_SIZE / WRITE_SIZE readings
+    calibrate the counters."""
     import argparse
     import numpy as np
     ap = argparse.ArgumentParser()
@@ -203,7 +204,7 @@ def probe_main(argv=None) -> None:
     Bs = [B] + [B.clone() for _ in range(R - 1)]
     Cs = [torch.empty((M, a.n), device="cuda") for _ in range(R)]
     torch.cuda.synchronize()
-    for i in range(a.launches):
+    for i in range(R + a.launches):
         plans[i % R](Cs[i % R], Bs[i % R])
     src = torch.ones(COPY_BYTES // 4, device="cuda")
     dst = torch.empty_like(src)
@@ -213,23 +214,34 @@ def probe_main(argv=None) -> None:
 
 COPY_BYTES = 256 << 20
 EXEC_KERNEL = "exec_kernel"
+# the directory holding this package (the probe child imports it from there)
+_IMPORT_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _counter_means(outdir: str) -> Dict[str, Dict[str, float]]:
-    """Mean per dispatch of every counter, per kernel name, from rocprofv3 CSV."""
+def _counter_values(outdir: str) -> Dict[str, Dict[str, List[float]]]:
+    """Every counter reading, per kernel name, in dispatch order, from
+    rocprofv3 CSV."""
     import csv
     import glob
-    acc: Dict[str, Dict[str, List[float]]] = {}
+    acc: Dict[str, Dict[str, List[Tuple[int, float]]]] = {}
     for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
         with open(path, newline="") as f:
             for r in csv.DictReader(f):
                 acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(
-                    float(r["Counter_Value"]))
-    return {k: {c: statistics.mean(v) for c, v in d.items()} for k, d in acc.items()}
+                    (int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    return {k: {c: [x for _, x in sorted(v)] for c, v in d.items()} for k, d in acc.items()}
+
+
+def _counter_means(outdir: str, last: Optional[int] = None) -> Dict[str, Dict[str, float]]:
+    """Mean per dispatch of every counter, per kernel name, from rocprofv3
+    CSV; with ``last``, over each kernel's last ``last`` dispatches only."""
+    return {k: {c: statistics.mean(v[-last:] if last else v) for c, v in d.items()}
+            for k, d in _counter_values(outdir).items()}
 
 
 def counter_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 4, launches: int = 40,
-                     rocprof: str = "rocprofv3", timeout: int = 240) -> List[Dict]:
+                     rocprof: str = "rocprofv3", timeout: int = 240, log=None,
+                     keep_dir: Optional[str] = None) -> List[Dict]:
     """rocprofv3 counter evidence for the ``top`` leading records: each runs
     in child processes under ``rocprofv3 --pmc FETCH_SIZE``, then
     ``--pmc WRITE_SIZE`` (cold rotated launches plus a calibration copy),
@@ -241,7 +253,9 @@ def counter_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 
     import tempfile
     import numpy as np
     csr = csr.to_sparse_csr() if csr.layout != torch.sparse_csr else csr
-    with tempfile.TemporaryDirectory(prefix="srt_counters_") as tmp:
+    with tempfile.TemporaryDirectory(prefix="srt_counters_") as tmp_own:
+        tmp = keep_dir or tmp_own  # keep_dir: the rocprofv3 output stays there
+        os.makedirs(tmp, exist_ok=True)
         npz = os.path.join(tmp, "a.npz")
         np.savez(npz, shape=np.array(csr.shape), rowptr=csr.crow_indices().cpu().numpy(),
                  col=csr.col_indices().cpu().numpy(), val=csr.values().cpu().float().numpy())
@@ -254,18 +268,25 @@ def counter_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 
                 cmd = [rocprof, "--pmc", counter, "-d", out, "-o", "run", "--output-format", "csv",
                        "--"] + _probe_command(npz, N, rec, launches)
                 try:
-                    subprocess.run(cmd, check=True, capture_output=True, timeout=timeout)
-                except (OSError, subprocess.SubprocessError):
+                    subprocess.run(cmd, check=True, capture_output=True, timeout=timeout, cwd=_IMPORT_ROOT)
+                except (OSError, subprocess.SubprocessError) as e:
+                    if log:
+                        log("counter pass %s of record %d failed: %s" % (counter, i, type(e).__name__))
                     break
-                means = _counter_means(out)
-                ex = [d[counter] for k, d in means.items() if EXEC_KERNEL in k and counter in d]
-                cp = [d[counter] for k, d in means.items() if "elementwise" in k and counter in d]
+                ex = [d[counter] for k, d in _counter_means(out, last=launches).items()
+                      if EXEC_KERNEL in k and counter in d]
+                # the calibration copy: the largest single elementwise dispatch
+                # (kernels of the same name also ran on small tensors)
+                cp = [x for k, d in _counter_values(out).items() if "elementwise" in k
+                      for x in d.get(counter, [])]
                 if not ex or not cp or max(cp) <= 0:
                     break
                 scaled[counter] = ex[0] * COPY_BYTES / max(cp)
             if len(scaled) != 2:
                 continue
             rec["hbm_bytes"] = int(round(scaled["FETCH_SIZE"] + scaled["WRITE_SIZE"]))
+            if log:
+                log("record %d: %d HBM bytes per launch" % (i, rec["hbm_bytes"]))
             t = rec.get("seconds_cold") or rec["seconds"]
             rec["hbm_gbps"] = round(rec["hbm_bytes"] / t / 1e9, 1)
             rec["mfma_busy"] = 0.0   # register-tiled executors issue no MFMA
@@ -290,7 +311,8 @@ def rank_with_evidence(records: List[Dict], slack: float = 0.03) -> List[Dict]:
 
 
 def tune_with_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 4,
-                       B: Optional[torch.Tensor] = None, cold_reps: int = 60, counters: bool = True) -> List[Dict]:
+                       B: Optional[torch.Tensor] = None, cold_reps: int = 60, counters: bool = True,
+                       log=None, keep_dir: Optional[str] = None) -> List[Dict]:
     """Second stage of tuning: the ``top`` records of a warm search re-timed
     on the cold-rotated protocol (``seconds_cold``) and, with ``counters``,
     given rocprofv3 HBM-traffic evidence; returned ranked by
@@ -300,8 +322,10 @@ def tune_with_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int 
         shape = {k: v for k, v in rec.items() if k in TileConfig.__dataclass_fields__}
         rec["seconds_cold"] = time_plan_cold(csr, N, named_mapping(rec["M_r"], rec["mapping"]),
                                              TileConfig(xcd_remap=1, **shape), reps=cold_reps, B=B)
+        if log:
+            log("cold %.2f us (warm %.2f us) %s" % (rec["seconds_cold"] * 1e6, rec["seconds"] * 1e6, shape))
     if counters:
-        counter_evidence(csr, N, head, top=top)
+        counter_evidence(csr, N, head, top=top, log=log, keep_dir=keep_dir)
     head = rank_with_evidence(head)
     for rec in head:
         rec["evidence"] = "cold" + ("+counters" if "hbm_bytes" in rec else "")
@@ -381,16 +405,19 @@ def main(argv=None) -> None:
     ap.add_argument("--out", required=True, help="JSON table to write (the shipped one is %s)" % SHIPPED)
     ap.add_argument("--reps", type=int, default=20, help="launches per timing (the shortlist gets 3x)")
     ap.add_argument("--evidence", default="",
-                    help="comma-separated MxK shapes (e.g. 2048x512) whose records get the second stage: "
-                         "cold-rotated timing and rocprofv3 counter evidence (from the table --out holds)")
+                    help="comma-separated workloads, MxK (every sparsity and N) or MxK:sparsity:N "
+                         "(e.g. 2048x512:0.9:512), whose records get the second stage: cold-rotated timing "
+                         "and rocprofv3 counter evidence (from the table --out holds)")
     ap.add_argument("--no-counters", action="store_true", help="evidence stage without rocprofv3")
     ap.add_argument("--top", type=int, default=4)
+    ap.add_argument("--counter-dir", help="keep the rocprofv3 counter output of the evidence stage here")
     a = ap.parse_args(argv)
     if a.evidence:
-        shapes = set(a.evidence.split(","))
+        wanted = set(a.evidence.split(","))
         cache = TuningCache(a.out)
         for case in wl.gpu_cases():
-            if "%dx%d" % (case.M, case.K) not in shapes:
+            shape = "%dx%d" % (case.M, case.K)
+            if shape not in wanted and "%s:%g:%d" % (shape, case.sparsity, case.N) not in wanted:
                 continue
             recs = cache.get(case.M, case.K, case.sparsity, case.N)
             if not recs:
@@ -398,7 +425,8 @@ def main(argv=None) -> None:
             A, B = wl.case_inputs(case, seed=0)
             csr = torch.from_numpy(A).to_sparse_csr()
             recs = tune_with_evidence(csr, case.N, recs, top=a.top, B=torch.from_numpy(B).cuda(),
-                                      counters=not a.no_counters)
+                                      counters=not a.no_counters, log=lambda m: print(case, m, flush=True),
+                                      keep_dir=a.counter_dir)
             cache.put(case.M, case.K, case.sparsity, case.N, recs)
             r = recs[0]
             print(case, "cold %.2f us" % (r["seconds_cold"] * 1e6), "hbm %s B" % r.get("hbm_bytes"), flush=True)

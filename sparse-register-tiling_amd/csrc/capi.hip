@@ -1355,7 +1355,15 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
     int dev = -1;
     SRT_HIP_TRY(hipGetDevice(&dev));
     if (dev != p->device) return fail(SRT_ERR_INVALID, "plan belongs to another device");
-    SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream));
+    // With every 16-row panel dense the register-tiled part is empty and the
+    // dense panels write every row of C.
+    const bool all_dense = p->dense.ndense > 0 && (int64_t)p->dense.ndense * 16 >= p->pk.M;
+    if (all_dense) {
+        if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
+        if (n > 0 && (!B || !C)) return fail(SRT_ERR_INVALID, "null B or C");
+        if (n > 0 && (ldb < n || ldc < n)) return fail(SRT_ERR_INVALID, "ldb and ldc must be >= n");
+    } else
+        SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream));
     if (p->dense.ndense && n > 0)  // the dense panels' rows, zero from the register-tiled part, overwritten
         SRT_HIP_TRY(launch_dense_panels(p->dense.ndense, p->dense.G, p->dense.apack, p->dense.panel_row, B, ldb, C,
                                         ldc, p->pk.M, n, p->pk.K, (hipStream_t)stream));

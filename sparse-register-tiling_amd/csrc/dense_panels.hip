@@ -5,14 +5,26 @@
 // past a density threshold (sbench/padding_reordering.py:36, 0.30).  A
 // hybrid plan takes the 16-row panels at or above its threshold out of the
 // register-tiled packing and stores them dense for v_mfma_f32_16x16x4_f32:
-// per panel and 16-column K group, lane l holds the four k-steps it feeds,
+// per panel and 16-column K group g, lane l holds the four k-steps it feeds,
 //   A[row = l % 16][k = 16 g + 4 j + l / 16],  j = 0..3,
 // as one 16-byte word (one load per lane per K group).
 //
-// A workgroup (4 waves) owns 4 dense panels x 64 columns of C: each K group's
-// B tile (16 rows x 64 columns) is staged in LDS (double-buffered) and every
-// wave runs 4 k-steps x 4 column tiles of MFMAs for its panel.  Accumulation
-// is FP32 in the matrix core, k ascending in steps of 4.
+// A workgroup (8 waves, two per SIMD) owns 4 dense panels x 32 columns of C.
+// K advances in stages of 128 (8 groups); wave w computes panel w % 4 over
+// groups 4 (w / 4) .. 4 (w / 4) + 3 of every stage, two 16x16 accumulator
+// tiles, and the two halves are added at the end (low half + high half).
+// The stage's B block (128 rows x 32 columns) is staged in LDS in MFMA
+// B-operand order, so each lane reads the four k-steps of a group and tile as
+// one 16-byte word:
+//   lds[group q][tile t][lane l] = B[16 q + 4 j + l / 16][16 t + l % 16], j = 0..3.
+// Stages are double-buffered: the next stage's B rows and A words are loaded
+// into registers while the current stage's MFMAs run, then written to the
+// other LDS buffer; one barrier per stage.  Accumulation is FP32 in the
+// matrix core, k ascending in steps of 4 within each half.
+//
+// Workgroups map to (group of 4 panels, 32-column slice) so that the blocks
+// the dispatcher sends to one XCD share 4 panel groups' A and 8 slices' B in
+// that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -24,67 +36,159 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kRows = 16;      // panel rows (MFMA M)
-constexpr int kGroup = 16;     // K columns per staged group (4 k-steps of 4)
-constexpr int kNT = 64;        // C columns per workgroup (4 MFMA N tiles)
-constexpr int kWaves = 4;      // panels per workgroup
+constexpr int kNT = 32;       // C columns per workgroup (2 MFMA N tiles)
+constexpr int kPanels = 4;    // panels per workgroup
+constexpr int kWaves = 8;     // two K halves per panel
+constexpr int kStageG = 8;    // K groups of 16 per stage
+constexpr int kHalfG = kStageG / 2;
+constexpr int kStageK = 16 * kStageG;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kXcds = 8;
+// B staging: a thread loads rows 16 q + 4 j + kr (j = 0..3) of one column and
+// writes them as one 16-byte LDS word; (q, kr) pairs per stage: 4 kStageG.
+constexpr int kBWords = kStageG * 4 * kNT / kThreads;  // 16-byte words per thread per stage (2)
 
-__global__ void __launch_bounds__(64 * kWaves)
-    dense_panels_kernel(int ndense, int G, const f32x4 *__restrict__ apack, const int32_t *__restrict__ panel_row,
-                        const float *__restrict__ B, int64_t ldb, float *__restrict__ C, int64_t ldc, int M, int N,
-                        int K) {
-    __shared__ float bt[2][kGroup][kNT];
+struct AStage {
+    f32x4 w[kHalfG];
+};
+
+__global__ void __launch_bounds__(kThreads)
+    dense_panels_kernel(int ndense, int G, int PG4, int NT, const f32x4 *__restrict__ apack, uint32_t a_bytes,
+                        const int32_t *__restrict__ panel_row, const float *__restrict__ B, int64_t ldb,
+                        uint32_t b_bytes, float *__restrict__ C, int64_t ldc, int M, int N, int K) {
+    __shared__ f32x4 bt[2][kStageG][2][64];
+    // XCD-aware block -> tile: blocks b, b + 8, ... run on one XCD; give them
+    // consecutive logical tiles (4 panel groups x consecutive slices).
+    const int T = PG4 * NT;
+    const int b = blockIdx.x;
+    const int L = (T % kXcds == 0) ? (b % kXcds) * (T / kXcds) + b / kXcds : b;
+    const int pgb = L / (4 * NT), rem = L % (4 * NT);
+    const int nt = rem / 4, pg = 4 * pgb + rem % 4;
+    if (pg * kPanels >= ndense) return;  // whole workgroup idle (uniform)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int d = blockIdx.x * kWaves + wave;  // dense panel of this wave (may be past the list)
+    const int half = wave / kPanels;
+    const int d = pg * kPanels + wave % kPanels;  // dense panel of this wave (may be past the list)
     const bool live = d < ndense;
-    const int n0 = blockIdx.y * kNT;
-    f32x4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int n0 = nt * kNT;
+    const int S = (K + kStageK - 1) / kStageK;
 
-    // stage group g of B into bt[buf]: 16 rows x 64 columns, 4 per thread,
-    // zeros past K and N
-    auto stage = [&](int g, int buf) {
-        const int r = threadIdx.x >> 4, c = (threadIdx.x & 15) * 4;
-        const int k = g * kGroup + r;
+    // Loads go through buffer resources: an offset past the resource's extent
+    // reads 0, so rows past K, columns past N and groups past G need no branch.
+    const __amdgpu_buffer_rsrc_t brsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(B), (short)0, (int)b_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t arsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<f32x4 *>(apack), (short)0, (int)a_bytes, 0x00020000);
+    const int bc = threadIdx.x % kNT;  // staged column
+    const int bn = n0 + bc;
+    const int bt_t = bc / 16, bt_jj = bc % 16;
+    f32x4 breg[kBWords];
+    auto load_b = [&](int s) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int n = n0 + c + e;
-            bt[buf][r][c + e] = (k < K && n < N) ? B[(int64_t)k * ldb + n] : 0.0f;
-        }
-    };
-    const f32x4 *ap = apack + (int64_t)(live ? d : 0) * G * 64 + lane;
-    stage(0, 0);
-    __syncthreads();
-    for (int g = 0; g < G; ++g) {
-        const int buf = g & 1;
-        if (g + 1 < G) stage(g + 1, buf ^ 1);
-        if (live) {
-            const f32x4 a = ap[(int64_t)g * 64];
+        for (int e = 0; e < kBWords; ++e) {
+            const int idx = threadIdx.x / kNT + (kThreads / kNT) * e;  // (q, kr) pair
+            const int q = idx / 4, kr = idx % 4;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int kr = 4 * j + (lane >> 4);
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const float b = bt[buf][kr][16 * t + (lane & 15)];
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b, acc[t], 0, 0, 0);
-                }
+                const int k = s * kStageK + 16 * q + 4 * j + kr;
+                const uint32_t at = ((uint32_t)k * (uint32_t)ldb + (uint32_t)bn) * 4u;
+                const uint32_t off = ((k < K) & (bn < N)) ? at : 0xffffffffu;
+                breg[e][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(brsrc, off, 0, 0));
             }
         }
+    };
+    auto write_b = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < kBWords; ++e) {
+            const int idx = threadIdx.x / kNT + (kThreads / kNT) * e;
+            const int q = idx / 4, kr = idx % 4;
+            bt[buf][q][bt_t][kr * 16 + bt_jj] = breg[e];
+        }
+    };
+    const uint32_t abase = ((uint32_t)(live ? d : 0) * (uint32_t)G * 64u + (uint32_t)lane) * 16u;
+    auto load_a = [&](AStage &a, int s) {
+#pragma unroll
+        for (int q = 0; q < kHalfG; ++q) {
+            const int g = s * kStageG + half * kHalfG + q;
+            const uint32_t off = (live & (g < G)) ? abase + (uint32_t)g * 1024u : 0xffffffffu;
+            a.w[q] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(arsrc, off, 0, 0));
+        }
+    };
+    f32x4 acc[2];
+    acc[0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    acc[1] = acc[0];
+    auto compute = [&](const AStage &a, int buf) {
+        // all of the half-stage's B words first (one LDS latency per stage)
+        f32x4 bw[kHalfG][2];
+#pragma unroll
+        for (int q = 0; q < kHalfG; ++q) {
+            bw[q][0] = bt[buf][half * kHalfG + q][0][lane];
+            bw[q][1] = bt[buf][half * kHalfG + q][1][lane];
+        }
+#pragma unroll
+        for (int q = 0; q < kHalfG; ++q) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w[q][j], bw[q][0][j], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w[q][j], bw[q][1][j], acc[1], 0, 0, 0);
+            }
+        }
+        // keep that order: the LDS reads as one group, then the MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x0100, 2 * kHalfG, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0008, 8 * kHalfG, 0);
+    };
+
+    // Two stages per iteration so the A register sets alternate without
+    // copies.  Stages past the last (at most one, to make the count even) are
+    // loaded, written and computed like the others: their offsets are out of
+    // range and read zeros, and adding those zero products leaves every
+    // accumulator bit-identical (it starts at +0, so it is never -0).  The
+    // loop therefore has a single path, and each of the compiler's waits is
+    // for loads issued one stage earlier.
+    AStage a0, a1;
+    load_b(0);
+    load_a(a0, 0);
+    write_b(0);
+    load_b(1);
+    load_a(a1, 1);
+    __syncthreads();
+    for (int s = 0; s < S; s += 2) {
+        compute(a0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        write_b(1);
+        load_b(s + 2);
+        load_a(a0, s + 2);
+        __syncthreads();
+        compute(a1, 1);
+        __builtin_amdgcn_sched_barrier(0);
+        write_b(0);
+        load_b(s + 3);
+        load_a(a1, s + 3);
         __syncthreads();
     }
-    if (!live) return;
+    // high half -> LDS (stage buffer 0 is free after the last barrier), low
+    // half adds it and stores
+    f32x4 *red = &bt[0][0][0][0];
+    const int slot = (wave % kPanels) * 2 * 64 + lane;
+    if (half == 1) {
+        red[slot] = acc[0];
+        red[slot + 64] = acc[1];
+    }
+    __syncthreads();
+    if (half == 1 || !live) return;
+    acc[0] += red[slot];
+    acc[1] += red[slot + 64];
     const int row0 = panel_row[d];
     // D layout: lane l holds rows 4 * (l / 16) + r of column l % 16.
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < 2; ++t) {
         const int n = n0 + 16 * t + (lane & 15);
-        if (n >= N) continue;
+        if (n < N) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = row0 + 4 * (lane >> 4) + r;
-            if (row < M) C[(int64_t)row * ldc + n] = acc[t][r];
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + 4 * (lane >> 4) + r;
+                if (row < M) C[(int64_t)row * ldc + n] = acc[t][r];
+            }
         }
     }
 }
@@ -94,9 +198,16 @@ __global__ void __launch_bounds__(64 * kWaves)
 hipError_t launch_dense_panels(int ndense, int G, const void *apack, const int32_t *panel_row, const float *B,
                                int64_t ldb, float *C, int64_t ldc, int M, int N, int K, hipStream_t s) {
     if (ndense <= 0 || N <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((ndense + kWaves - 1) / kWaves), (unsigned)((N + kNT - 1) / kNT));
-    hipLaunchKernelGGL(dense_panels_kernel, grid, dim3(64 * kWaves), 0, s, ndense, G,
-                       static_cast<const f32x4 *>(apack), panel_row, B, ldb, C, ldc, M, N, K);
+    // byte offsets are 32-bit (buffer loads); the extent must stay below the
+    // 0xffffffff offset that marks a zero read
+    const int64_t a_bytes = (int64_t)ndense * G * 64 * 16, b_bytes = (int64_t)K * ldb * 4;
+    if (a_bytes >= 0xffffffffLL || b_bytes >= 0xffffffffLL) return hipErrorInvalidValue;
+    const int PG = (ndense + kPanels - 1) / kPanels;
+    const int PG4 = (PG + 3) / 4 * 4;
+    const int NT = (N + kNT - 1) / kNT;
+    hipLaunchKernelGGL(dense_panels_kernel, dim3((unsigned)(PG4 * NT)), dim3(kThreads), 0, s, ndense, G, PG4, NT,
+                       static_cast<const f32x4 *>(apack), (uint32_t)a_bytes, panel_row, B, ldb, (uint32_t)b_bytes, C,
+                       ldc, M, N, K);
     return hipGetLastError();
 }
 

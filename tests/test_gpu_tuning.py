@@ -15,6 +15,10 @@ def test_table_covers_every_workload(srt):
     for case in srt.workloads.gpu_cases():
         recs = cache.get(case.M, case.K, case.sparsity, case.N)
         assert recs, case
+        # records given the evidence stage (cold timing, counters) lead, in its order
+        ev = [r for r in recs if r.get("evidence")]
+        assert recs[:len(ev)] == ev and ev == at.rank_with_evidence(ev)
+        recs = recs[len(ev):]
         refined = [r for r in recs if r.get("refined")]
         rest = recs[len(refined):]
         assert recs[:len(refined)] == refined and not any(r.get("refined") for r in rest)

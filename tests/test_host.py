@@ -299,3 +299,4 @@ def test_counter_means_reads_rocprof_csv(srt, tmp_path):
     m = at._counter_means(str(tmp_path))
     assert m["exec_kernel<4>"] == {"FETCH_SIZE": 5000.0, "WRITE_SIZE": 2048.0}
     assert m["elementwise_kernel"] == {"FETCH_SIZE": 262144.0}
+    assert at._counter_means(str(tmp_path), last=1)["exec_kernel<4>"]["FETCH_SIZE"] == 6000.0
