@@ -54,7 +54,8 @@ COPY_BYTES = 256 << 20
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "executor_2048x512_s90_n512_traffic.txt")
 EXEC_KERNEL = "exec_kernel"
 # realised tile-config fields that name a measured configuration
-GPU_FIELDS = ("K_c", "vec_width", "unroll", "waves_per_block", "nsplit", "chunk_cols", "b_staging", "layout")
+GPU_FIELDS = ("K_c", "vec_width", "unroll", "waves_per_block", "nsplit", "chunk_cols", "b_staging", "layout",
+              "matrix_cores")
 
 
 def log(*a):

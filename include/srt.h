@@ -102,6 +102,15 @@ typedef struct srt_tile_config {
                               CUs at about chunk_cols columns per wave), each
                               workgroup's slots then given one by one to its
                               panel with the longest chunk; 0 = choose (2)        */
+    int matrix_cores;      /* 1: run the packed columns on matrix cores, one
+                              v_mfma_f32_4x4x1_16b_f32 outer product per packed
+                              column and 64 C columns (B slice staged in LDS).
+                              Needs M_r = 4, a mapping whose only kernel pattern
+                              is 0b1111 (every column padded to the full 4-row
+                              column, e.g. the "full" mapping 7da09) and K <= the
+                              LDS rows (640 on gfx950); the fused multiply-adds
+                              run in packed order, so results are the register
+                              tile's bits.  0 = register tile (VALU)            */
 } srt_tile_config;
 
 /* Default configuration (the reference's SOPConfig defaults, SpMM_SOP.h:33-54,

@@ -41,7 +41,7 @@ class TileConfigC(ct.Structure):
         ("runtime_schedule", ct.c_int), ("vec_width", ct.c_int),
         ("waves_per_block", ct.c_int), ("xcd_remap", ct.c_int), ("unroll", ct.c_int),
         ("nsplit", ct.c_int), ("chunk_cols", ct.c_int), ("b_staging", ct.c_int),
-        ("layout", ct.c_int),
+        ("layout", ct.c_int), ("matrix_cores", ct.c_int),
     ]
 
 

@@ -38,6 +38,10 @@ TABLE_SHAPES = [dict(b_staging=2, vec_width=v, unroll=u, waves_per_block=w, nspl
                [dict(b_staging=1, vec_width=1, unroll=u, waves_per_block=w, nsplit=s, K_c=0, layout=lay)
                 for u in (2, 4) for (w, s) in ((16, 16), (16, 8), (8, 8)) for lay in (1, 2)]
 TABLE_MAPPINGS = [(4, "orig"), (8, "orig"), (4, "identity"), (2, "identity")]
+# Matrix-core candidate: the full 4-row packing on v_mfma_f32_4x4x1_16b
+# (its B slice, K rows x 64 columns, must fit the LDS: K <= 640).
+MATRIX_CORE_CANDIDATE = ((4, "full"), dict(matrix_cores=1))
+MATRIX_CORE_MAX_K = 640
 BUCKETS = [(0.0, 0.7), (0.7, 0.8), (0.8, 0.9), (0.9, 1.0)]
 
 
@@ -381,6 +385,39 @@ def tune_workloads(cases, path: str, reps: int = 20, seed: int = 0, log=None) ->
     return cache
 
 
+def add_candidate(cache: "TuningCache", case, mapping: Tuple[int, str], shape: Dict, reps: int = 60,
+                  seed: int = 0) -> Dict:
+    """Time one more candidate for a workload already in the table and file
+    it among the refined records (timed with the refined records' 3x
+    replays) at its place by time; evidence marks are dropped (the order
+    they certified changed: run the evidence stage again)."""
+    from . import workloads as wl
+    A, B = wl.case_inputs(case, seed=seed)
+    csr = torch.from_numpy(A).to_sparse_csr()
+    Bd = torch.from_numpy(B).cuda()
+    C = torch.empty((case.M, case.N), device="cuda")
+    M_r, name = mapping
+    cfg = TileConfig(xcd_remap=1, **shape)
+    plan = MatMulSpecialized.from_csr(csr, case.N, mapping_id=named_mapping(M_r, name), tile_config=cfg)
+    real = plan.get_config()
+    rec = {"M_r": M_r, "mapping": name, "seconds": time_plan(plan, Bd, C, 3 * reps),
+           "padding": plan.stats()["padding"]}
+    rec.update(shape)
+    rec["realised"] = {k: getattr(real, k) for k in ("b_staging", "vec_width", "waves_per_block", "nsplit",
+                                                     "chunk_cols", "K_c", "layout", "matrix_cores")}
+    rec["refined"] = True
+    plan.close()
+    old = cache.get(case.M, case.K, case.sparsity, case.N) or []
+    keys = ("b_staging", "vec_width", "unroll", "waves_per_block", "nsplit", "K_c", "layout", "matrix_cores")
+    same = lambda r: r["M_r"] == M_r and r["mapping"] == name and all(r.get(k) == rec.get(k) for k in keys)
+    old = [{k: v for k, v in r.items() if k not in ("evidence", "seconds_cold", "hbm_bytes", "hbm_gbps",
+                                                        "mfma_busy")} for r in old if not same(r)]
+    refined = sorted([r for r in old if r.get("refined")] + [rec], key=lambda r: r["seconds"])
+    rest = [r for r in old if not r.get("refined")]
+    cache.put(case.M, case.K, case.sparsity, case.N, refined + rest, keep=len(refined) + len(rest))
+    return rec
+
+
 def tuned_plan(csr: torch.Tensor, N: int, path: str = SHIPPED) -> MatMulSpecialized:
     """A plan with the best measured configuration of the tuning table for
     this shape, sparsity bucket and N (the reference's heuristic-JSON lookup,
@@ -411,7 +448,20 @@ def main(argv=None) -> None:
     ap.add_argument("--no-counters", action="store_true", help="evidence stage without rocprofv3")
     ap.add_argument("--top", type=int, default=4)
     ap.add_argument("--counter-dir", help="keep the rocprofv3 counter output of the evidence stage here")
+    ap.add_argument("--add-matrix-cores", action="store_true",
+                    help="time the matrix-core candidate for every workload with K <= 640 and file it "
+                         "among the refined records of the table --out holds")
     a = ap.parse_args(argv)
+    if a.add_matrix_cores:
+        cache = TuningCache(a.out)
+        for case in wl.gpu_cases():
+            if case.K > MATRIX_CORE_MAX_K or not cache.get(case.M, case.K, case.sparsity, case.N):
+                continue
+            rec = add_candidate(cache, case, *MATRIX_CORE_CANDIDATE)
+            best = cache.get(case.M, case.K, case.sparsity, case.N)[0]
+            print(case, "matrix cores %.2f us, best %.2f us (%s)" % (rec["seconds"] * 1e6, best["seconds"] * 1e6,
+                                                                    best["mapping"]), flush=True)
+        return
     if a.evidence:
         wanted = set(a.evidence.split(","))
         cache = TuningCache(a.out)

@@ -39,7 +39,8 @@ CFLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I" + INCLUDE, "-I" + CSRC,
 ]
-HAND_SOURCES = ["capi.hip", "packer.hip", "csr_spmm.hip", "mfma_panel.hip", "mfma_capi.hip", "probe.hip", "dense_panels.hip"]
+HAND_SOURCES = ["capi.hip", "packer.hip", "csr_spmm.hip", "mfma_panel.hip", "mfma_capi.hip", "probe.hip", "dense_panels.hip",
+                "mfma_exec.hip"]
 
 
 def _load_sibling(name):

@@ -365,6 +365,9 @@ struct srt_plan {
     int device = 0;
     std::vector<RetiredSchedule> retired;  // schedules of earlier widths, alive until destroy
     DensePart dense;                       // hybrid plans: panels on matrix cores
+    // matrix_cores: panel ranges for R ranges (the last entry is current;
+    // earlier ones stay alive for graphs that captured them)
+    std::vector<std::pair<int, int32_t *>> mfma_ranges;
 };
 
 struct PanelBuf {
@@ -424,6 +427,7 @@ void srt_tile_config_default(srt_tile_config *c) {
     c->chunk_cols = 0;
     c->b_staging = 0;
     c->layout = 0;
+    c->matrix_cores = 0;
 }
 
 // ----------------------------------------------------------------- mapping
@@ -593,6 +597,7 @@ static void realise_config(srt_tile_config &c, int M_r, int K, int K_tile, int64
     if (c.xcd_remap != 0 && c.xcd_remap != 1) c.xcd_remap = 1;
     if (!pow2_upto8(c.unroll)) c.unroll = 4;
     if (c.layout != 1 && c.layout != 2) c.layout = 2;
+    if (c.matrix_cores != 1) c.matrix_cores = 0;
     c.M_c = M_r * c.waves_per_block;
     c.N_c = 64 * c.vec_width;
     c.K_c = K_tile;
@@ -1156,6 +1161,47 @@ static int pack_csr(Packed &pk, MappingImpl &map, int m, int k, int K_c, const f
 
 extern "C" {
 
+// Matrix-core execution: the plan's mapping must be the full 4-row pattern.
+static bool full_pattern(const ExecutorDesc *e) {
+    return e && e->M_r == 4 && e->num_patterns == 1 && e->patterns[0] == 15;
+}
+
+// Panel ranges of the matrix-core executor for B width n: R contiguous
+// ranges balanced by packed columns (+4 per panel for its C rows), R x
+// (64-column slices) = one round of 256 CUs; the ranges of a width used
+// before are reused, earlier arrays stay alive until the plan is destroyed.
+static int mfma_ranges_for(srt_plan *p, int n) {
+    const int S = (std::max(n, 1) + 63) / 64;
+    const int P = p->pk.P;
+    const int R = std::max(1, std::min(P, (256 + S - 1) / S));
+    for (size_t i = 0; i < p->mfma_ranges.size(); ++i)
+        if (p->mfma_ranges[i].first == R) {
+            std::swap(p->mfma_ranges[i], p->mfma_ranges.back());
+            return SRT_OK;
+        }
+    const std::vector<int64_t> &cp = p->pk.h_col_ptr;
+    if ((int)cp.size() != P + 1) return fail(SRT_ERR_INVALID, "plan has no host column pointer");
+    auto weight = [&](int q) { return (cp[q] - cp[0]) + 4 * (int64_t)q; };
+    const int64_t total = weight(P);
+    std::vector<int32_t> rg((size_t)R + 1, 0);
+    int q = 0;
+    for (int j = 1; j < R; ++j) {
+        const int64_t target = total * j / R;
+        while (q < P && weight(q) < target) ++q;
+        rg[j] = q;
+    }
+    rg[R] = P;
+    int32_t *d = nullptr;
+    SRT_HIP_TRY(hipMalloc(&d, sizeof(int32_t) * rg.size()));
+    hipError_t e = hipMemcpy(d, rg.data(), sizeof(int32_t) * rg.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        SRT_HIP_TRY(e);
+    }
+    p->mfma_ranges.emplace_back(R, d);
+    return SRT_OK;
+}
+
 int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const void *Li,
                     int index_bytes, const srt_tile_config *cfg, int nThreads,
                     const char *executor_id, const srt_mapping *mapping, srt_stream_t stream,
@@ -1190,10 +1236,25 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
     plan->n_max = n;
     plan->cfg = plan->cfg_user;
     realise_config(plan->cfg, map.M_r, k, plan->pk.K_c, plan->pk.total_cols, std::max(n, 1));
+    if (plan->cfg.matrix_cores == 1) {
+        if (!full_pattern(exec))
+            rc = fail(SRT_ERR_UNSUPPORTED, "matrix_cores needs M_r = 4 and a mapping whose only kernel pattern is 15");
+        else if (k > mfma4_max_k())
+            rc = fail(SRT_ERR_UNSUPPORTED, "matrix_cores stages all K rows of a 64-column B slice in LDS: K <= " +
+                                               std::to_string(mfma4_max_k()));
+        else
+            rc = mfma_ranges_for(plan.get(), n);
+        if (rc) {
+            plan->pk.release();
+            for (auto &r : plan->mfma_ranges) (void)hipFree(r.second);
+            return rc;
+        }
+    }
     rc = build_schedule(plan->pk, plan->cfg, plan->exec, config_tiles(plan->cfg, std::max(n, 1)), plan->sch,
                         plan->retired, plan->cfg, (hipStream_t)stream);
     if (rc) {
         plan->pk.release();
+        for (auto &r : plan->mfma_ranges) (void)hipFree(r.second);
         return rc;
     }
     *out = plan.release();
@@ -1346,6 +1407,7 @@ int srt_plan_allocate_executor(srt_plan *p, int n) {
     srt_tile_config sch_cfg = p->cfg;
     SRT_TRY(switch_schedule(p->pk, c, tiles, p->exec, p->sch, sch_cfg, p->retired, nullptr));
     p->cfg = c;
+    if (c.matrix_cores == 1) SRT_TRY(mfma_ranges_for(p, n));
     return SRT_OK;
 }
 
@@ -1358,10 +1420,17 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
     // With every 16-row panel dense the register-tiled part is empty and the
     // dense panels write every row of C.
     const bool all_dense = p->dense.ndense > 0 && (int64_t)p->dense.ndense * 16 >= p->pk.M;
-    if (all_dense) {
+    if (all_dense || p->cfg.matrix_cores == 1) {
         if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
         if (n > 0 && (!B || !C)) return fail(SRT_ERR_INVALID, "null B or C");
         if (n > 0 && (ldb < n || ldc < n)) return fail(SRT_ERR_INVALID, "ldb and ldc must be >= n");
+        if (!all_dense && n > 0 && p->pk.M > 0) {
+            if (!p->pk.cols) return fail(SRT_ERR_INVALID, "matrix-core execution needs the column indices (released)");
+            if (p->mfma_ranges.empty()) return fail(SRT_ERR_INVALID, "no matrix-core panel ranges");
+            const auto &rg = p->mfma_ranges.back();
+            SRT_HIP_TRY(launch_mfma4_exec(p->pk.col_ptr, p->pk.cols, p->pk.vals, rg.second, rg.first, B, ldb, C,
+                                          ldc, p->pk.M, n, p->pk.K, (hipStream_t)stream));
+        }
     } else
         SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream));
     if (p->dense.ndense && n > 0)  // the dense panels' rows, zero from the register-tiled part, overwritten
@@ -1372,6 +1441,8 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
 
 int srt_plan_release_columns(srt_plan *p) {
     if (!p) return fail(SRT_ERR_INVALID, "null plan");
+    if (p->cfg.matrix_cores == 1)
+        return fail(SRT_ERR_UNSUPPORTED, "matrix-core execution reads the column indices at every launch");
     SRT_HIP_TRY(hipDeviceSynchronize());  // no launch that builds a schedule may still read them
     (void)hipFree(p->pk.cols);
     p->pk.cols = nullptr;
@@ -1442,6 +1513,7 @@ void srt_plan_destroy(srt_plan *p) {
     p->sch.release();
     for (RetiredSchedule &r : p->retired) r.sch.release();
     p->dense.release();
+    for (auto &r : p->mfma_ranges) (void)hipFree(r.second);
     delete p;
 }
 

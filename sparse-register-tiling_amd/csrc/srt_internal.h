@@ -268,6 +268,13 @@ hipError_t launch_grouped_b_prep(bool fp8, const BPrepProblem *probs, const int 
 hipError_t launch_grouped_b_widen(const BPrepProblem *probs, const int *hdr, int grid, hipStream_t s);
 int mfma_shape_tile(int shape, int *nt, int *wp, int *pw);
 
+// Full-pattern 4-row packing on matrix cores (mfma_exec.hip): panel ranges
+// [ranges[r], ranges[r + 1]) of r < R, 64-column B slices staged in LDS.
+hipError_t launch_mfma4_exec(const int64_t *col_ptr, const int32_t *cols, const float *vals,
+                             const int32_t *ranges, int R, const float *B, int64_t ldb, float *C, int64_t ldc,
+                             int M, int N, int K, hipStream_t s);
+int mfma4_max_k();  // B rows the LDS slice holds
+
 // Dense 16-row panels of hybrid plans on FP32 matrix cores (dense_panels.hip).
 hipError_t launch_dense_panels(int ndense, int G, const void *apack, const int32_t *panel_row, const float *B,
                                int64_t ldb, float *C, int64_t ldc, int M, int N, int K, hipStream_t s);

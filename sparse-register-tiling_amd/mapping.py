@@ -257,6 +257,10 @@ STRATEGIES: Dict[str, Callable[[], SOPOrig]] = {
     # M_r=4 "orig": 3-of-4 patterns + 0b1111, singles split
     # (sop_hand_configs.py:99-103, merge order that yields id da01e).
     "M4_orig": lambda: SOPOrig([14, 13, 11, 7, 0b1111], [1 << i for i in range(4)], split_cost=10000),
+    # M_r=4 "full": every non-empty pattern padded to the full 4-row column
+    # (merge pattern 0b1111 with min_merge 1, as sop_hand_configs.py:88-93
+    # builds its merge-only strategy); the packing the matrix-core executor runs.
+    "M4_full": lambda: SOPOrig([0b1111], [], split_cost=10000, min_merge=1),
     # M_r=8 "orig" (sop_hand_configs.py:209-219) and "alt" (:160-169).
     "M8_orig": lambda: SOPOrig(_M8_MERGE, [1 << i for i in range(8)], split_cost=1.1, merge_cost=1,
                                recursive=True),
@@ -264,11 +268,11 @@ STRATEGIES: Dict[str, Callable[[], SOPOrig]] = {
 }
 
 STRATEGY_M_R = {"M2_identity": 2, "M4_identity": 4, "M6_identity": 6, "M8_identity": 8,
-                "M4_orig": 4, "M8_orig": 8, "M8_alt": 8}
+                "M4_orig": 4, "M4_full": 4, "M8_orig": 8, "M8_alt": 8}
 
 # tools/paper/configs.py:84-87, plus the identity mappings of the other
 # panel heights (sbench/SOP/identity_mappings/{2,6,8}).
-NAMED_IDS = {2: {"identity": "4af51"}, 4: {"identity": "61fee", "orig": "da01e"},
+NAMED_IDS = {2: {"identity": "4af51"}, 4: {"identity": "61fee", "orig": "da01e", "full": "7da09"},
              6: {"identity": "001e3"}, 8: {"orig": "400fa", "alt": "747f9", "identity": "161fb"}}
 
 

@@ -41,9 +41,12 @@ class TileConfig:
     workgroup), xcd_remap, unroll (packed columns in flight per wave),
     nsplit (waves sharing one panel; 1 keeps the reference's single chain),
     chunk_cols (target packed columns per wave), b_staging (1: B tile
-    staged in LDS, 2: B rows read from global memory) and layout (1: panels
+    staged in LDS, 2: B rows read from global memory), layout (1: panels
     laid into workgroups in order; 2: panels binned heaviest-first into
-    whole rounds of the chip's CUs, slots given to the longest chunks)."""
+    whole rounds of the chip's CUs, slots given to the longest chunks) and
+    matrix_cores (1: one v_mfma_f32_4x4x1_16b outer product per packed
+    column; needs the M_r = 4 "full" mapping, whose only kernel pattern is
+    0b1111, and K <= 640; same bits as the register tile)."""
     M_c: int = 16
     K_c: int = 0
     N_c: int = 64
@@ -61,6 +64,7 @@ class TileConfig:
     chunk_cols: int = 0
     b_staging: int = 0
     layout: int = 0
+    matrix_cores: int = 0
 
     def to_c(self) -> _lib.TileConfigC:
         c = _lib.TileConfigC()

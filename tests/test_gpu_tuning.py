@@ -67,3 +67,15 @@ def test_tune_refines_the_fastest(srt):
     assert [r["seconds"] for r in recs[:3]] == sorted(r["seconds"] for r in recs[:3])
     assert [r["seconds"] for r in recs[3:]] == sorted(r["seconds"] for r in recs[3:])
     assert all(r["realised"]["layout"] == r["layout"] for r in recs)
+
+
+def test_add_matrix_core_candidate(srt, tmp_path):
+    """autotune.add_candidate times the matrix-core candidate and files it
+    among the refined records, sorted, with its realised configuration."""
+    at = srt.autotune
+    cache = at.TuningCache(str(tmp_path / "t.json"))
+    case = srt.workloads.SpMMCase(512, 512, 0.9, 128)
+    rec = at.add_candidate(cache, case, *at.MATRIX_CORE_CANDIDATE, reps=5)
+    assert rec["matrix_cores"] == 1 and rec["realised"]["matrix_cores"] == 1 and rec["mapping"] == "full"
+    recs = at.TuningCache(str(tmp_path / "t.json")).get(512, 512, 0.9, 128)
+    assert recs == [rec]

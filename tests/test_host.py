@@ -242,8 +242,9 @@ def test_default_tile_config(srt):
     assert (t.vec_width, t.waves_per_block, t.unroll, t.nsplit, t.chunk_cols, t.b_staging, t.layout) == (0,) * 7
     assert srt.TileConfig.from_c(t.to_c()) == t
     # the ctypes mirror has every field of srt_tile_config, in order
-    assert [f[0] for f in srt._lib.TileConfigC._fields_][-3:] == ["chunk_cols", "b_staging", "layout"]
-    assert ct.sizeof(srt._lib.TileConfigC) == 17 * 4
+    assert [f[0] for f in srt._lib.TileConfigC._fields_][-4:] == ["chunk_cols", "b_staging", "layout", "matrix_cores"]
+    assert t.matrix_cores == 0
+    assert ct.sizeof(srt._lib.TileConfigC) == 18 * 4
 
 
 def test_group_abi_rejects_bad_arguments(srt):

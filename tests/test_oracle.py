@@ -226,6 +226,29 @@ def test_csr_oracle(oracle):
     assert np.all(np.abs(C - so.dense_f64(A, B)) <= so.fp64_error_bound(A, B))
 
 
+def test_full_pattern_chain_is_the_csr_chain(oracle):
+    """Under the M_r = 4 "full" mapping every packed column carries all four
+    rows, zeros included.  fmaf(0, b, acc) = acc for finite b when acc is not
+    -0 (a chain that starts at +0 never reaches -0 in round-to-nearest), so
+    per output element the packed chain is the CSR chain over the stored
+    entries in column order: the two executors agree bit for bit."""
+    so, cb = oracle
+    rng = np.random.default_rng(21)
+    table = so.named_table(4, "full")
+    assert so.kernel_pattern_table(table) == [15] and all(v == [15] for v in table.values())
+    for (M, K, d) in [(61, 140, 0.1), (128, 64, 0.5), (9, 300, 0.02)]:
+        A = ((rng.random((M, K)) < d) * rng.standard_normal((M, K))).astype(np.float32)
+        A[A.shape[0] // 2] = 0.0  # an empty row
+        B = rng.uniform(-1, 1, (K, 40)).astype(np.float32)
+        pk = cb.pack_dense(A, 4, table)
+        assert pk["vals"].size == 4 * pk["cols"].size   # every packed column full
+        nz = A != 0
+        rp = np.concatenate([[0], np.cumsum(nz.sum(1))])
+        r, c = np.nonzero(nz)
+        C_csr = cb.csr_exec(rp, c, A[r, c], M, B)
+        assert np.array_equal(cb.exec_packed(pk, B).view(np.uint32), C_csr.view(np.uint32))
+
+
 # ---------------------------------------------------------- planted mistakes
 def test_planted_mistakes_are_caught(oracle):
     """Mutations of the oracle that a plausible bug would make must fail the
