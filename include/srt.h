@@ -486,8 +486,12 @@ typedef struct srt_exec_problem {
 
 /* One table record for plan p (B, C may be NULL with n = 0 for a template
  * whose operand fields the caller fills in later; then it must keep
- * b_bytes = K * ldb * 4 and the vec_width alignment rules: vec_width | n, ldb,
- * ldc and 4 * vec_width-byte aligned B and C). */
+ * b_bytes = K * ldb * 4 and the vec_width rules: vec_width | ldb, ldc; ldb
+ * and ldc >= n rounded up to vec_width; 4 * vec_width-byte aligned B and C).
+ * n need not be a multiple of vec_width: a lane's last vector reaches column
+ * round_up(n, vec_width) - 1, so B's columns n .. that one are read (zeros
+ * keep them harmless) and C's are overwritten; columns below n are the same
+ * bits at every lane width. */
 int srt_exec_problem_fill(const srt_plan *p, const float *B, int64_t ldb, float *C, int64_t ldc, int n,
                           int vec_width, int block_begin, srt_exec_problem *out);
 int srt_exec_persistent_execute(const srt_plan *proto, int vec_width, const srt_exec_problem *d_probs,

@@ -249,10 +249,11 @@ def counter_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 
     """rocprofv3 counter evidence for the ``top`` leading records: each runs
     in child processes under ``rocprofv3 --pmc FETCH_SIZE``, then
     ``--pmc WRITE_SIZE`` (cold rotated launches plus a calibration copy),
-    and gets ``hbm_bytes`` (HBM
-    bytes per launch, each counter scaled by what it reads for the copy of
-    known size) and ``hbm_gbps`` (those bytes over its cold time).  Records
-    the counters could not be read for are left without them."""
+    then ``--pmc MfmaUtil``, and gets ``hbm_bytes`` (HBM bytes per launch,
+    each counter scaled by what it reads for the copy of known size),
+    ``hbm_gbps`` (those bytes over its cold time) and ``mfma_util_pct``
+    (matrix-core busy share of the executor's SIMD cycles).  Records the
+    counters could not be read for are left without them."""
     import subprocess
     import tempfile
     import numpy as np
@@ -289,11 +290,25 @@ def counter_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 
             if len(scaled) != 2:
                 continue
             rec["hbm_bytes"] = int(round(scaled["FETCH_SIZE"] + scaled["WRITE_SIZE"]))
-            if log:
-                log("record %d: %d HBM bytes per launch" % (i, rec["hbm_bytes"]))
             t = rec.get("seconds_cold") or rec["seconds"]
             rec["hbm_gbps"] = round(rec["hbm_bytes"] / t / 1e9, 1)
-            rec["mfma_busy"] = 0.0   # register-tiled executors issue no MFMA
+            # Matrix-core utilisation (rocprofv3's derived MfmaUtil: MFMA-busy
+            # SIMD cycles over active cycles x SIMDs, percent), its own pass.
+            out = os.path.join(tmp, "p%d_MfmaUtil" % i)
+            cmd = [rocprof, "--pmc", "MfmaUtil", "-d", out, "-o", "run", "--output-format", "csv",
+                   "--"] + _probe_command(npz, N, rec, launches)
+            try:
+                subprocess.run(cmd, check=True, capture_output=True, timeout=timeout, cwd=_IMPORT_ROOT)
+                util = [d["MfmaUtil"] for k, d in _counter_means(out, last=launches).items()
+                        if EXEC_KERNEL in k and "MfmaUtil" in d]
+                if util:
+                    rec["mfma_util_pct"] = round(util[0], 2)
+            except (OSError, subprocess.SubprocessError) as e:
+                if log:
+                    log("counter pass MfmaUtil of record %d failed: %s" % (i, type(e).__name__))
+            if log:
+                log("record %d: %d HBM bytes per launch, MFMA util %s %%" % (i, rec["hbm_bytes"],
+                                                                          rec.get("mfma_util_pct")))
     return records
 
 
@@ -411,7 +426,7 @@ def add_candidate(cache: "TuningCache", case, mapping: Tuple[int, str], shape: D
     keys = ("b_staging", "vec_width", "unroll", "waves_per_block", "nsplit", "K_c", "layout", "matrix_cores")
     same = lambda r: r["M_r"] == M_r and r["mapping"] == name and all(r.get(k) == rec.get(k) for k in keys)
     old = [{k: v for k, v in r.items() if k not in ("evidence", "seconds_cold", "hbm_bytes", "hbm_gbps",
-                                                        "mfma_busy")} for r in old if not same(r)]
+                                                        "mfma_busy", "mfma_util_pct")} for r in old if not same(r)]
     refined = sorted([r for r in old if r.get("refined")] + [rec], key=lambda r: r["seconds"])
     rest = [r for r in old if not r.get("refined")]
     cache.put(case.M, case.K, case.sparsity, case.N, refined + rest, keep=len(refined) + len(rest))

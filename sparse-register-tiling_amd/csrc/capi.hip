@@ -1673,8 +1673,12 @@ int srt_exec_problem_fill(const srt_plan *p, const float *B, int64_t ldb, float 
         return fail(SRT_ERR_UNSUPPORTED, "the plan's schedule is not cut for unroll 4 (TileConfig.unroll = 4)");
     if (p->dense.ndense) return fail(SRT_ERR_UNSUPPORTED, "hybrid plans execute alone");
     if (n > 0 && (!B || !C)) return fail(SRT_ERR_INVALID, "null B or C");
-    if (n % V || ldb % V || ldc % V || !aligned(B, 4 * V) || !aligned(C, 4 * V))
-        return fail(SRT_ERR_INVALID, "vec_width must divide n, ldb, ldc and the operands be 4 * vec_width aligned");
+    // A lane's last vector reaches column round_up(n, V) - 1: the strides
+    // must hold it (B's columns there are read, C's are written).
+    const int64_t n_up = (int64_t)(n + V - 1) / V * V;
+    if (ldb % V || ldc % V || ldb < n_up || ldc < n_up || !aligned(B, 4 * V) || !aligned(C, 4 * V))
+        return fail(SRT_ERR_INVALID, "vec_width must divide ldb and ldc, both >= n rounded up to vec_width, and "
+                                     "the operands be 4 * vec_width aligned");
     if ((int64_t)p->pk.K * ldb * (int64_t)sizeof(float) > (int64_t)0xffffffffLL)
         return fail(SRT_ERR_UNSUPPORTED, "B larger than 4 GiB (K * ldb * 4 bytes)");
     if (p->sch.reduce && !lds_fits(false, p->pk.K, p->sch.waves_per_block, p->pk.M_r, V, 1))

@@ -175,6 +175,10 @@ class MfmaBackend:
             _lib.call("srt_mfma_problem_fill", w._h, ct.c_void_p(base), int(self.wldk[i]), None,
                       ct.c_void_p(base), 1, 1, 0, self.rec[i:i + 1].ctypes.data_as(ct.c_void_p))
 
+    @staticmethod
+    def result_ld(N: int) -> int:
+        return int(N)
+
     def operand_bytes(self, wi: int, N: int) -> int:
         return _align(N * int(self.wldk[wi]) * self.esz)
 
@@ -219,8 +223,9 @@ class RegTileWeights:
     """Register-tiled sparse weights for serving: each W rounded to BF16
     values (held as FP32), packed with one mapping and one launch shape so
     every weight runs in the same persistent grouped launch.  The launch
-    shape keeps one float per lane (any N), panel-order workgroups (a
-    schedule that serves every N) and unroll 4 (the persistent variant)."""
+    shape has four floats per lane (B and C strides rounded up to 4, so any
+    N), panel-order workgroups (a schedule that serves every N) and unroll 4
+    (the persistent variant)."""
 
     def __init__(self, weights, mapping=(4, "orig"), waves_per_block: int = 8, nsplit: int = 8,
                  chunk_cols: int = 64, n_max: int = 512):
@@ -228,7 +233,7 @@ class RegTileWeights:
         from .matmul import MatMulSpecialized, TileConfig
         from .native import resolve_mapping
         nm = resolve_mapping(named_mapping(*mapping))
-        cfg = TileConfig(vec_width=1, unroll=4, waves_per_block=waves_per_block, nsplit=nsplit,
+        cfg = TileConfig(vec_width=4, unroll=4, waves_per_block=waves_per_block, nsplit=nsplit,
                          chunk_cols=chunk_cols, layout=1, b_staging=2, xcd_remap=1)
         self.mapping = mapping
         self.plans = []
@@ -268,7 +273,7 @@ class RegTileBackend:
 
     kind = "regtile"
     prob_dtype = EXEC_PROBLEM_DTYPE
-    V = 1
+    V = 4
 
     def __init__(self, weights: RegTileWeights, grid: int):
         self.weights = weights
@@ -281,12 +286,18 @@ class RegTileBackend:
         n = len(self.plans)
         self.rec = np.zeros(n, dtype=EXEC_PROBLEM_DTYPE)
         for i, p in enumerate(self.plans):
-            _lib.call("srt_exec_problem_fill", p._h, None, 1, None, 1, 0, self.V, 0,
+            _lib.call("srt_exec_problem_fill", p._h, None, self.V, None, self.V, 0, self.V, 0,
                       self.rec[i:i + 1].ctypes.data_as(ct.c_void_p))
         self.wblocks = self.rec["n_wblocks"].astype(np.int64)
 
+    @staticmethod
+    def result_ld(N: int) -> int:
+        """Row stride of B and C: N rounded up to the lane width (columns
+        past N are zero in B and scratch in C)."""
+        return -(-int(N) // RegTileBackend.V) * RegTileBackend.V
+
     def operand_bytes(self, wi: int, N: int) -> int:
-        return _align(N * int(self.wK[wi]) * 4)
+        return _align(self.result_ld(N) * int(self.wK[wi]) * 4)
 
     def resident_bytes(self) -> int:
         return self.weights.resident_bytes()
@@ -294,23 +305,24 @@ class RegTileBackend:
     def fill(self, slot: "_Slot", wi, N, bt, cp, bp) -> None:
         n = len(wi)
         K = self.wK[wi]
+        ld = -(-N // self.V) * self.V
         blocks = self.wblocks[wi] * -(-N // (64 * self.V))
-        wtiles = -(-(K * N) // 1024)   # srt_widen_problem_tiles: the K x N output walked flat
+        wtiles = -(-(K * ld) // 1024)   # srt_widen_problem_tiles: the K x ld output walked flat
         pr = slot.prob[:n]
         pr[:] = self.rec[wi]
         pr["B"] = bt
         pr["C"] = cp
-        pr["ldb"] = N
-        pr["ldc"] = N
+        pr["ldb"] = ld
+        pr["ldc"] = ld
         pr["N"] = N
-        pr["b_bytes"] = np.maximum(K * N * 4, 4)
+        pr["b_bytes"] = np.maximum(K * ld * 4, 4)
         pr["block_begin"] = np.cumsum(blocks) - blocks
         pp = slot.prep[:n]
         pp["src"] = bp
         pp["dst"] = bt
         pp["cscale"] = 0
         pp["ld_src"] = N
-        pp["ldk"] = N
+        pp["ldk"] = ld
         pp["K"] = K
         pp["N"] = N
         pp["tile_begin"] = np.cumsum(wtiles) - wtiles
@@ -406,7 +418,8 @@ class Server:
 
     # -- admission ------------------------------------------------------------
     def request_bytes(self, req: Request) -> int:
-        return self.backend.operand_bytes(req.weight, req.N) + int(self.backend.wM[req.weight]) * req.N * 4
+        return (self.backend.operand_bytes(req.weight, req.N)
+                + int(self.backend.wM[req.weight]) * self.backend.result_ld(req.N) * 4)
 
     def admit(self, req: Request) -> bool:
         """Reserve the request's operand and result in the budget (False = wait)."""
@@ -423,7 +436,8 @@ class Server:
         return True
 
     def result_view(self, req: Request) -> torch.Tensor:
-        return self.pool.view(req._c_off, (int(self.backend.wM[req.weight]), req.N), torch.float32)
+        ld = self.backend.result_ld(req.N)
+        return self.pool.view(req._c_off, (int(self.backend.wM[req.weight]), ld), torch.float32)[:, :req.N]
 
     def release(self, req: Request, keep_result: bool = False, stream: Optional[torch.cuda.Stream] = None) -> None:
         """Return the request's memory to the budget.  With keep_result the
