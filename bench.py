@@ -20,6 +20,10 @@ Prints one JSON line with the driver's fields plus:
   cpu_baseline  the C restatement of the reference's nano-kernel executor
                 (oracle/srt_oracle.c, OpenMP over panels) on the host cores,
                 same matrix and formula; MKL CSR (torch CPU sparse mm) beside it
+  serving       served p50/p99 latency and requests/s against one plain FP32
+                launch per request, 48 resident 90%-pruned weights, for the
+                register-tiled sparse served form; ``serving_bf16_mfma`` the
+                same for the 16-row dense BF16 panels on MFMA
 
 Profiling helpers (used to make the files under profiles/):
   --config FILE / --save-config FILE   fix / record the tuned configuration
@@ -968,17 +972,21 @@ def main():
     if not args.no_serving:
         del plans, Bs, Cs, g_steps, g_warm, g_hot
         torch.cuda.empty_cache()
-        sv = serving_comparison([float(q) for q in args.serving_qps.split(",")], args.serving_requests,
-                                args.slo_ms, args.mfma_shape, accuracy_sample=8)
-        out["serving"] = {k: sv[k] for k in ("served_form", "weights", "requests_per_level", "slo_ms",
-                                             "max_qps_within_slo", "throughput_ratio_at_equal_p99",
-                                             "p99_ms_at_slo_load", "saturation_req_per_s", "saturation_ratio",
-                                             "served_rel_err_vs_fp64_max", "weight_nnz_bytes_bf16",
-                                             "resident_weight_bytes", "dense_weight_bytes_bf16")}
-        out["serving"]["p50_p99_ms"] = [{"offered_qps": l["offered_qps"],
-                                         "served": [l["served"]["p50_ms"], l["served"]["p99_ms"]],
-                                         "plain_fp32": [l["plain_fp32"]["p50_ms"], l["plain_fp32"]["p99_ms"]]}
-                                        for l in sv["levels"]]
+        # both served forms: register-tiled sparse weights ("serving") and
+        # 16-row dense BF16 panels on MFMA ("serving_bf16_mfma")
+        for key, form in (("serving", "regtile"), ("serving_bf16_mfma", "mfma")):
+            sv = serving_comparison([float(q) for q in args.serving_qps.split(",")], args.serving_requests,
+                                    args.slo_ms, args.mfma_shape, accuracy_sample=8, form=form)
+            out[key] = {k: sv[k] for k in ("served_form", "weights", "requests_per_level", "slo_ms",
+                                           "max_qps_within_slo", "throughput_ratio_at_equal_p99",
+                                           "p99_ms_at_slo_load", "saturation_req_per_s", "saturation_ratio",
+                                           "served_rel_err_vs_fp64_max", "weight_nnz_bytes_bf16",
+                                           "resident_weight_bytes", "dense_weight_bytes_bf16")}
+            out[key]["p50_p99_ms"] = [{"offered_qps": l["offered_qps"],
+                                       "served": [l["served"]["p50_ms"], l["served"]["p99_ms"]],
+                                       "plain_fp32": [l["plain_fp32"]["p50_ms"], l["plain_fp32"]["p99_ms"]]}
+                                      for l in sv["levels"]]
+            torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
 
