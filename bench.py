@@ -292,13 +292,15 @@ def choose_config(args, srt, csr, N, Bd):
             csr.shape[0], csr.shape[1], 1.0 - csr.values().numel() / float(csr.shape[0] * csr.shape[1]), N)
         (M_r, name), cfg = autotune.best_config(recs)
         tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
-                                             "chunk_cols", "xcd_remap", "b_staging", "K_c", "layout")}
+                                             "chunk_cols", "xcd_remap", "b_staging", "K_c", "layout",
+                                             "matrix_cores")}
         log("shipped tuning record:", recs[0])
     elif args.mapping == "auto":
         recs = autotune.tune(csr, N, reps=args.tune_reps, B=Bd)
         (M_r, name), cfg = autotune.best_config(recs)
         tile = {k: getattr(cfg, k) for k in ("vec_width", "unroll", "waves_per_block", "nsplit",
-                                             "chunk_cols", "xcd_remap", "b_staging", "K_c", "layout")}
+                                             "chunk_cols", "xcd_remap", "b_staging", "K_c", "layout",
+                                             "matrix_cores")}
         tune_top = [{k: (round(v * 1e6, 3) if k == "seconds" else v) for k, v in r.items()}
                     for r in recs[:5]]
         log("autotune best (us):", tune_top[0])

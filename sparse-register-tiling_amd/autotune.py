@@ -141,10 +141,11 @@ MALL_BYTES = 256 << 20
 
 
 def time_plan_cold(csr: torch.Tensor, N: int, mapping, cfg: TileConfig, reps: int = 60,
-                   B: Optional[torch.Tensor] = None) -> float:
+                   B: Optional[torch.Tensor] = None, blocks: int = 3, rounds: int = 10) -> float:
     """Seconds per launch with every launch cold in cache: launches rotate
     over R replicas of (packed A, B, C) whose footprint exceeds twice the
-    256 MB last-level cache, captured in one graph (the headline protocol)."""
+    256 MB last-level cache, captured in one graph of ``rounds`` rounds of
+    the replicas (the headline protocol); median over ``blocks`` replays."""
     M, K = csr.shape
     if B is None:
         B = torch.rand((K, N), device="cuda") * 2 - 1
@@ -162,7 +163,7 @@ def time_plan_cold(csr: torch.Tensor, N: int, mapping, cfg: TileConfig, reps: in
         i[0] += 1
         plans[j](Cs[j], Bs[j])
 
-    t = time_graph(launch, reps=max(reps, R), blocks=1)
+    t = time_graph(launch, reps=max(reps, rounds * R), blocks=blocks)
     for p in plans:
         p.close()
     return t
@@ -314,16 +315,21 @@ def counter_evidence(csr: torch.Tensor, N: int, records: List[Dict], top: int = 
 
 def rank_with_evidence(records: List[Dict], slack: float = 0.03) -> List[Dict]:
     """Order records by cold time; among those within ``slack`` of the
-    fastest, the one moving the fewest HBM bytes leads (the same speed with
-    less traffic leaves more of the memory system to concurrent work)."""
+    fastest that carry counter evidence, the one with the lowest
+    cold time x sqrt(HBM bytes / fewest bytes among them) leads: less traffic
+    wins a near-tie only when it saves at least about twice the relative time
+    it costs (the same speed with less traffic leaves more of the memory
+    system to concurrent work)."""
+    import math
     key = lambda r: r.get("seconds_cold", r["seconds"])
     recs = sorted(records, key=key)
     if not recs:
         return recs
     best = key(recs[0])
-    near = [r for r in recs if key(r) <= best * (1 + slack) and "hbm_bytes" in r]
+    near = [r for r in recs if key(r) <= best * (1 + slack) and r.get("hbm_bytes")]
     if near:
-        lead = min(near, key=lambda r: (r["hbm_bytes"], key(r)))
+        fewest = min(r["hbm_bytes"] for r in near)
+        lead = min(near, key=lambda r: (key(r) * math.sqrt(r["hbm_bytes"] / fewest), key(r)))
         recs.remove(lead)
         recs.insert(0, lead)
     return recs

@@ -271,15 +271,21 @@ def test_sparsity_bucket_rounds_pruning_counts(srt):
 
 
 def test_rank_with_evidence_prefers_less_traffic_within_slack(srt):
-    """Cold time orders the records; inside 3 % of the fastest the one with
-    the fewest HBM bytes leads; records without counters never jump."""
+    """Cold time orders the records; inside 3 % of the fastest, cold time x
+    sqrt(HBM bytes / fewest) picks the leader; records without counters
+    never jump."""
     at = srt.autotune
     recs = [dict(seconds=1.0, seconds_cold=10.0e-6, hbm_bytes=9_000_000, id="a"),
             dict(seconds=1.0, seconds_cold=10.2e-6, hbm_bytes=6_000_000, id="b"),
             dict(seconds=1.0, seconds_cold=10.25e-6, id="c"),
             dict(seconds=1.0, seconds_cold=12.0e-6, hbm_bytes=1_000_000, id="d")]
+    # b: 2 % slower, a third less traffic -> leads
     assert [r["id"] for r in at.rank_with_evidence(recs)] == ["b", "a", "c", "d"]
-    recs[1]["seconds_cold"] = 10.4e-6
+    # b: 1 % slower but only 1 % less traffic -> the fastest keeps the lead
+    recs[1].update(seconds_cold=10.1e-6, hbm_bytes=8_910_000)
+    assert [r["id"] for r in at.rank_with_evidence(recs)] == ["a", "b", "c", "d"]
+    # b beyond the slack: time alone
+    recs[1].update(seconds_cold=10.4e-6, hbm_bytes=1_000_000)
     assert [r["id"] for r in at.rank_with_evidence(recs)] == ["a", "c", "b", "d"]
 
 
