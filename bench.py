@@ -608,9 +608,10 @@ def bench_sweep(args):
         # the configuration of the shipped tuning table (its lookup is what a
         # caller gets from autotune.tuned_plan)
         plan = at.tuned_plan(csr_t, case.N)
+        recs = at.TuningCache(at.SHIPPED).get(case.M, case.K, case.sparsity, case.N)
         t = at.time_graph(lambda: plan(C, Bd), reps=50)
         real = plan.get_config()
-        best = (t, (plan.mapping.M_r, "orig"), real)
+        best = (t, (plan.mapping.M_r, recs[0]["mapping"] if recs else plan.mapping.mapping.id), real)
         plan(C, Bd)
         parts = srt.csr._parts(csr_t)
         Ccsr = torch.empty_like(C)
