@@ -9,10 +9,11 @@
 //   A[row = l % 16][k = 16 g + 4 j + l / 16],  j = 0..3,
 // as one 16-byte word (one load per lane per K group).
 //
-// A workgroup (8 waves, two per SIMD) owns 4 dense panels x 32 columns of C.
-// K advances in stages of 128 (8 groups); wave w computes panel w % 4 over
-// groups 4 (w / 4) .. 4 (w / 4) + 3 of every stage, two 16x16 accumulator
-// tiles, and the two halves are added at the end (low half + high half).
+// A workgroup (16 waves, four per SIMD) owns 4 dense panels x 32 columns of
+// C.  K advances in stages of 128 (8 groups); wave w computes panel w % 4
+// over groups 2 (w / 4), 2 (w / 4) + 1 of every stage (its K quarter), two
+// 16x16 accumulator tiles, and the quarters are added at the end in order
+// ((q0 + q1) + q2) + q3.
 // The stage's B block (128 rows x 32 columns) is staged in LDS in MFMA
 // B-operand order, so each lane reads the four k-steps of a group and tile as
 // one 16-byte word:
@@ -20,7 +21,7 @@
 // Stages are double-buffered: the next stage's B rows and A words are loaded
 // into registers while the current stage's MFMAs run, then written to the
 // other LDS buffer; one barrier per stage.  Accumulation is FP32 in the
-// matrix core, k ascending in steps of 4 within each half.
+// matrix core, k ascending in steps of 4 within each quarter.
 //
 // Workgroups map to (group of 4 panels, 32-column slice) so that the blocks
 // the dispatcher sends to one XCD share 4 panel groups' A and 8 slices' B in
@@ -38,18 +39,19 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNT = 32;       // C columns per workgroup (2 MFMA N tiles)
 constexpr int kPanels = 4;    // panels per workgroup
-constexpr int kWaves = 8;     // two K halves per panel
+constexpr int kSplit = 4;     // K quarters per panel
+constexpr int kWaves = kPanels * kSplit;
 constexpr int kStageG = 8;    // K groups of 16 per stage
-constexpr int kHalfG = kStageG / 2;
+constexpr int kPartG = kStageG / kSplit;  // groups of a stage per wave
 constexpr int kStageK = 16 * kStageG;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kXcds = 8;
 // B staging: a thread loads rows 16 q + 4 j + kr (j = 0..3) of one column and
 // writes them as one 16-byte LDS word; (q, kr) pairs per stage: 4 kStageG.
-constexpr int kBWords = kStageG * 4 * kNT / kThreads;  // 16-byte words per thread per stage (2)
+constexpr int kBWords = kStageG * 4 * kNT / kThreads;  // 16-byte words per thread per stage (1)
 
 struct AStage {
-    f32x4 w[kHalfG];
+    f32x4 w[kPartG];
 };
 
 __global__ void __launch_bounds__(kThreads)
@@ -67,7 +69,7 @@ __global__ void __launch_bounds__(kThreads)
     if (pg * kPanels >= ndense) return;  // whole workgroup idle (uniform)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int half = wave / kPanels;
+    const int part = wave / kPanels;
     const int d = pg * kPanels + wave % kPanels;  // dense panel of this wave (may be past the list)
     const bool live = d < ndense;
     const int n0 = nt * kNT;
@@ -82,8 +84,8 @@ __global__ void __launch_bounds__(kThreads)
     const int bc = threadIdx.x % kNT;  // staged column
     const int bn = n0 + bc;
     const int bt_t = bc / 16, bt_jj = bc % 16;
-    f32x4 breg[kBWords];
-    auto load_b = [&](int s) {
+    typedef f32x4 BRegs[kBWords];
+    auto load_b = [&](BRegs &breg, int s) {
 #pragma unroll
         for (int e = 0; e < kBWords; ++e) {
             const int idx = threadIdx.x / kNT + (kThreads / kNT) * e;  // (q, kr) pair
@@ -97,7 +99,7 @@ __global__ void __launch_bounds__(kThreads)
             }
         }
     };
-    auto write_b = [&](int buf) {
+    auto write_b = [&](const BRegs &breg, int buf) {
 #pragma unroll
         for (int e = 0; e < kBWords; ++e) {
             const int idx = threadIdx.x / kNT + (kThreads / kNT) * e;
@@ -108,8 +110,8 @@ __global__ void __launch_bounds__(kThreads)
     const uint32_t abase = ((uint32_t)(live ? d : 0) * (uint32_t)G * 64u + (uint32_t)lane) * 16u;
     auto load_a = [&](AStage &a, int s) {
 #pragma unroll
-        for (int q = 0; q < kHalfG; ++q) {
-            const int g = s * kStageG + half * kHalfG + q;
+        for (int q = 0; q < kPartG; ++q) {
+            const int g = s * kStageG + part * kPartG + q;
             const uint32_t off = (live & (g < G)) ? abase + (uint32_t)g * 1024u : 0xffffffffu;
             a.w[q] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(arsrc, off, 0, 0));
         }
@@ -118,15 +120,15 @@ __global__ void __launch_bounds__(kThreads)
     acc[0] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     acc[1] = acc[0];
     auto compute = [&](const AStage &a, int buf) {
-        // all of the half-stage's B words first (one LDS latency per stage)
-        f32x4 bw[kHalfG][2];
+        // all of the wave's B words of the stage first (one LDS latency per stage)
+        f32x4 bw[kPartG][2];
 #pragma unroll
-        for (int q = 0; q < kHalfG; ++q) {
-            bw[q][0] = bt[buf][half * kHalfG + q][0][lane];
-            bw[q][1] = bt[buf][half * kHalfG + q][1][lane];
+        for (int q = 0; q < kPartG; ++q) {
+            bw[q][0] = bt[buf][part * kPartG + q][0][lane];
+            bw[q][1] = bt[buf][part * kPartG + q][1][lane];
         }
 #pragma unroll
-        for (int q = 0; q < kHalfG; ++q) {
+        for (int q = 0; q < kPartG; ++q) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w[q][j], bw[q][0][j], acc[0], 0, 0, 0);
@@ -134,50 +136,55 @@ __global__ void __launch_bounds__(kThreads)
             }
         }
         // keep that order: the LDS reads as one group, then the MFMAs
-        __builtin_amdgcn_sched_group_barrier(0x0100, 2 * kHalfG, 0);
-        __builtin_amdgcn_sched_group_barrier(0x0008, 8 * kHalfG, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0100, 2 * kPartG, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0008, 8 * kPartG, 0);
     };
 
-    // Two stages per iteration so the A register sets alternate without
-    // copies.  Stages past the last (at most one, to make the count even) are
-    // loaded, written and computed like the others: their offsets are out of
-    // range and read zeros, and adding those zero products leaves every
-    // accumulator bit-identical (it starts at +0, so it is never -0).  The
-    // loop therefore has a single path, and each of the compiler's waits is
-    // for loads issued one stage earlier.
+    // Two stages per iteration so the B register sets and LDS buffers
+    // alternate without copies.  Stages past the last (at most one, to make
+    // the count even) are loaded, written and computed like the others: their
+    // offsets are out of range and read zeros, and adding those zero products
+    // leaves every accumulator bit-identical (it starts at +0, so it is never
+    // -0).  The loop therefore has a single path, and each of the compiler's
+    // waits is for loads issued one stage earlier.
     AStage a0, a1;
-    load_b(0);
+    BRegs b0, b1;
+    load_b(b0, 0);
     load_a(a0, 0);
-    write_b(0);
-    load_b(1);
+    write_b(b0, 0);
+    load_b(b1, 1);
     load_a(a1, 1);
     __syncthreads();
     for (int s = 0; s < S; s += 2) {
         compute(a0, 0);
         __builtin_amdgcn_sched_barrier(0);
-        write_b(1);
-        load_b(s + 2);
+        write_b(b1, 1);
+        load_b(b0, s + 2);
         load_a(a0, s + 2);
         __syncthreads();
         compute(a1, 1);
         __builtin_amdgcn_sched_barrier(0);
-        write_b(0);
-        load_b(s + 3);
+        write_b(b0, 0);
+        load_b(b1, s + 3);
         load_a(a1, s + 3);
         __syncthreads();
     }
-    // high half -> LDS (stage buffer 0 is free after the last barrier), low
-    // half adds it and stores
+    // quarters 1..3 -> LDS (the stage buffers are free after the last
+    // barrier), quarter 0 adds them in order and stores
     f32x4 *red = &bt[0][0][0][0];
-    const int slot = (wave % kPanels) * 2 * 64 + lane;
-    if (half == 1) {
+    const int slot = ((part - 1) * kPanels + wave % kPanels) * 2 * 64 + lane;
+    if (part > 0) {
         red[slot] = acc[0];
         red[slot + 64] = acc[1];
     }
     __syncthreads();
-    if (half == 1 || !live) return;
-    acc[0] += red[slot];
-    acc[1] += red[slot + 64];
+    if (part > 0 || !live) return;
+#pragma unroll
+    for (int q = 1; q < kSplit; ++q) {
+        const int o = ((q - 1) * kPanels + wave % kPanels) * 2 * 64 + lane;
+        acc[0] += red[o];
+        acc[1] += red[o + 64];
+    }
     const int row0 = panel_row[d];
     // D layout: lane l holds rows 4 * (l / 16) + r of column l % 16.
 #pragma unroll
