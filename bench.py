@@ -72,7 +72,7 @@ def parse():
     ap.add_argument("--form", default="plain",
                     choices=["plain", "precision", "serving", "cnn", "sweep", "floor", "hybrid"],
                     help="plain = the headline FP32 workload (default); the others print their own line")
-    ap.add_argument("--qps", default="5000,10000,20000,40000,80000,160000",
+    ap.add_argument("--qps", default="5000,10000,20000,40000,60000,80000,160000",
                     help="serving: offered loads (requests/s)")
     ap.add_argument("--requests", type=int, default=1500, help="serving: requests per load level")
     ap.add_argument("--slo-ms", type=float, default=2.0, help="serving: p99 target for the throughput ratio")

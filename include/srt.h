@@ -452,6 +452,10 @@ typedef struct srt_bprep_problem {
     int K, N;
     int tile_begin;
     int src_bf16;   /* 1: src holds BF16, 0: fp32 */
+    int pad_to;     /* widening only: columns written per row, N <= pad_to <= ldk
+                       ([N, pad_to) zero; columns from pad_to on untouched, so
+                       several problems can share one destination); 0 = ldk */
+    int reserved;
 } srt_bprep_problem;
 
 int srt_bprep_problem_tiles(const srt_bprep_problem *pr);
@@ -499,8 +503,9 @@ int srt_exec_persistent_execute(const srt_plan *proto, int vec_width, const srt_
 
 /* Grouped BF16 -> FP32 widening of the requests' B operands (same table and
  * header convention, srt_bprep_problem with ldk = the FP32 row stride,
- * columns [N, ldk) written as zero; src_bf16 must be 1).  A problem's tiles
- * are 1024-element pieces of its K x ldk output: ceil(K * ldk / 1024). */
+ * columns [N, pad_to) written as zero, pad_to = 0 meaning ldk; src_bf16 must
+ * be 1).  A problem's tiles are 1024-element pieces of its K x pad_to output:
+ * ceil(K * pad_to / 1024). */
 int srt_widen_problem_tiles(const srt_bprep_problem *pr);
 int srt_exec_grouped_b_widen(const srt_bprep_problem *d_probs, const int *d_hdr, int grid, srt_stream_t stream);
 

@@ -343,8 +343,9 @@ int srt_bprep_problem_tiles(const srt_bprep_problem *pr) {
 
 int srt_widen_problem_tiles(const srt_bprep_problem *pr) {
     if (!pr) return -1;
-    if (pr->K <= 0 || pr->ldk <= 0) return 0;
-    return (int)(((int64_t)pr->K * pr->ldk + 1023) / 1024);
+    const int64_t w = pr->pad_to > 0 ? pr->pad_to : pr->ldk;
+    if (pr->K <= 0 || w <= 0) return 0;
+    return (int)(((int64_t)pr->K * w + 1023) / 1024);
 }
 
 int srt_exec_grouped_b_widen(const srt_bprep_problem *d_probs, const int *d_hdr, int grid, srt_stream_t stream) {

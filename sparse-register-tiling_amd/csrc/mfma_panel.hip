@@ -622,28 +622,30 @@ __global__ void grouped_transpose_kernel(ConstBPrepProblems probs, const int *__
 }
 
 // Grouped widening for the register-tile serving form: every problem's BF16
-// B (K x N, row-major, ld_src) into FP32 K x ldk row-major (dst), columns
-// [N, ldk) zero.  The K x ldk output is walked flat, 1024 elements per tile
-// (4 per thread, 256 apart): problem q owns ceil(K * ldk / 1024) tiles from
-// tile_begin, whatever N is.  BF16 -> FP32 is exact.
+// B (K x N, row-major, ld_src) into FP32 rows of stride ldk (dst), columns
+// [N, w) zero, w = pad_to (ldk when 0); columns from w on are not touched, so
+// problems may write side by side into one destination.  The K x w output is
+// walked flat, 1024 elements per tile (4 per thread, 256 apart): problem q
+// owns ceil(K * w / 1024) tiles from tile_begin.  BF16 -> FP32 is exact.
 __global__ void __launch_bounds__(256) grouped_widen_kernel(ConstBPrepProblems probs, const int *__restrict__ hdr) {
     const int nprob = hdr[0], total_tiles = hdr[1];
     int q = 0;
     for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
         while (q + 1 < nprob && probs[q + 1].tile_begin <= t) ++q;
         const int64_t ldk = probs[q].ldk;
-        const int64_t total = (int64_t)probs[q].K * ldk;
+        const int64_t w = probs[q].pad_to > 0 ? probs[q].pad_to : ldk;
+        const int64_t total = (int64_t)probs[q].K * w;
         const int64_t e0 = (int64_t)(t - probs[q].tile_begin) * 1024 + threadIdx.x;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t e = e0 + 256 * u;
             if (e < total) {
-                const int64_t k = e / ldk, n = e - k * ldk;
+                const int64_t k = e / w, n = e - k * w;
                 float v = 0.0f;
                 if (n < probs[q].N)
                     v = __builtin_bit_cast(
                         float, (uint32_t)static_cast<const uint16_t *>(probs[q].src)[k * probs[q].ld_src + n] << 16);
-                static_cast<float *>(probs[q].dst)[e] = v;
+                static_cast<float *>(probs[q].dst)[k * ldk + n] = v;
             }
         }
     }

@@ -260,6 +260,8 @@ struct BPrepProblem {
     int K, N;
     int tile_begin;
     int src_bf16;
+    int pad_to;  // widening: columns written per row (0 = ldk)
+    int reserved;
 };
 // hdr (device): {number of problems, total tiles}.
 hipError_t launch_mfma_grouped(bool fp8, int shape, const MfmaProblem *probs, const int *hdr, int grid,

@@ -56,8 +56,9 @@ PROBLEM_DTYPE = np.dtype([("blocks", "<u8"), ("Bt", "<u8"), ("C", "<u8"), ("rsca
                           ("ldb", "<i8"), ("ldc", "<i8"), ("M", "<i4"), ("N", "<i4"), ("KB", "<i4"), ("P", "<i4"),
                           ("tile_begin", "<i4"), ("vec_ok", "<i4")])
 BPREP_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("cscale", "<u8"), ("ld_src", "<i8"), ("ldk", "<i8"),
-                        ("K", "<i4"), ("N", "<i4"), ("tile_begin", "<i4"), ("src_bf16", "<i4")])
-assert PROBLEM_DTYPE.itemsize == 80 and BPREP_DTYPE.itemsize == 56
+                        ("K", "<i4"), ("N", "<i4"), ("tile_begin", "<i4"), ("src_bf16", "<i4"),
+                        ("pad_to", "<i4"), ("reserved", "<i4")])
+assert PROBLEM_DTYPE.itemsize == 80 and BPREP_DTYPE.itemsize == 64
 
 
 class SlabPool:
@@ -126,6 +127,7 @@ class Request:
     _bytes: int = 0
     _bt_off: int = 0   # slab offsets of the transposed operand and the result
     _c_off: int = 0
+    _group: Optional["_Merged"] = None   # set while the request runs merged with others of its weight
 
     @property
     def N(self) -> int:
@@ -210,6 +212,7 @@ class MfmaBackend:
         pp["N"] = N
         pp["tile_begin"] = np.cumsum(ptiles) - ptiles
         pp["src_bf16"] = 1
+        pp["pad_to"] = 0
         slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()))
 
     def enqueue(self, slot: "_Slot", s: ct.c_void_p) -> None:
@@ -228,7 +231,7 @@ class RegTileWeights:
     (the persistent variant)."""
 
     def __init__(self, weights, mapping=(4, "orig"), waves_per_block: int = 8, nsplit: int = 8,
-                 chunk_cols: int = 64, n_max: int = 512):
+                 chunk_cols: int = 128, n_max: int = 512):
         from .mapping import named_mapping
         from .matmul import MatMulSpecialized, TileConfig
         from .native import resolve_mapping
@@ -267,6 +270,18 @@ class RegTileWeights:
                 "other": sum(p.require_storage - 4 * s["packed_values"] for p, s in zip(self.plans, st))}
 
 
+class _Merged:
+    """Requests of one weight run as one SpMM over their B columns side by
+    side: one slab block holds the merged FP32 B (K x ld) and C (M x ld);
+    member i owns columns [col[i], col[i] + N_i); the block is freed with the
+    last member."""
+
+    def __init__(self, off: int, nbytes: int, b_bytes: int, ld: int, members: List["Request"], cols: List[int]):
+        self.off, self.nbytes, self.b_bytes, self.ld = off, nbytes, b_bytes, ld
+        self.col = {id(r): c for r, c in zip(members, cols)}
+        self.refs = len(members)
+
+
 class RegTileBackend:
     """Register-tiled sparse panels (``RegTileWeights``): BF16 B widened to
     FP32 on the device, one persistent grouped executor launch per batch."""
@@ -275,10 +290,11 @@ class RegTileBackend:
     prob_dtype = EXEC_PROBLEM_DTYPE
     V = 4
 
-    def __init__(self, weights: RegTileWeights, grid: int):
+    def __init__(self, weights: RegTileWeights, grid: int, merge: bool = True):
         self.weights = weights
         self.plans = weights.plans
         self.grid = int(grid)
+        self.merge = bool(merge)
         self.wM = np.array([p.m for p in self.plans], dtype=np.int64)
         self.wK = np.array([p.k for p in self.plans], dtype=np.int64)
 
@@ -301,6 +317,88 @@ class RegTileBackend:
 
     def resident_bytes(self) -> int:
         return self.weights.resident_bytes()
+
+    def fill_batch(self, server: "Server", slot: "_Slot", batch: List["Request"]) -> None:
+        """Problem tables of one batch.  Requests of the same weight are
+        merged (``merge``): one slab block takes their B columns side by side
+        and their results, and one SpMM serves them all -- a single pass over
+        the weight's packed columns instead of one per request.  Each column's
+        result is the same bits as alone.  Members give back their own blocks
+        (first, when the budget is full); when the merged block does not fit
+        even then, they run alone."""
+        if not self.merge:
+            return self.fill(slot, *server._batch_arrays(batch))
+        groups: Dict[int, List[Request]] = {}
+        for r in batch:
+            groups.setdefault(r.weight, []).append(r)
+        pool = server.pool
+        rows = []     # (weight, N, ld, B addr, C addr) per SpMM
+        prep = []     # (src, dst, ldk, pad_to, K, N) per request
+        for w, members in groups.items():
+            K, M = int(self.wK[w]), int(self.wM[w])
+            if len(members) > 1:
+                lds = [self.result_ld(r.N) for r in members]
+                ld = sum(lds)
+                b_bytes = _align(K * ld * 4)
+                nbytes = b_bytes + M * ld * 4
+                off = pool.alloc(nbytes)
+                if off is None:
+                    # a full budget: the members' own blocks (nothing written
+                    # to them yet) make room; if the merged block still does
+                    # not fit, the members take blocks of the same classes back
+                    for r in members:
+                        pool.free(r._off, r._bytes)
+                        r._off = -1
+                    off = pool.alloc(nbytes)
+                    if off is None:
+                        for r in members:
+                            if not server._place(r):
+                                raise RuntimeError("serving budget: a freed request block could not be re-placed")
+                if off is not None:
+                    cols = list(np.cumsum(lds) - lds)
+                    g = _Merged(off, nbytes, b_bytes, ld, members, [int(c) for c in cols])
+                    for r, c, l in zip(members, g.col.values(), lds):
+                        if r._off >= 0:
+                            pool.free(r._off, r._bytes)
+                        r._off, r._group = -1, g
+                        prep.append((r.B.data_ptr(), pool.base + off + 4 * c, ld, l, K, r.N))
+                    rows.append((w, ld, ld, pool.base + off, pool.base + off + b_bytes))
+                    continue
+            for r in members:
+                ld = self.result_ld(r.N)
+                prep.append((r.B.data_ptr(), pool.base + r._bt_off, ld, ld, K, r.N))
+                rows.append((w, r.N, ld, pool.base + r._bt_off, pool.base + r._c_off))
+        n = len(rows)
+        wi = np.fromiter((x[0] for x in rows), dtype=np.int64, count=n)
+        N = np.fromiter((x[1] for x in rows), dtype=np.int64, count=n)
+        ld = np.fromiter((x[2] for x in rows), dtype=np.int64, count=n)
+        K = self.wK[wi]
+        blocks = self.wblocks[wi] * -(-N // (64 * self.V))
+        pr = slot.prob[:n]
+        pr[:] = self.rec[wi]
+        pr["B"] = np.fromiter((x[3] for x in rows), dtype=np.int64, count=n)
+        pr["C"] = np.fromiter((x[4] for x in rows), dtype=np.int64, count=n)
+        pr["ldb"] = ld
+        pr["ldc"] = ld
+        pr["N"] = N
+        pr["b_bytes"] = np.maximum(K * ld * 4, 4)
+        pr["block_begin"] = np.cumsum(blocks) - blocks
+        m = len(prep)
+        pK = np.fromiter((x[4] for x in prep), dtype=np.int64, count=m)
+        pad = np.fromiter((x[3] for x in prep), dtype=np.int64, count=m)
+        wtiles = -(-(pK * pad) // 1024)   # srt_widen_problem_tiles: K x pad_to walked flat
+        pp = slot.prep[:m]
+        pp["src"] = np.fromiter((x[0] for x in prep), dtype=np.int64, count=m)
+        pp["dst"] = np.fromiter((x[1] for x in prep), dtype=np.int64, count=m)
+        pp["cscale"] = 0
+        pp["ld_src"] = np.fromiter((x[5] for x in prep), dtype=np.int64, count=m)
+        pp["ldk"] = np.fromiter((x[2] for x in prep), dtype=np.int64, count=m)
+        pp["K"] = pK
+        pp["N"] = pp["ld_src"]
+        pp["tile_begin"] = np.cumsum(wtiles) - wtiles
+        pp["src_bf16"] = 1
+        pp["pad_to"] = pad
+        slot.hdr[:] = (m, int(wtiles.sum()), n, int(blocks.sum()))
 
     def fill(self, slot: "_Slot", wi, N, bt, cp, bp) -> None:
         n = len(wi)
@@ -327,6 +425,7 @@ class RegTileBackend:
         pp["N"] = N
         pp["tile_begin"] = np.cumsum(wtiles) - wtiles
         pp["src_bf16"] = 1
+        pp["pad_to"] = 0
         slot.hdr[:] = (n, int(wtiles.sum()), n, int(blocks.sum()))
 
     def enqueue(self, slot: "_Slot", s: ct.c_void_p) -> None:
@@ -389,11 +488,12 @@ class Server:
     persistent launches; ``shape``: MFMA launch shape (MFMA weights only)."""
 
     def __init__(self, weights, budget_bytes: int = 256 << 20, max_batch: int = 64,
-                 slots: int = 3, shape: int = 6, grid: int = 1024, use_graphs: bool = True):
+                 slots: int = 3, shape: int = 6, grid: int = 1024, use_graphs: bool = True,
+                 merge_same_weight: bool = True):
         if weights is None or len(weights) == 0:
             raise ValueError("no weights")
         if isinstance(weights, RegTileWeights):
-            self.backend = RegTileBackend(weights, grid)
+            self.backend = RegTileBackend(weights, grid, merge=merge_same_weight)
             self.dtype = "bf16"
         else:
             self.backend = MfmaBackend(weights, shape, grid)
@@ -426,6 +526,9 @@ class Server:
         K = int(self.backend.wK[req.weight])
         if req.B.shape[0] != K or req.B.dtype != torch.bfloat16 or not req.B.is_contiguous():
             raise ValueError("request B must be a contiguous K x N BF16 tensor for weight %d" % req.weight)
+        return self._place(req)
+
+    def _place(self, req: Request) -> bool:
         nbytes = self.request_bytes(req)
         off = self.pool.alloc(nbytes)
         if off is None:
@@ -436,8 +539,13 @@ class Server:
         return True
 
     def result_view(self, req: Request) -> torch.Tensor:
+        M = int(self.backend.wM[req.weight])
+        g = req._group
+        if g is not None:   # merged: the request's columns of the block's C
+            c = g.col[id(req)]
+            return self.pool.view(g.off + g.b_bytes, (M, g.ld), torch.float32)[:, c:c + req.N]
         ld = self.backend.result_ld(req.N)
-        return self.pool.view(req._c_off, (int(self.backend.wM[req.weight]), ld), torch.float32)[:, :req.N]
+        return self.pool.view(req._c_off, (M, ld), torch.float32)[:, :req.N]
 
     def release(self, req: Request, keep_result: bool = False, stream: Optional[torch.cuda.Stream] = None) -> None:
         """Return the request's memory to the budget.  With keep_result the
@@ -447,18 +555,31 @@ class Server:
             with torch.cuda.stream(stream or torch.cuda.current_stream()):
                 req.C = self.result_view(req).clone()
             (stream or torch.cuda.current_stream()).synchronize()
-        self.pool.free(req._off, req._bytes)
+        g = req._group
+        if g is not None:
+            req._group = None
+            g.refs -= 1
+            if g.refs == 0:
+                self.pool.free(g.off, g.nbytes)
+        else:
+            self.pool.free(req._off, req._bytes)
 
     # -- batches ------------------------------------------------------------
-    def _fill(self, slot: _Slot, batch: List[Request]) -> None:
-        """Problem tables of one batch, vectorised over its requests."""
+    def _batch_arrays(self, batch: List[Request]):
         n = len(batch)
         wi = np.fromiter((r.weight for r in batch), dtype=np.int64, count=n)
         N = np.fromiter((r.N for r in batch), dtype=np.int64, count=n)
         bt = self.pool.base + np.fromiter((r._bt_off for r in batch), dtype=np.int64, count=n)
         cp = self.pool.base + np.fromiter((r._c_off for r in batch), dtype=np.int64, count=n)
         bp = np.fromiter((r.B.data_ptr() for r in batch), dtype=np.int64, count=n)
-        self.backend.fill(slot, wi, N, bt, cp, bp)
+        return wi, N, bt, cp, bp
+
+    def _fill(self, slot: _Slot, batch: List[Request]) -> None:
+        """Problem tables of one batch, vectorised over its requests."""
+        if hasattr(self.backend, "fill_batch"):
+            self.backend.fill_batch(self, slot, batch)
+        else:
+            self.backend.fill(slot, *self._batch_arrays(batch))
 
     def run_batch(self, batch: List[Request], slot_index: int = 0) -> None:
         """Admit, run and release one batch synchronously; results stay in

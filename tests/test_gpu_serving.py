@@ -184,8 +184,9 @@ def test_regtile_served_results_exact(rt_env, oracle, graphs):
     assert server.pool.used == 0
 
 
-def test_regtile_alone_equals_mixed_and_budget(rt_env):
+def test_regtile_alone_equals_mixed_and_budget(rt_env, oracle):
     srt, torch, shapes, Ws, weights = rt_env
+    so, cb = oracle
     server = srt.serving.Server(weights, budget_bytes=64 << 20, max_batch=16, slots=2, grid=64)
     reqs = _requests(srt, torch, shapes, 12, qps=1e6, seed=15)
     alone = []
@@ -194,14 +195,24 @@ def test_regtile_alone_equals_mixed_and_budget(rt_env):
         alone.append(r.C.cpu().numpy())
     mixed = [srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B) for r in reqs]
     server.run_batch(list(reversed(mixed)), slot_index=1)
+    # 12 requests over 4 weights: requests of one weight ran merged, one SpMM each
+    assert server.slots[1].hdr[2] == len({r.weight for r in mixed}) < len(mixed)
+    assert server.pool.used == 0
     for r, a in zip(mixed, alone):
         assert np.array_equal(r.C.cpu().numpy().view(np.uint32), a.view(np.uint32)), "request %d" % r.rid
     small = srt.serving.Server(weights, budget_bytes=256 << 10, max_batch=64, slots=2, grid=64)
     with pytest.raises(MemoryError):
         big = srt.serving.Request(rid=99, weight=3, B=torch.zeros((520, 512), dtype=torch.bfloat16, device="cuda"))
         small.run_batch([big])
-    st = small.serve(_requests(srt, torch, shapes, 24, qps=1e6, seed=17), keep_results=True)
+    # a budget this small is full at every batch: merged blocks take the
+    # members' own blocks, or the members run alone; the results stay exact
+    tight = _requests(srt, torch, shapes, 24, qps=1e6, seed=17)
+    st = small.serve(tight, keep_results=True)
     assert st["batches"] > 1
+    assert small.pool.used == 0
+    for r in tight:
+        ref = _oracle_served(so, cb, Ws[r.weight], r.B, weights[r.weight])
+        assert np.array_equal(r.C.cpu().numpy().view(np.uint32), ref.view(np.uint32)), "request %d" % r.rid
 
 
 def test_regtile_full_serving_workload(oracle):
