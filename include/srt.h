@@ -360,9 +360,11 @@ void srt_tile_destroy(srt_tile *t);
  * No counterpart in the reference (its executors are FP32 AVX-512 only,
  * tools/codegen/gen_vector_specialized.py:266-349); the packing follows the
  * reference's ILP padding idea (sbench/ilp_pad/nano_solver.py:76-116) at its
- * dense extreme: 16-row panels x 32-column blocks stored dense in MFMA operand
- * layout (block (p, kb) at slot p * KB + kb), for panels dense enough for
- * matrix cores.
+ * dense extreme: 16-row panels x 32-column blocks in MFMA operand layout,
+ * each block holding a nonzero stored dense; blocks without one are not
+ * stored (the plan's [P][KB] slot table holds -1 for them and the executors
+ * read zeros), so memory follows the occupied blocks.  With every block
+ * present, block (p, kb) sits at p * KB + kb and no table is read.
  * FP8: A rows scaled by s_i = max|A_i| / 448, B columns by t_j = max|B_:j| /
  * 448, both computed in FP64 on the device; C = diag(s) (qA qB) diag(t).
  * ---------------------------------------------------------------------- */
@@ -433,6 +435,8 @@ typedef struct srt_mfma_problem {
     int M, N, KB, P;
     int tile_begin;
     int vec_ok;
+    const int32_t *slot;  /* plan's block table ([P][KB] stored index or -1),
+                             NULL when every block is stored (at p * KB + kb) */
 } srt_mfma_problem;
 
 int srt_mfma_problem_fill(const srt_mfma_plan *p, const void *Bt, int64_t ldb, const float *col_scales, float *C,

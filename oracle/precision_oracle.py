@@ -112,9 +112,10 @@ def dequant(codes, dtype: str) -> np.ndarray:
 
 
 def pack_blocks(A: np.ndarray, dtype: str):
-    """Reference packing: (slot [P, KB], blk_kb, blocks [P * KB, 512]) with
-    blocks in MFMA operand order, every block stored (dense 16-row panels),
-    block (p, kb) at slot p * KB + kb; blocks without a nonzero are zero."""
+    """Reference packing: (slot [P, KB], blk_kb, blocks [n, 512]) with
+    blocks in MFMA operand order; only the n blocks holding a nonzero are
+    stored, in (panel, K block) order, and slot[p, kb] is the stored index of
+    block (p, kb) or -1 (the executor reads zeros for it)."""
     A = np.asarray(A, dtype=np.float32)
     M, K = A.shape
     P = -(-M // PANEL)
@@ -129,6 +130,8 @@ def pack_blocks(A: np.ndarray, dtype: str):
     lane = np.arange(64)
     for p in range(P):
         for b in range(KB):
+            if not nzpad[p * PANEL:(p + 1) * PANEL, b * BLOCK_K:(b + 1) * BLOCK_K].any():
+                continue
             slot[p, b] = len(blocks)
             kb_list.append(b)
             tile = pad[p * PANEL:(p + 1) * PANEL, b * BLOCK_K:(b + 1) * BLOCK_K]

@@ -37,6 +37,7 @@ struct srt_mfma_plan {
     int dtype = 0;  // SRT_DT_BF16 or SRT_DT_FP8
     int M = 0, K = 0, P = 0, KB = 0;
     int64_t nnz = 0, nblk = 0, nz_blocks = 0;
+    bool compact = false;  // some block absent: executors look blocks up in `slot`
     uint8_t *present = nullptr;
     int32_t *slot = nullptr;
     int32_t *blk_kb = nullptr;
@@ -152,10 +153,16 @@ int srt_mfma_plan_create(int m, int k, const float *Lx, const void *Lp, const vo
         SRT_HIP_TRY(hipStreamSynchronize(s));
         if (hbad) return fail(SRT_ERR_INVALID, "column index outside [0, k)");
     }
-    p->nblk = (int64_t)p->P * p->KB;  // every block stored (dense panels)
+    // Only the blocks holding a nonzero are stored.  At least one block's
+    // bytes are allocated (and zeroed): the executor's operand load for an
+    // absent block reads stored block 0 before zeroing the operand.
+    p->nblk = p->nz_blocks;
+    p->compact = p->nblk < (int64_t)cells;
+    if (p->compact && cells * 4 > (size_t)INT32_MAX)
+        return fail(SRT_ERR_UNSUPPORTED, "slot table of " + std::to_string(cells) + " blocks exceeds 2 GiB");
     MF_TRY(dalloc(&p->blk_kb, (size_t)p->nblk));
-    const size_t blk_bytes = (size_t)p->nblk * 512 * elt_bytes(dtype);
-    SRT_HIP_TRY(hipMalloc(&p->blocks, std::max<size_t>(blk_bytes, 16)));
+    const size_t blk_bytes = std::max<size_t>((size_t)p->nblk, 1) * 512 * elt_bytes(dtype);
+    SRT_HIP_TRY(hipMalloc(&p->blocks, blk_bytes));
     if (m > 0) {
         SRT_HIP_TRY(hipMemsetAsync(p->blocks, 0, blk_bytes, s));
         SRT_HIP_TRY(launch_mfma_slots(p->P, p->KB, p->present, blk_ptr, p->slot, p->blk_kb, s));
@@ -224,7 +231,7 @@ static int mfma_execute(const srt_mfma_plan *p, float *C, int64_t ldc, const voi
     a.B = B;
     a.ldb = ldb;
     a.blocks = p->blocks;
-    a.slot = p->slot;
+    a.slot = p->compact ? p->slot : nullptr;
     a.rscale = p->rscale;
     a.cscale = col_scales;
     a.C = C;
@@ -313,6 +320,7 @@ int srt_mfma_problem_fill(const srt_mfma_plan *p, const void *Bt, int64_t ldb, c
     out->tile_begin = tile_begin;
     const int eb = elt_bytes(p->dtype);
     out->vec_ok = ((uintptr_t)Bt % 16 == 0 && (ldb * eb) % 16 == 0) ? 1 : 0;
+    out->slot = p->compact ? p->slot : nullptr;
     return SRT_OK;
 }
 

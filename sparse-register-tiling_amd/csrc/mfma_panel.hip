@@ -105,21 +105,26 @@ __global__ void blk_count_kernel(int P, int KB, const uint8_t *__restrict__ pres
     nblk[p] = n;
 }
 
-// One thread per panel: every block of the panel is stored, at slot
-// p * KB + kb (dense, so the executor computes block addresses without a
-// lookup); blocks without a nonzero stay zero.
+// One thread per panel: only blocks holding a nonzero are stored, the
+// panel's from blk_ptr[p] on in K order; slot[p * KB + kb] is the stored
+// index of block (p, kb) or -1.  With every block present the slots are
+// p * KB + kb and the executors address blocks without the table.
 __global__ void blk_slot_kernel(int P, int KB, const uint8_t *__restrict__ present,
                                 const int64_t *__restrict__ blk_ptr, int32_t *__restrict__ slot,
                                 int32_t *__restrict__ blk_kb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
+    int64_t at = blk_ptr[p];
     for (int b = 0; b < KB; ++b) {
         const int64_t s = (int64_t)p * KB + b;
-        slot[s] = (int32_t)s;
-        blk_kb[s] = b;
+        if (present[s]) {
+            slot[s] = (int32_t)at;
+            blk_kb[at] = b;
+            ++at;
+        } else {
+            slot[s] = -1;
+        }
     }
-    (void)present;
-    (void)blk_ptr;
 }
 
 // One wave per row: FP8 row scale s = max|A_i| / 448 computed in FP64.
@@ -232,10 +237,14 @@ using TileBuf = typename Elt<FP8>::T[2][kStageBlocks * NT * Elt<FP8>::kLdsK];
 
 // One output tile: panels (pg * WP + wave) * PW .. + PW, columns nt * NT ..,
 // K range z of ks.  Every wave of the workgroup calls it (barriers inside).
-template <bool FP8, int NT, int WP, int PW>
+// CMP: blocks are stored compactly and found through slot[p * KB + kb]
+// (-1 = no nonzero: the A operand is zero); otherwise block (p, kb) sits at
+// p * KB + kb.
+template <bool FP8, int NT, int WP, int PW, bool CMP>
 __device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, int KB, int P,
                                           const typename Elt<FP8>::T *__restrict__ Bt, int64_t ldb,
-                                          const void *__restrict__ blocks, const float *__restrict__ rscale,
+                                          const void *__restrict__ blocks, const int32_t *__restrict__ slot,
+                                          const float *__restrict__ rscale,
                                           const float *__restrict__ cscale, float *__restrict__ C,
                                           int64_t ldc, int ks, float *__restrict__ work, int vec_ok, int pg,
                                           int nt, int z) {
@@ -268,6 +277,22 @@ __device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, 
     constexpr int DEPTH = 4;
     uint4 breg[DEPTH][CPT];
     AOp areg[DEPTH][PW];
+    // CMP: slots ride one ring ahead of the A operands (sreg[j] holds the
+    // slots of the blocks areg[j] loads next), so an A load never waits on a
+    // slot issued just before it; aok[j] says whether areg[j]'s block exists.
+    // Slots are read with buffer loads (vector memory, completed in order
+    // with the B and A loads).
+    int sreg[CMP ? DEPTH : 1][PW];
+    bool aok[CMP ? DEPTH : 1][PW];
+    const __amdgpu_buffer_rsrc_t srsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int32_t *>(slot), (short)0, CMP ? (int)((int64_t)P * KB * 4) : 0, 0x00020000);
+    auto load_slot = [&](int (&dst)[PW], int kb) {
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            const int p = p0 + i < P ? p0 + i : P - 1;
+            dst[i] = __builtin_amdgcn_raw_buffer_load_b32(srsrc, (int)(((int64_t)p * KB + kb) * 4), 0, 0);
+        }
+    };
     // Branch-free (the host guarantees 16-byte aligned rows): a lane past
     // column N loads row 0 (valid) and its chunk is zeroed when stored, so
     // no load sits in a branch or feeds a select (either would force a wait
@@ -293,12 +318,38 @@ __device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, 
             dst[i] = reinterpret_cast<const AOp *>(blocks)[((int64_t)p * KB + kb) * 64 + lane];
         }
     };
-    const bool any = kb0 < kb1;  // uniform; an empty K range still stores zeros
+    // CMP: the A operands of the blocks whose slots are in sl; an absent
+    // block loads stored block 0 (the plan keeps at least one block's bytes)
+    // and is zeroed where it is used.
+    auto load_a_slots = [&](AOp (&dst)[PW], bool (&ok)[PW], const int (&sl)[PW]) {
 #pragma unroll
-    for (int j = 0; j < DEPTH && any; ++j) {
-        const int kb = kb0 + j < kb1 ? kb0 + j : kb1 - 1;
-        load_b(breg[j], kb);
-        load_a(areg[j], kb);
+        for (int i = 0; i < PW; ++i) {
+            ok[i] = sl[i] >= 0;
+            dst[i] = reinterpret_cast<const AOp *>(blocks)[(int64_t)(ok[i] ? sl[i] : 0) * 64 + lane];
+        }
+    };
+    const bool any = kb0 < kb1;  // uniform; an empty K range still stores zeros
+    if constexpr (CMP) {
+        // slots of the first 2 DEPTH blocks, A operands of the first DEPTH
+#pragma unroll
+        for (int j = 0; j < DEPTH && any; ++j) {
+            const int kb = kb0 + j < kb1 ? kb0 + j : kb1 - 1;
+            load_b(breg[j], kb);
+            load_slot(sreg[j], kb);
+        }
+#pragma unroll
+        for (int j = 0; j < DEPTH && any; ++j) {
+            load_a_slots(areg[j], aok[j], sreg[j]);
+            const int kn = kb0 + DEPTH + j < kb1 ? kb0 + DEPTH + j : kb1 - 1;
+            load_slot(sreg[j], kn);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < DEPTH && any; ++j) {
+            const int kb = kb0 + j < kb1 ? kb0 + j : kb1 - 1;
+            load_b(breg[j], kb);
+            load_a(areg[j], kb);
+        }
     }
     int buf = 0;
     static_assert(DEPTH % kStageBlocks == 0, "the ring holds whole stages");
@@ -326,11 +377,20 @@ __device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, 
                 const bool run = kb < kb1;  // uniform; every wave takes every barrier
                 AOp a[PW];
 #pragma unroll
-                for (int i = 0; i < PW; ++i) a[i] = areg[j][i];
+                for (int i = 0; i < PW; ++i) {
+                    a[i] = areg[j][i];
+                    if constexpr (CMP) a[i] = aok[j][i] ? a[i] : AOp{};
+                }
                 {
                     const int kn = kb + DEPTH < kb1 ? kb + DEPTH : kb1 - 1;
                     load_b(breg[j], kn);
-                    load_a(areg[j], kn);
+                    if constexpr (CMP) {
+                        load_a_slots(areg[j], aok[j], sreg[j]);
+                        const int kn2 = kb + 2 * DEPTH < kb1 ? kb + 2 * DEPTH : kb1 - 1;
+                        load_slot(sreg[j], kn2);
+                    } else {
+                        load_a(areg[j], kn);
+                    }
                 }
 #pragma unroll
                 for (int t = 0; t < TN && run; ++t) {
@@ -378,15 +438,15 @@ __device__ __forceinline__ void mfma_tile(TileBuf<FP8, NT> &tile, int M, int N, 
     }
 }
 
-template <bool FP8, int NT, int WP, int PW>
+template <bool FP8, int NT, int WP, int PW, bool CMP>
 __global__ void __launch_bounds__(64 * WP)
     mfma_exec_kernel(int M, int N, int KB, int P, const typename Elt<FP8>::T *__restrict__ Bt, int64_t ldb,
-                     const void *__restrict__ blocks, const float *__restrict__ rscale,
-                     const float *__restrict__ cscale, float *__restrict__ C, int64_t ldc, int ks,
-                     float *__restrict__ work, int vec_ok) {
+                     const void *__restrict__ blocks, const int32_t *__restrict__ slot,
+                     const float *__restrict__ rscale, const float *__restrict__ cscale, float *__restrict__ C,
+                     int64_t ldc, int ks, float *__restrict__ work, int vec_ok) {
     __shared__ __attribute__((aligned(16))) TileBuf<FP8, NT> tile;
-    mfma_tile<FP8, NT, WP, PW>(tile, M, N, KB, P, Bt, ldb, blocks, rscale, cscale, C, ldc, ks, work, vec_ok,
-                               blockIdx.x, blockIdx.y, blockIdx.z);
+    mfma_tile<FP8, NT, WP, PW, CMP>(tile, M, N, KB, P, Bt, ldb, blocks, slot, rscale, cscale, C, ldc, ks, work,
+                                    vec_ok, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Grouped, persistent: the tiles of every problem in the table, problem q
@@ -412,10 +472,17 @@ __global__ void __launch_bounds__(64 * WP)
         const int local = t - probs[q].tile_begin;
         const int N = probs[q].N;
         const int ntiles = (N + NT - 1) / NT;
-        mfma_tile<FP8, NT, WP, PW>(tile, probs[q].M, N, probs[q].KB, probs[q].P,
-                                   static_cast<const typename Elt<FP8>::T *>(probs[q].Bt), probs[q].ldb,
-                                   probs[q].blocks, probs[q].rscale, probs[q].cscale, probs[q].C, probs[q].ldc, 1,
-                                   nullptr, probs[q].vec_ok, local / ntiles, local % ntiles, 0);
+        const auto *Bt = static_cast<const typename Elt<FP8>::T *>(probs[q].Bt);
+        if (probs[q].slot)  // uniform: compactly stored blocks
+            mfma_tile<FP8, NT, WP, PW, true>(tile, probs[q].M, N, probs[q].KB, probs[q].P, Bt, probs[q].ldb,
+                                             probs[q].blocks, probs[q].slot, probs[q].rscale, probs[q].cscale,
+                                             probs[q].C, probs[q].ldc, 1, nullptr, probs[q].vec_ok, local / ntiles,
+                                             local % ntiles, 0);
+        else
+            mfma_tile<FP8, NT, WP, PW, false>(tile, probs[q].M, N, probs[q].KB, probs[q].P, Bt, probs[q].ldb,
+                                              probs[q].blocks, nullptr, probs[q].rscale, probs[q].cscale,
+                                              probs[q].C, probs[q].ldc, 1, nullptr, probs[q].vec_ok,
+                                              local / ntiles, local % ntiles, 0);
         __syncthreads();  // the next tile rewrites the LDS buffers
     }
 }
@@ -511,9 +578,14 @@ static hipError_t launch_exec_t(const MfmaLaunch &a, hipStream_t s) {
     const dim3 grid((a.P + WP * PW - 1) / (WP * PW), (a.N + NT - 1) / NT, a.ks);
     const int vec_ok =
         ((uintptr_t)a.B % 16 == 0 && (a.ldb * (int64_t)sizeof(typename Elt<FP8>::T)) % 16 == 0) ? 1 : 0;
-    hipLaunchKernelGGL((mfma_exec_kernel<FP8, NT, WP, PW>), grid, dim3(64 * WP), 0, s, a.M, a.N, a.KB, a.P,
-                       static_cast<const typename Elt<FP8>::T *>(a.B), a.ldb, a.blocks, a.rscale, a.cscale,
-                       a.C, a.ldc, a.ks, a.work, vec_ok);
+    if (a.slot)
+        hipLaunchKernelGGL((mfma_exec_kernel<FP8, NT, WP, PW, true>), grid, dim3(64 * WP), 0, s, a.M, a.N, a.KB,
+                           a.P, static_cast<const typename Elt<FP8>::T *>(a.B), a.ldb, a.blocks, a.slot, a.rscale,
+                           a.cscale, a.C, a.ldc, a.ks, a.work, vec_ok);
+    else
+        hipLaunchKernelGGL((mfma_exec_kernel<FP8, NT, WP, PW, false>), grid, dim3(64 * WP), 0, s, a.M, a.N, a.KB,
+                           a.P, static_cast<const typename Elt<FP8>::T *>(a.B), a.ldb, a.blocks,
+                           (const int32_t *)nullptr, a.rscale, a.cscale, a.C, a.ldc, a.ks, a.work, vec_ok);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.ks <= 1) return e;
     const int64_t n = (int64_t)a.M * a.N;

@@ -220,7 +220,7 @@ struct MfmaLaunch {
     const void *B;  // Bt: bf16 or e4m3, N x ldb (k contiguous), ldb >= KB * 32
     int64_t ldb;
     const void *blocks;     // stored blocks in MFMA A-operand layout
-    const int32_t *slot;    // [P][KB] block slot or -1
+    const int32_t *slot;    // [P][KB] stored index of each block or -1; null: block (p, kb) at p * KB + kb
     const float *rscale;    // FP8 row scales (M)
     const float *cscale;    // FP8 column scales of B (N)
     float *C;
@@ -251,6 +251,7 @@ struct MfmaProblem {
     int M, N, KB, P;
     int tile_begin;  // first global tile of this problem
     int vec_ok;      // Bt rows 16-byte aligned
+    const int32_t *slot;  // [P][KB] stored index of each block or -1; null: block (p, kb) at p * KB + kb
 };
 struct BPrepProblem {
     const void *src;  // fp32 or bf16 (src_bf16), K x N row-major, ld_src

@@ -54,11 +54,11 @@ from .precision import DTYPES, SHAPES, PanelPlan
 
 PROBLEM_DTYPE = np.dtype([("blocks", "<u8"), ("Bt", "<u8"), ("C", "<u8"), ("rscale", "<u8"), ("cscale", "<u8"),
                           ("ldb", "<i8"), ("ldc", "<i8"), ("M", "<i4"), ("N", "<i4"), ("KB", "<i4"), ("P", "<i4"),
-                          ("tile_begin", "<i4"), ("vec_ok", "<i4")])
+                          ("tile_begin", "<i4"), ("vec_ok", "<i4"), ("slot", "<u8")])
 BPREP_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("cscale", "<u8"), ("ld_src", "<i8"), ("ldk", "<i8"),
                         ("K", "<i4"), ("N", "<i4"), ("tile_begin", "<i4"), ("src_bf16", "<i4"),
                         ("pad_to", "<i4"), ("reserved", "<i4")])
-assert PROBLEM_DTYPE.itemsize == 80 and BPREP_DTYPE.itemsize == 64
+assert PROBLEM_DTYPE.itemsize == 88 and BPREP_DTYPE.itemsize == 64
 
 
 class SlabPool:

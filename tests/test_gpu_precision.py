@@ -177,3 +177,47 @@ def test_k_split_sum_is_deterministic(srt, torch_cuda, dtype, shape, ksplit):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(C.view(torch.int32), first.view(torch.int32))
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+@pytest.mark.parametrize("shape", [0, 3, 8])
+def test_compact_blocks_alone_and_grouped(srt, torch_cuda, dtype, shape):
+    """Only blocks holding a nonzero are stored: a block-sparse matrix keeps a
+    fraction of the dense block count, and its results -- alone, with a K
+    split, and in a grouped launch beside a plan with every block stored --
+    match the oracle's bound and each other bit for bit."""
+    torch = torch_cuda
+    _log("precision compact", dtype, shape)
+    rng = np.random.default_rng(31)
+    M, K = 300, 700
+    P, KB = -(-M // 16), -(-K // 32)
+    keep = rng.random((P, KB)) < 0.15          # about 15 % of the 16 x 32 blocks occupied
+    keep[3] = False                            # an empty panel
+    mask = np.repeat(np.repeat(keep, 16, axis=0), 32, axis=1)[:M, :K]
+    A = (mask * (rng.random((M, K)) < 0.5) * rng.standard_normal((M, K))).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, 96)).astype(np.float32)
+    plan, C = _run(srt, torch, A, B, dtype, shape=shape, ksplit=1)
+    st = plan.stats()
+    slot, _, blocks = po.pack_blocks(A, dtype)
+    assert st["stored_blocks"] == st["nonzero_blocks"] == len(blocks) < P * KB // 4
+    _within(C, A, B, dtype)
+    Bq, t = plan.prepare_b(torch.from_numpy(B).cuda())
+    C2 = torch.full((M, 96), float("nan"), device="cuda")
+    plan.reserve(96, 3)
+    plan(C2, Bq, t, shape=shape, ksplit=3)
+    torch.cuda.synchronize()
+    _within(C2.cpu().numpy(), A, B, dtype)
+    # grouped: the compact plan beside a plan with every block present
+    D = _sparse(200, 700, 0.5, 32)
+    dense = srt.precision.PanelPlan.from_dense(torch.from_numpy(D), dtype)
+    assert dense.stats()["stored_blocks"] == 13 * 22
+    Dq, td = dense.prepare_b(torch.from_numpy(B).cuda())
+    Cd = torch.full((200, 96), float("nan"), device="cuda")
+    dense(Cd, Dq, td, shape=shape, ksplit=1)
+    Cg = [torch.full((M, 96), float("nan"), device="cuda"), torch.full((200, 96), float("nan"), device="cuda")]
+    g = srt.cnn.GroupedPanels([plan, dense], [Bq, Dq], Cg, col_scales=[t, td], shape=shape, grid=37)
+    for _ in range(2):
+        g()
+        torch.cuda.synchronize()
+        assert np.array_equal(Cg[0].cpu().numpy().view(np.uint32), C.view(np.uint32))
+        assert np.array_equal(Cg[1].cpu().numpy().view(np.uint32), Cd.cpu().numpy().view(np.uint32))

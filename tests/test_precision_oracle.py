@@ -54,7 +54,7 @@ def test_block_layout_reconstructs_dense(dtype):
     A = ((rng.random((37, 70)) < 0.1) * rng.standard_normal((37, 70))).astype(np.float32)
     A[16:32, 32:64] = 0.0  # one empty block
     slot, kb, blocks = po.pack_blocks(A, dtype)
-    assert slot[1, 1] == 1 * slot.shape[1] + 1 and not blocks[slot[1, 1]].any()
+    assert slot[1, 1] == -1  # the empty block is not stored
     codes, _ = po.quantize_a(A, dtype)
     rec = np.zeros((48, 96), dtype=codes.dtype)
     lane = np.arange(64)
@@ -66,7 +66,15 @@ def test_block_layout_reconstructs_dense(dtype):
             for e in range(8):
                 rec[p * 16 + lane % 16, b * 32 + 8 * (lane // 16) + e] = blk[:, e]
     assert np.array_equal(rec[:37, :70], np.where(A != 0, codes, 0))
-    assert np.array_equal(slot.ravel(), np.arange(slot.size))
+    # stored blocks in (panel, K block) order: slots ascend, blk_kb names each
+    # block's K block, and exactly the blocks holding a nonzero are stored
+    stored = slot.ravel()[slot.ravel() >= 0]
+    assert np.array_equal(stored, np.arange(len(blocks)))
+    assert np.array_equal(kb, np.nonzero(slot >= 0)[1])
+    nz = np.zeros((48, 96), dtype=bool)
+    nz[:37, :70] = A != 0
+    assert np.array_equal(slot >= 0, nz.reshape(3, 16, 3, 32).any(axis=(1, 3)))
+    assert len(blocks) == 8 and all(blocks[i].any() for i in range(len(blocks)))
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "fp8"])
