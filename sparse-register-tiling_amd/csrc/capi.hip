@@ -15,6 +15,7 @@
 #include <memory>
 #include <mutex>
 #include <queue>
+#include <tuple>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -772,6 +773,11 @@ static int config_tiles(const srt_tile_config &c, int n) {
 // chunk_cols packed columns per wave (so no CU runs one workgroup more than
 // another); then each workgroup's free slots go one at a time to its panel
 // whose chunk is longest, while that chunk is longer than 2 * kMinChunk.
+// A panel adds at least one average slot's columns to its bin's load, and
+// equal loads go to the bin with fewer panels: empty or light panels (rows
+// of a hybrid plan's dense panels, dead rows of a pruned matrix) would
+// otherwise all pile into one bin, taking the slots its heavy panel needs
+// for splitting (one unsplit chain of thousands of columns).
 // Fills kp and appends (p, j, kp) per slot, W slots per workgroup.
 static void balanced_slots(const Packed &pk, Schedule &sch, int n_tiles, std::vector<int32_t> &slots) {
     constexpr int64_t kMinChunk = 8;
@@ -794,15 +800,16 @@ static void balanced_slots(const Packed &pk, Schedule &sch, int n_tiles, std::ve
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nc[a] > nc[b]; });
     std::vector<std::vector<int>> bins((size_t)G);
     std::vector<int64_t> load((size_t)G, 0);
-    typedef std::pair<int64_t, int64_t> Entry;  // (load, bin): the smallest pops first
+    const int64_t per_slot = std::max<int64_t>(1, total / (G * W));
+    typedef std::tuple<int64_t, int, int64_t> Entry;  // (load, panels, bin): the smallest pops first
     std::priority_queue<Entry, std::vector<Entry>, std::greater<Entry>> heap;
-    for (int64_t b = 0; b < G; ++b) heap.push(Entry(0, b));
+    for (int64_t b = 0; b < G; ++b) heap.push(Entry(0, 0, b));
     for (int p : order) {  // G * W >= P, so a bin with a free slot remains
-        const int64_t b = heap.top().second;
+        const int64_t b = std::get<2>(heap.top());
         heap.pop();
         bins[b].push_back(p);
-        load[b] += nc[p];
-        if ((int)bins[b].size() < W) heap.push(Entry(load[b], b));
+        load[b] += std::max(nc[p], per_slot);
+        if ((int)bins[b].size() < W) heap.push(Entry(load[b], (int)bins[b].size(), b));
     }
     for (int64_t b = 0; b < G; ++b) {
         const std::vector<int> &L = bins[b];

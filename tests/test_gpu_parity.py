@@ -214,6 +214,32 @@ def test_empty_and_sparse_rows(srt, oracle, torch_cuda):
                                  nsplit=plan.split_counts()))
 
 
+def test_balanced_layout_splits_panels_beside_empty_ones(srt, oracle, torch_cuda):
+    """Rows emptied in whole 16-row blocks (a hybrid plan's dense panels, dead
+    rows of a pruned matrix) under the balanced layout: the empty panels
+    spread over the workgroups instead of filling one, so every non-empty
+    panel still gets its nsplit waves (an unsplit panel is a chain of
+    thousands of columns on one wave, several times the launch's time), and
+    the result is the oracle's chain over that split."""
+    so, cb = oracle
+    torch = torch_cuda
+    _log("balanced layout with empty panels")
+    rng = np.random.default_rng(41)
+    M, K, N = 1024, 1024, 512
+    A = ((rng.random((M, K)) < 0.3) * rng.standard_normal((M, K))).astype(np.float32)
+    for g in (0, 5, 6, 13, 21, 22, 30, 41, 50, 57, 63):
+        A[16 * g:16 * (g + 1)] = 0.0
+    B = srt.workloads.dense_b(K, N, seed=42)
+    plan = _plan(srt, torch, A, N, 4, "identity", vec_width=4, waves_per_block=16, nsplit=8, chunk_cols=100,
+                 layout=2)
+    kp = plan.split_counts()
+    ncols = np.diff(plan.packed()["col_ptr"])
+    assert (ncols[ncols == 0].size == 44) and (kp[ncols > 0] == 8).all(), np.bincount(kp[ncols > 0])
+    _bitexact(_run(srt, torch, plan, B),
+              cb.exec_packed(cb.pack_dense(A, 4, _oracle_table(so, 4, "identity")), B, nsplit=kp))
+    plan.close()
+
+
 def test_strided_b_c_and_stream(srt, oracle, torch_cuda):
     """B and C as row-strided views; execution on a side stream; n < B width."""
     so, cb = oracle
