@@ -346,11 +346,21 @@ struct DensePart {
     int32_t *panel_row = nullptr;   // first row of each dense panel
     std::vector<int32_t> h_panel_row;
     size_t bytes = 0;
+    // Concurrent execution: the register-tiled schedule leaves the dense
+    // panels' rows out, so the two launches write disjoint rows and the dense
+    // one runs on `side`, forked from and joined back to the caller's stream.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     void release() {
         (void)hipFree(apack);
         (void)hipFree(panel_row);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         apack = nullptr;
         panel_row = nullptr;
+        side = nullptr;
+        fork = join = nullptr;
         ndense = 0;
     }
 };
@@ -795,8 +805,10 @@ static void balanced_slots(const Packed &pk, Schedule &sch, int n_tiles, std::ve
     G = std::min<int64_t>(G, P);
     G = std::max<int64_t>(G, (P + W - 1) / W);
     G = std::max<int64_t>(G, 1);
-    std::vector<int> order(P);
-    for (int p = 0; p < P; ++p) order[p] = p;
+    std::vector<int> order;
+    order.reserve(P);
+    for (int p = 0; p < P; ++p)
+        if (pk.h_skip.empty() || !pk.h_skip[p]) order.push_back(p);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nc[a] > nc[b]; });
     std::vector<std::vector<int>> bins((size_t)G);
     std::vector<int64_t> load((size_t)G, 0);
@@ -876,6 +888,7 @@ static int build_schedule(const Packed &pk, const srt_tile_config &cfg, const Ex
             used = 0;
         };
         for (int p = 0; p < pk.P; ++p) {
+            if (!pk.h_skip.empty() && pk.h_skip[p]) continue;
             const int64_t nc = pk.h_col_ptr[p + 1] - pk.h_col_ptr[p];
             const int64_t want = (nc + sch.chunk_cols - 1) / sch.chunk_cols;
             const int kp = (int)std::max<int64_t>(1, std::min<int64_t>(sch.nsplit, want));
@@ -1375,6 +1388,26 @@ int srt_plan_create_hybrid(int m, int k, int n, const float *Lx, const void *Lp,
     SRT_HIP_TRY(hipMemcpyAsync(dp.apack, ap.data(), sizeof(float) * ap.size(), hipMemcpyHostToDevice, s));
     SRT_HIP_TRY(hipMemcpyAsync(dp.panel_row, dense_rows.data(), sizeof(int32_t) * (size_t)nd, hipMemcpyHostToDevice, s));
     SRT_HIP_TRY(hipStreamSynchronize(s));
+    // With 16-row dense panels made of whole register-tile panels (M_r
+    // dividing 16) and some rows left to the register tile, the two launches
+    // run concurrently: the register-tiled schedule is rebuilt without the
+    // dense panels' rows (nothing can have captured the first one yet).
+    const bool all_dense = (int64_t)nd * 16 >= m;
+    Packed &pk = plan->pk;
+    if (nd > 0 && !all_dense && 16 % pk.M_r == 0 && plan->cfg.matrix_cores != 1) {
+        pk.h_skip.assign((size_t)pk.P, 0);
+        for (int d = 0; d < nd; ++d)
+            for (int r = dense_rows[d]; r < std::min(m, dense_rows[d] + 16); r += pk.M_r) pk.h_skip[(size_t)(r / pk.M_r)] = 1;
+        Schedule first = std::move(plan->sch);
+        plan->sch = Schedule();
+        const int rc = build_schedule(pk, plan->cfg, plan->exec, config_tiles(plan->cfg, std::max(n, 1)), plan->sch,
+                                      plan->retired, plan->cfg, s);
+        first.release();
+        SRT_TRY(rc);
+        SRT_HIP_TRY(hipStreamCreateWithFlags(&dp.side, hipStreamNonBlocking));
+        SRT_HIP_TRY(hipEventCreateWithFlags(&dp.fork, hipEventDisableTiming));
+        SRT_HIP_TRY(hipEventCreateWithFlags(&dp.join, hipEventDisableTiming));
+    }
     plan->pk.nnz = nnz;  // the whole matrix's entries (the register-tiled part holds the others)
     *out = guard.release();
     return SRT_OK;
@@ -1438,6 +1471,19 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
             SRT_HIP_TRY(launch_mfma4_exec(p->pk.col_ptr, p->pk.cols, p->pk.vals, rg.second, rg.first, B, ldb, C,
                                           ldc, p->pk.M, n, p->pk.K, (hipStream_t)stream));
         }
+    } else if (p->dense.side && n > 0) {
+        // dense panels on the side stream, forked from and joined back to the
+        // caller's (graph capture records both branches)
+        const hipStream_t s = (hipStream_t)stream;
+        const DensePart &dp = p->dense;
+        SRT_HIP_TRY(hipEventRecord(dp.fork, s));
+        SRT_HIP_TRY(hipStreamWaitEvent(dp.side, dp.fork, 0));
+        SRT_HIP_TRY(launch_dense_panels(dp.ndense, dp.G, dp.apack, dp.panel_row, B, ldb, C, ldc, p->pk.M, n,
+                                        p->pk.K, dp.side));
+        SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, s));
+        SRT_HIP_TRY(hipEventRecord(dp.join, dp.side));
+        SRT_HIP_TRY(hipStreamWaitEvent(s, dp.join, 0));
+        return SRT_OK;
     } else
         SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream));
     if (p->dense.ndense && n > 0)  // the dense panels' rows, zero from the register-tiled part, overwritten

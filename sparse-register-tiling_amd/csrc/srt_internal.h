@@ -127,6 +127,8 @@ struct Packed {
     int64_t *grp_ptr = nullptr;  // [P + 1]
     int32_t *id_pattern = nullptr;  // [NK] kernel pattern of each id
     std::vector<int64_t> h_col_ptr;  // host copy of col_ptr (scheduling)
+    std::vector<uint8_t> h_skip;     // panels the schedule leaves out (their C rows are
+                                     // written by someone else: a hybrid plan's dense panels)
     int64_t total_cols = 0, total_vals = 0, total_groups = 0, padding = 0, nnz = 0;
     void release();
     size_t bytes() const;

@@ -33,17 +33,22 @@ def _dense_panels(A, tau):
     return out
 
 
-@pytest.mark.parametrize("M,K,N,tau", [(500, 1000, 200, 0.3), (256, 300, 64, 0.3), (333, 517, 130, 0.05),
-                                       (1024, 4096, 512, 0.3)])
-def test_hybrid_matches_oracle(srt, oracle, M, K, N, tau):
+@pytest.mark.parametrize("M,K,N,tau,mapping", [(500, 1000, 200, 0.3, (4, "orig")), (256, 300, 64, 0.3, (4, "orig")),
+                                               (333, 517, 130, 0.05, (4, "orig")), (1024, 4096, 512, 0.3, (4, "orig")),
+                                               (500, 1000, 200, 0.3, (8, "orig")),
+                                               (500, 1000, 200, 0.3, (6, "identity"))])
+def test_hybrid_matches_oracle(srt, oracle, M, K, N, tau, mapping):
+    """M_r = 4 and 8 run the two parts concurrently (the register-tiled
+    schedule leaves the dense panels' rows out); M_r = 6 panels straddle the
+    16-row panels, so the dense launch follows and overwrites."""
     import torch
     so, cb = oracle
-    print("hybrid %dx%d N=%d tau=%g" % (M, K, N, tau), flush=True)
+    print("hybrid %dx%d N=%d tau=%g mapping %s" % (M, K, N, tau, mapping), flush=True)
     A = srt.workloads.magnitude_pruned(M, K, 0.7, seed=1) if M == 1024 else _mixed(M, K, seed=M)
     B = srt.workloads.dense_b(K, N, seed=2)
     rp, ci, v = srt.workloads.to_csr(A)
     plan = srt.MatMulSpecialized(M, K, N, torch.from_numpy(v), torch.from_numpy(rp), torch.from_numpy(ci),
-                                 mapping_id=srt.named_mapping(4, "orig"), dense_threshold=tau)
+                                 mapping_id=srt.named_mapping(*mapping), dense_threshold=tau)
     want = _dense_panels(A, tau)
     dp = plan.dense_panels()
     assert list(dp["rows"]) == want and dp["count"] == len(want) > 0
@@ -53,7 +58,7 @@ def test_hybrid_matches_oracle(srt, oracle, M, K, N, tau):
         dense_rows[r0:r0 + 16] = True
     As = A.copy()
     As[dense_rows] = 0
-    ref = cb.exec_packed(cb.pack_dense(As, 4, _oracle_table(so, 4, "orig")), B, nsplit=plan.split_counts())
+    ref = cb.exec_packed(cb.pack_dense(As, mapping[0], _oracle_table(so, *mapping)), B, nsplit=plan.split_counts())
     assert np.array_equal(C[~dense_rows].view(np.uint32), ref[~dense_rows].view(np.uint32))
     C64 = cb.dense_f64(A, B)
     assert np.all(np.abs(C - C64) <= so.fp64_error_bound(A, B))
