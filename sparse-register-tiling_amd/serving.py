@@ -213,7 +213,7 @@ class MfmaBackend:
         pp["tile_begin"] = np.cumsum(ptiles) - ptiles
         pp["src_bf16"] = 1
         pp["pad_to"] = 0
-        slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()))
+        slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()), 0, 0)
 
     def enqueue(self, slot: "_Slot", s: ct.c_void_p) -> None:
         _lib.call("srt_mfma_grouped_b_prep", self.code, ct.c_void_p(slot.d_prep.data_ptr()),
@@ -290,11 +290,15 @@ class RegTileBackend:
     prob_dtype = EXEC_PROBLEM_DTYPE
     V = 4
 
-    def __init__(self, weights: RegTileWeights, grid: int, merge: bool = True):
+    def __init__(self, weights: RegTileWeights, grid: int, merge: bool = True, narrow_cols: int = 64):
         self.weights = weights
         self.plans = weights.plans
         self.grid = int(grid)
         self.merge = bool(merge)
+        # SpMMs at most this wide run in a second persistent launch at one
+        # float per lane (a wave then spans 64 columns, not 256: a narrow
+        # request's B gathers and FMAs shrink 4x); 0 = one launch
+        self.narrow_cols = int(narrow_cols)
         self.wM = np.array([p.m for p in self.plans], dtype=np.int64)
         self.wK = np.array([p.k for p in self.plans], dtype=np.int64)
 
@@ -372,17 +376,9 @@ class RegTileBackend:
         wi = np.fromiter((x[0] for x in rows), dtype=np.int64, count=n)
         N = np.fromiter((x[1] for x in rows), dtype=np.int64, count=n)
         ld = np.fromiter((x[2] for x in rows), dtype=np.int64, count=n)
-        K = self.wK[wi]
-        blocks = self.wblocks[wi] * -(-N // (64 * self.V))
-        pr = slot.prob[:n]
-        pr[:] = self.rec[wi]
-        pr["B"] = np.fromiter((x[3] for x in rows), dtype=np.int64, count=n)
-        pr["C"] = np.fromiter((x[4] for x in rows), dtype=np.int64, count=n)
-        pr["ldb"] = ld
-        pr["ldc"] = ld
-        pr["N"] = N
-        pr["b_bytes"] = np.maximum(K * ld * 4, 4)
-        pr["block_begin"] = np.cumsum(blocks) - blocks
+        bt = np.fromiter((x[3] for x in rows), dtype=np.int64, count=n)
+        cp = np.fromiter((x[4] for x in rows), dtype=np.int64, count=n)
+        spmm = self._spmm_tables(slot, wi, N, ld, bt, cp)
         m = len(prep)
         pK = np.fromiter((x[4] for x in prep), dtype=np.int64, count=m)
         pad = np.fromiter((x[3] for x in prep), dtype=np.int64, count=m)
@@ -398,23 +394,39 @@ class RegTileBackend:
         pp["tile_begin"] = np.cumsum(wtiles) - wtiles
         pp["src_bf16"] = 1
         pp["pad_to"] = pad
-        slot.hdr[:] = (m, int(wtiles.sum()), n, int(blocks.sum()))
+        slot.hdr[:] = (m, int(wtiles.sum())) + spmm
+
+    def _spmm_tables(self, slot: "_Slot", wi, N, ld, bt, cp):
+        """SpMM records of a batch: wider than narrow_cols into slot.prob (4
+        floats per lane), the others into slot.prob2 (1 float per lane);
+        returns the two headers {problems, workgroups}."""
+        narrow = N <= self.narrow_cols if self.narrow_cols > 0 else np.zeros(len(N), dtype=bool)
+        out = ()
+        for table, sel, V in ((slot.prob, ~narrow, self.V), (slot.prob2, narrow, 1)):
+            if table is None:
+                out += (0, 0)
+                continue
+            w, n_, l_ = wi[sel], N[sel], ld[sel]
+            k = len(w)
+            blocks = self.wblocks[w] * -(-n_ // (64 * V))
+            pr = table[:k]
+            pr[:] = self.rec[w]
+            pr["B"] = bt[sel]
+            pr["C"] = cp[sel]
+            pr["ldb"] = l_
+            pr["ldc"] = l_
+            pr["N"] = n_
+            pr["b_bytes"] = np.maximum(self.wK[w] * l_ * 4, 4)
+            pr["block_begin"] = np.cumsum(blocks) - blocks
+            out += (k, int(blocks.sum()))
+        return out
 
     def fill(self, slot: "_Slot", wi, N, bt, cp, bp) -> None:
         n = len(wi)
         K = self.wK[wi]
         ld = -(-N // self.V) * self.V
-        blocks = self.wblocks[wi] * -(-N // (64 * self.V))
         wtiles = -(-(K * ld) // 1024)   # srt_widen_problem_tiles: the K x ld output walked flat
-        pr = slot.prob[:n]
-        pr[:] = self.rec[wi]
-        pr["B"] = bt
-        pr["C"] = cp
-        pr["ldb"] = ld
-        pr["ldc"] = ld
-        pr["N"] = N
-        pr["b_bytes"] = np.maximum(K * ld * 4, 4)
-        pr["block_begin"] = np.cumsum(blocks) - blocks
+        spmm = self._spmm_tables(slot, wi, N, ld, bt, cp)
         pp = slot.prep[:n]
         pp["src"] = bp
         pp["dst"] = bt
@@ -426,13 +438,16 @@ class RegTileBackend:
         pp["tile_begin"] = np.cumsum(wtiles) - wtiles
         pp["src_bf16"] = 1
         pp["pad_to"] = 0
-        slot.hdr[:] = (n, int(wtiles.sum()), n, int(blocks.sum()))
+        slot.hdr[:] = (n, int(wtiles.sum())) + spmm
 
     def enqueue(self, slot: "_Slot", s: ct.c_void_p) -> None:
         _lib.call("srt_exec_grouped_b_widen", ct.c_void_p(slot.d_prep.data_ptr()),
                   ct.c_void_p(slot.d_hdr.data_ptr()), self.grid, s)
         _lib.call("srt_exec_persistent_execute", self.plans[0]._h, self.V, ct.c_void_p(slot.d_prob.data_ptr()),
                   ct.c_void_p(slot.d_hdr.data_ptr() + 8), self.grid, s)
+        if slot.prob2 is not None:
+            _lib.call("srt_exec_persistent_execute", self.plans[0]._h, 1, ct.c_void_p(slot.d_prob2.data_ptr()),
+                      ct.c_void_p(slot.d_hdr.data_ptr() + 16), self.grid, s)
 
 
 class _Slot:
@@ -442,11 +457,17 @@ class _Slot:
         pdt = backend.prob_dtype
         self.h_prob = torch.zeros(max_batch * pdt.itemsize, dtype=torch.uint8).pin_memory()
         self.h_prep = torch.zeros(max_batch * BPREP_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
-        self.h_hdr = torch.zeros(4, dtype=torch.int32).pin_memory()  # prep {n, tiles}, spmm {n, tiles}
+        # headers: prep {n, tiles}, spmm {n, tiles}, narrow spmm {n, tiles}
+        self.h_hdr = torch.zeros(6, dtype=torch.int32).pin_memory()
         self.d_prob = torch.zeros_like(self.h_prob, device=device)
         self.d_prep = torch.zeros_like(self.h_prep, device=device)
-        self.d_hdr = torch.zeros(4, dtype=torch.int32, device=device)
+        self.d_hdr = torch.zeros(6, dtype=torch.int32, device=device)
         self.prob = np.frombuffer(self.h_prob.numpy(), dtype=pdt)
+        self.h_prob2 = self.d_prob2 = self.prob2 = None
+        if getattr(backend, "narrow_cols", 0) > 0:  # a second SpMM table (narrow problems)
+            self.h_prob2 = torch.zeros_like(self.h_prob).pin_memory()
+            self.d_prob2 = torch.zeros_like(self.h_prob, device=device)
+            self.prob2 = np.frombuffer(self.h_prob2.numpy(), dtype=pdt)
         self.prep = np.frombuffer(self.h_prep.numpy(), dtype=BPREP_DTYPE)
         self.hdr = self.h_hdr.numpy()
         self.graph: Optional[torch.cuda.CUDAGraph] = None
@@ -456,6 +477,8 @@ class _Slot:
     def _enqueue(self) -> None:
         s = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
         self.d_prob.copy_(self.h_prob, non_blocking=True)
+        if self.d_prob2 is not None:
+            self.d_prob2.copy_(self.h_prob2, non_blocking=True)
         self.d_prep.copy_(self.h_prep, non_blocking=True)
         self.d_hdr.copy_(self.h_hdr, non_blocking=True)
         self.backend.enqueue(self, s)
@@ -485,15 +508,17 @@ class Server:
 
     ``weights``: a ``RegTileWeights`` (register-tiled sparse panels) or a
     list of ``PanelPlan`` (MFMA panels).  ``grid``: workgroups of the
-    persistent launches; ``shape``: MFMA launch shape (MFMA weights only)."""
+    persistent launches; ``shape``: MFMA launch shape (MFMA weights only);
+    ``narrow_cols`` (register-tiled weights): SpMMs at most this wide run in
+    a second persistent launch at one float per lane (0 = one launch)."""
 
     def __init__(self, weights, budget_bytes: int = 256 << 20, max_batch: int = 64,
                  slots: int = 3, shape: int = 6, grid: int = 1024, use_graphs: bool = True,
-                 merge_same_weight: bool = True):
+                 merge_same_weight: bool = True, narrow_cols: int = 64):
         if weights is None or len(weights) == 0:
             raise ValueError("no weights")
         if isinstance(weights, RegTileWeights):
-            self.backend = RegTileBackend(weights, grid, merge=merge_same_weight)
+            self.backend = RegTileBackend(weights, grid, merge=merge_same_weight, narrow_cols=narrow_cols)
             self.dtype = "bf16"
         else:
             self.backend = MfmaBackend(weights, shape, grid)

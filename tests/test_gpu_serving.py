@@ -195,8 +195,10 @@ def test_regtile_alone_equals_mixed_and_budget(rt_env, oracle):
         alone.append(r.C.cpu().numpy())
     mixed = [srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B) for r in reqs]
     server.run_batch(list(reversed(mixed)), slot_index=1)
-    # 12 requests over 4 weights: requests of one weight ran merged, one SpMM each
-    assert server.slots[1].hdr[2] == len({r.weight for r in mixed}) < len(mixed)
+    # 12 requests over 4 weights: requests of one weight ran merged, one SpMM
+    # each (in the wide table or the narrow one)
+    hdr = server.slots[1].hdr
+    assert hdr[2] + hdr[4] == len({r.weight for r in mixed}) < len(mixed)
     assert server.pool.used == 0
     for r, a in zip(mixed, alone):
         assert np.array_equal(r.C.cpu().numpy().view(np.uint32), a.view(np.uint32)), "request %d" % r.rid
