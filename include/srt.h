@@ -192,15 +192,15 @@ int srt_plan_create(int m, int k, int n, const float *Lx, const void *Lp, const 
 
 /* A hybrid plan: the 16-row panels whose density (nonzeros / (rows * k))
  * reaches dense_threshold are stored dense and run on FP32 matrix cores
- * (v_mfma_f32_16x16x4_f32) beside the register-tiled launch over the other
+ * (v_mfma_f32_16x16x4_f32) after the register-tiled launch over the other
  * panels; the rest is as srt_plan_create (K tiles, schedules, execute).
- * When M_r divides 16 the register-tiled schedule leaves the dense panels'
- * rows out and execute runs the two launches concurrently: the dense one on
- * the plan's own stream, forked from and joined back to the caller's stream
- * with the plan's two events (graph capture records both branches), so
- * executes of one hybrid plan from several host threads at once must be
- * serialised by the caller; otherwise the dense launch follows and
- * overwrites the rows the register tile wrote as zero.  The
+ * The dense launch follows the register-tiled one (which leaves the dense
+ * panels' rows out when M_r divides 16, else writes them as zero).  It
+ * splits K over up to 32 workgroups per 128 x 128 output tile when that
+ * fills the chip, adding the partials (kept in a plan-owned workspace sized
+ * for the widest n of create / srt_plan_allocate_executor) in split order;
+ * executes of one hybrid plan overlapping in time (several host threads or
+ * streams) must be serialised by the caller: they share the workspace.  The
  * reference marks a tile dense at 0.30 (sbench/padding_reordering.py:36) and
  * pads patterns by its ILP cost model (sbench/ilp_pad/nano_solver.py:76-116);
  * a threshold above 1 gives a plain plan.  Dense panels accumulate in the

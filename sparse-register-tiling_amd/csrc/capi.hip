@@ -346,21 +346,32 @@ struct DensePart {
     int32_t *panel_row = nullptr;   // first row of each dense panel
     std::vector<int32_t> h_panel_row;
     size_t bytes = 0;
-    // Concurrent execution: the register-tiled schedule leaves the dense
-    // panels' rows out, so the two launches write disjoint rows and the dense
-    // one runs on `side`, forked from and joined back to the caller's stream.
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    // K-split partials of the dense launch (sized for the widest B so far;
+    // earlier ones stay alive for graphs that captured them)
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    std::vector<void *> retired_ws;
+    int reserve(int n) {
+        const size_t need = dense_panels_workspace_bytes(ndense, G, n);
+        if (need <= ws_bytes) return SRT_OK;
+        void *w = nullptr;
+        const hipError_t e = hipMalloc(&w, need);
+        if (e != hipSuccess) return fail(SRT_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        if (ws) retired_ws.push_back(ws);
+        ws = w;
+        ws_bytes = need;
+        return SRT_OK;
+    }
     void release() {
         (void)hipFree(apack);
         (void)hipFree(panel_row);
-        if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
-        if (side) (void)hipStreamDestroy(side);
+        (void)hipFree(ws);
+        for (void *w : retired_ws) (void)hipFree(w);
+        retired_ws.clear();
+        ws = nullptr;
+        ws_bytes = 0;
         apack = nullptr;
         panel_row = nullptr;
-        side = nullptr;
-        fork = join = nullptr;
         ndense = 0;
     }
 };
@@ -1388,10 +1399,14 @@ int srt_plan_create_hybrid(int m, int k, int n, const float *Lx, const void *Lp,
     SRT_HIP_TRY(hipMemcpyAsync(dp.apack, ap.data(), sizeof(float) * ap.size(), hipMemcpyHostToDevice, s));
     SRT_HIP_TRY(hipMemcpyAsync(dp.panel_row, dense_rows.data(), sizeof(int32_t) * (size_t)nd, hipMemcpyHostToDevice, s));
     SRT_HIP_TRY(hipStreamSynchronize(s));
+    SRT_TRY(dp.reserve(n));
     // With 16-row dense panels made of whole register-tile panels (M_r
-    // dividing 16) and some rows left to the register tile, the two launches
-    // run concurrently: the register-tiled schedule is rebuilt without the
-    // dense panels' rows (nothing can have captured the first one yet).
+    // dividing 16) and some rows left to the register tile, the register-
+    // tiled schedule is rebuilt without the dense panels' rows (nothing can
+    // have captured the first one yet): the two launches write disjoint rows.
+    // (Run concurrently on two streams they slowed each other down: each
+    // holds whole CUs, so the dense launch stretched from 17 to 63 us and
+    // the register tile from 57 to 114 us on 1024 x 4096 at 70 %.)
     const bool all_dense = (int64_t)nd * 16 >= m;
     Packed &pk = plan->pk;
     if (nd > 0 && !all_dense && 16 % pk.M_r == 0 && plan->cfg.matrix_cores != 1) {
@@ -1404,9 +1419,6 @@ int srt_plan_create_hybrid(int m, int k, int n, const float *Lx, const void *Lp,
                                       plan->retired, plan->cfg, s);
         first.release();
         SRT_TRY(rc);
-        SRT_HIP_TRY(hipStreamCreateWithFlags(&dp.side, hipStreamNonBlocking));
-        SRT_HIP_TRY(hipEventCreateWithFlags(&dp.fork, hipEventDisableTiming));
-        SRT_HIP_TRY(hipEventCreateWithFlags(&dp.join, hipEventDisableTiming));
     }
     plan->pk.nnz = nnz;  // the whole matrix's entries (the register-tiled part holds the others)
     *out = guard.release();
@@ -1438,6 +1450,7 @@ int srt_plan_allocate_executor(srt_plan *p, int n) {
     if (n < 0) return fail(SRT_ERR_INVALID, "n must be >= 0");
     p->n_alloc = n;
     if (n > p->n_max) p->n_max = n;
+    if (p->dense.ndense) SRT_TRY(p->dense.reserve(n));
     // Re-realise the choices that depend on the B width and switch to that
     // schedule (schedules of other widths stay alive: graphs captured with
     // them replay unchanged).  Not concurrently with other calls on the plan.
@@ -1471,24 +1484,14 @@ int srt_plan_execute(const srt_plan *p, float *C, int64_t ldc, const float *B, i
             SRT_HIP_TRY(launch_mfma4_exec(p->pk.col_ptr, p->pk.cols, p->pk.vals, rg.second, rg.first, B, ldb, C,
                                           ldc, p->pk.M, n, p->pk.K, (hipStream_t)stream));
         }
-    } else if (p->dense.side && n > 0) {
-        // dense panels on the side stream, forked from and joined back to the
-        // caller's (graph capture records both branches)
-        const hipStream_t s = (hipStream_t)stream;
-        const DensePart &dp = p->dense;
-        SRT_HIP_TRY(hipEventRecord(dp.fork, s));
-        SRT_HIP_TRY(hipStreamWaitEvent(dp.side, dp.fork, 0));
-        SRT_HIP_TRY(launch_dense_panels(dp.ndense, dp.G, dp.apack, dp.panel_row, B, ldb, C, ldc, p->pk.M, n,
-                                        p->pk.K, dp.side));
-        SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, s));
-        SRT_HIP_TRY(hipEventRecord(dp.join, dp.side));
-        SRT_HIP_TRY(hipStreamWaitEvent(s, dp.join, 0));
-        return SRT_OK;
     } else
         SRT_TRY(run_executor(p->pk, p->sch, p->exec, p->cfg, B, ldb, C, ldc, n, 0, (hipStream_t)stream));
-    if (p->dense.ndense && n > 0)  // the dense panels' rows, zero from the register-tiled part, overwritten
+    // the dense panels' rows (left out by the register-tiled schedule, or
+    // written by it as zero and overwritten here)
+    if (p->dense.ndense && n > 0)
         SRT_HIP_TRY(launch_dense_panels(p->dense.ndense, p->dense.G, p->dense.apack, p->dense.panel_row, B, ldb, C,
-                                        ldc, p->pk.M, n, p->pk.K, (hipStream_t)stream));
+                                        ldc, p->pk.M, n, p->pk.K, p->dense.ws, p->dense.ws_bytes,
+                                        (hipStream_t)stream));
     return SRT_OK;
 }
 

@@ -282,7 +282,10 @@ int mfma4_max_k();  // B rows the LDS slice holds
 
 // Dense 16-row panels of hybrid plans on FP32 matrix cores (dense_panels.hip).
 hipError_t launch_dense_panels(int ndense, int G, const void *apack, const int32_t *panel_row, const float *B,
-                               int64_t ldb, float *C, int64_t ldc, int M, int N, int K, hipStream_t s);
+                               int64_t ldb, float *C, int64_t ldc, int M, int N, int K, void *ws, size_t ws_bytes,
+                               hipStream_t s);
+// Workspace a dense-panel launch of width n splits K with (0: no split).
+size_t dense_panels_workspace_bytes(int ndense, int G, int n);
 
 // CSR baseline (csr_spmm.hip).
 hipError_t launch_csr(int m, int k, int n, int64_t nnz, const void *rowptr, const void *colind,

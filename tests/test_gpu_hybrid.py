@@ -38,9 +38,10 @@ def _dense_panels(A, tau):
                                                (500, 1000, 200, 0.3, (8, "orig")),
                                                (500, 1000, 200, 0.3, (6, "identity"))])
 def test_hybrid_matches_oracle(srt, oracle, M, K, N, tau, mapping):
-    """M_r = 4 and 8 run the two parts concurrently (the register-tiled
-    schedule leaves the dense panels' rows out); M_r = 6 panels straddle the
-    16-row panels, so the dense launch follows and overwrites."""
+    """With M_r = 4 and 8 the register-tiled schedule leaves the dense
+    panels' rows out; M_r = 6 panels straddle the 16-row panels, so the
+    register tile writes those rows as zero and the dense launch overwrites
+    them."""
     import torch
     so, cb = oracle
     print("hybrid %dx%d N=%d tau=%g mapping %s" % (M, K, N, tau, mapping), flush=True)
@@ -96,4 +97,45 @@ def test_hybrid_threshold_above_one_is_plain(srt, oracle):
     C = _run(srt, torch, plan, B)
     ref = cb.exec_packed(cb.pack_dense(A, 4, _oracle_table(so, 4, "orig")), B, nsplit=plan.split_counts())
     assert np.array_equal(C.view(np.uint32), ref.view(np.uint32))
+    plan.close()
+
+
+def test_hybrid_all_dense_split_k_and_width_change(srt, oracle):
+    """Every panel dense (the dense launch splits K over several workgroups
+    per output tile and adds the partials in split order): within the FP64
+    bound, repeatable; a graph captured at one width replays bit-exact after
+    the plan was widened (its K-split workspace grew; the old one stays
+    alive), and the wide launch matches the bound too."""
+    import torch
+    so, cb = oracle
+    M, K = 200, 3000
+    A = srt.workloads.magnitude_pruned(M, K, 0.5, seed=7)
+    B = srt.workloads.dense_b(K, 520, seed=8)
+    rp, ci, v = srt.workloads.to_csr(A)
+    plan = srt.MatMulSpecialized(M, K, 64, torch.from_numpy(v), torch.from_numpy(rp), torch.from_numpy(ci),
+                                 mapping_id=srt.named_mapping(4, "orig"), dense_threshold=0.1)
+    assert plan.dense_panels()["count"] == -(-M // 16)
+    B64 = np.ascontiguousarray(B[:, :64])
+    C = _run(srt, torch, plan, B64)
+    assert np.all(np.abs(C - cb.dense_f64(A, B64)) <= so.fp64_error_bound(A, B64))
+    assert np.array_equal(_run(srt, torch, plan, B64).view(np.uint32), C.view(np.uint32))
+    Bd = torch.from_numpy(B64).cuda()
+    Cd = torch.zeros((M, 64), device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan(Cd, Bd, stream=s)
+    torch.cuda.synchronize()
+    plan.allocate_executor(520)
+    Cw = _run(srt, torch, plan, B)
+    assert np.all(np.abs(Cw - cb.dense_f64(A, B)) <= so.fp64_error_bound(A, B))
+    assert np.array_equal(np.ascontiguousarray(Cw[:, :64]).view(np.uint32), C.view(np.uint32))
+    for _ in range(2):
+        Cd.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(Cd.cpu().numpy().view(np.uint32), C.view(np.uint32))
+    del g
     plan.close()
