@@ -17,7 +17,9 @@
  *     NULL = the default stream).  Calls that must read sizes back (plan
  *     creation, pack_tile) synchronise that stream before returning.
  *   - A plan's packed data is immutable after creation, and concurrent
- *     executes of one plan on different streams are safe.  The launch
+ *     executes of one plan on different streams are safe (except hybrid
+ *     plans, whose dense launch splits K through a plan-owned workspace,
+ *     and the allocation-free MFMA entry point, whose scratch is per GPU).  The launch
  *     schedule may change (srt_plan_allocate_executor, srt_tile_execute with
  *     another configuration), but a schedule once built stays alive until
  *     the plan / tile is destroyed, so a graph captured before the change
