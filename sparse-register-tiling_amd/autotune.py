@@ -453,6 +453,47 @@ def tuned_plan(csr: torch.Tensor, N: int, path: str = SHIPPED) -> MatMulSpeciali
     return MatMulSpecialized.from_csr(csr, N, mapping_id=named_mapping(M_r, name), tile_config=cfg)
 
 
+HYBRID_THRESHOLDS = (0.2, 0.25, 0.3, 0.35, 0.4)
+
+
+def tuned_hybrid(csr: torch.Tensor, N: int, thresholds: Sequence[float] = HYBRID_THRESHOLDS, reps: int = 30,
+                 path: str = SHIPPED) -> Tuple[MatMulSpecialized, Dict[str, object]]:
+    """The fastest of the tuned plain plan and its hybrids (16-row panels at
+    least `threshold` dense on FP32 matrix cores, the rest register-tiled
+    with the plain plan's mapping and configuration), graph-timed on the
+    device at width N.  The reference fixes its dense-tile threshold at 0.30
+    (sbench/padding_reordering.py:36); here each threshold that yields a
+    different set of dense panels is timed and the measurement decides.
+    Returns the plan and a report {"plain_s", "choice": threshold or None,
+    "candidates": [(threshold, dense panels, seconds)]}; the other plans are
+    closed."""
+    M, K = csr.shape
+    dev = torch.device("cuda", torch.cuda.current_device())
+    g = torch.Generator(device="cpu").manual_seed(1)
+    B = (torch.rand((K, N), generator=g) * 2 - 1).to(dev)
+    C = torch.empty((M, N), device=dev)
+    best = tuned_plan(csr, N, path)
+    best_s = time_graph(lambda: best(C, B), reps=reps)
+    report: Dict[str, object] = {"plain_s": best_s, "choice": None, "candidates": []}
+    cfg, mapping = best.get_config(), best.mapping
+    seen = set()
+    for tau in sorted(thresholds):
+        hp = MatMulSpecialized.from_csr(csr, N, mapping_id=mapping, tile_config=cfg, dense_threshold=tau)
+        rows = tuple(hp.dense_panels()["rows"])
+        if not rows or rows in seen:  # no dense panel (the plain plan) or a set already timed
+            hp.close()
+            continue
+        seen.add(rows)
+        t = time_graph(lambda: hp(C, B), reps=reps)
+        report["candidates"].append((tau, len(rows), t))
+        if t < best_s:
+            best.close()
+            best, best_s, report["choice"] = hp, t, tau
+        else:
+            hp.close()
+    return best, report
+
+
 def main(argv=None) -> None:
     """Re-measure the tuning table over the 48 pruned-FFN workloads on the
     current GPU (a few minutes on an MI355X):

@@ -139,3 +139,27 @@ def test_hybrid_all_dense_split_k_and_width_change(srt, oracle):
         assert np.array_equal(Cd.cpu().numpy().view(np.uint32), C.view(np.uint32))
     del g
     plan.close()
+
+
+def test_tuned_hybrid_chooses_by_measurement(srt, oracle):
+    """autotune.tuned_hybrid times the plain plan against its hybrids: at 50 %
+    sparsity the dense panels on matrix cores win by a wide margin (about
+    47 vs 80 us on this shape), at 95 % no panel reaches a threshold and the
+    plain plan is kept; the chosen plan's result is within the FP64 bound."""
+    import importlib
+    import torch
+    so, cb = oracle
+    at = importlib.import_module("sparse-register-tiling_amd.autotune")
+    A = srt.workloads.magnitude_pruned(1024, 4096, 0.5, seed=3)
+    B = srt.workloads.dense_b(4096, 512, seed=4)
+    plan, rep = at.tuned_hybrid(torch.from_numpy(A).to_sparse_csr(), 512)
+    print("tuned hybrid 50 %:", rep, flush=True)
+    assert rep["choice"] is not None and plan.dense_panels()["count"] > 0
+    assert min(t for _, _, t in rep["candidates"]) < 0.8 * rep["plain_s"]
+    C = _run(srt, torch, plan, B)
+    assert np.all(np.abs(C - cb.dense_f64(A, B)) <= so.fp64_error_bound(A, B))
+    plan.close()
+    A2 = srt.workloads.magnitude_pruned(512, 2048, 0.95, seed=5)
+    plan2, rep2 = at.tuned_hybrid(torch.from_numpy(A2).to_sparse_csr(), 256)
+    assert rep2["choice"] is None and rep2["candidates"] == [] and plan2.dense_panels()["count"] == 0
+    plan2.close()
