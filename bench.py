@@ -70,7 +70,8 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--form", default="plain",
-                    choices=["plain", "precision", "serving", "cnn", "sweep", "floor", "hybrid"],
+                    choices=["plain", "precision", "serving", "cnn", "sweep", "floor", "hybrid", "matrix-cores",
+                             "serving-sweep"],
                     help="plain = the headline FP32 workload (default); the others print their own line")
     ap.add_argument("--qps", default="5000,10000,20000,40000,60000,80000,160000",
                     help="serving: offered loads (requests/s)")
@@ -495,6 +496,71 @@ def bench_hybrid(args):
                       "results": rows}), flush=True)
 
 
+def bench_matrix_cores(args):
+    """The full-pattern tile on matrix cores (TileConfig.matrix_cores = 1,
+    mapping 'full' 7da09) against the shipped table's best register-tile
+    record, on the pruned-FFN workloads with K <= 640: graph-timed warm (one
+    operand set) and cold (replicas rotated past the 256 MB cache)."""
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    at = srt.autotune
+    wl = srt.workloads
+    cases = [(2048, 512, 0.9, 512), (512, 512, 0.9, 512), (2048, 512, 0.7, 512), (512, 512, 0.95, 512),
+             (2048, 512, 0.95, 512), (512, 512, 0.7, 128)]
+    rows = []
+    for (M, K, sp, N) in cases:
+        A, B = wl.case_inputs(wl.SpMMCase(M, K, sp, N), seed=0)
+        csr = torch.from_numpy(A).to_sparse_csr()
+        Bd = torch.from_numpy(B).cuda()
+        full = srt.named_mapping(4, "full")
+        cold = at.time_plan_cold(csr, N, full, srt.TileConfig(xcd_remap=1, matrix_cores=1), B=Bd)
+        p = srt.MatMulSpecialized.from_csr(csr, N, mapping_id=full, tile_config=srt.TileConfig(matrix_cores=1))
+        warm = at.time_plan(p, Bd, torch.empty((M, N), device="cuda"), 100)
+        p.close()
+        rec = at.TuningCache(at.SHIPPED).get(M, K, sp, N)[0]
+        rows.append({"M": M, "K": K, "sparsity": sp, "N": N, "matrix_cores_warm_us": round(warm * 1e6, 2),
+                     "matrix_cores_cold_us": round(cold * 1e6, 2), "table_mapping": rec["mapping"],
+                     "table_warm_us": round(rec["seconds"] * 1e6, 2),
+                     "table_cold_us": round(rec["seconds_cold"] * 1e6, 2) if "seconds_cold" in rec else None})
+        log("%dx%d s=%g N=%d  matrix cores warm %.2f us cold %.2f us | table best warm %.2f cold %s (%s)"
+            % (M, K, sp, N, warm * 1e6, cold * 1e6, rec["seconds"] * 1e6,
+               ("%.2f" % (rec["seconds_cold"] * 1e6)) if "seconds_cold" in rec else "-", rec["mapping"]))
+    print(json.dumps({"form": "matrix-cores", "metric": "SpMM us, full-pattern tile on matrix cores vs the table's "
+                      "register tile", "unit": "us", "n_gpus": 1, "data": "synthetic", "results": rows}), flush=True)
+
+
+def bench_serving_sweep(args):
+    """Launch settings of the register-tiled served form on the 48 weights:
+    the narrow launch's width, and slots x batch size, each measured with
+    1200 requests arriving at once (saturation), then 800 at 60k and 80k
+    requests/s."""
+    import torch
+    srt = importlib.import_module("sparse-register-tiling_amd")
+    sv = srt.serving
+    Ws = sv.serving_weights()
+    shapes = [W.shape for W in Ws]
+    w = sv.RegTileWeights(Ws)
+    settings = [dict(narrow_cols=c) for c in (0, 32, 64, 128)] + \
+               [dict(slots=s, max_batch=b) for s, b in ((2, 64), (3, 32), (3, 48), (4, 32), (4, 64), (6, 32))]
+    rows = []
+    for kw in settings:
+        opts = dict(budget_bytes=512 << 20, max_batch=64, slots=3, grid=1024)
+        opts.update(kw)
+        server = sv.Server(w, **opts)
+        sat = server.serve(sv.poisson_requests(1200, 1e9, shapes, seed=99))
+        row = {"setting": kw, "saturation_req_per_s": round(sat["req_per_s"])}
+        for qps in (60000.0, 80000.0):
+            st = server.serve(sv.poisson_requests(800, qps, shapes, seed=int(qps) % 1000 + 1))
+            row["%dk" % (qps / 1000)] = {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                                         "req_per_s": round(st["req_per_s"])}
+        log(json.dumps(row))
+        rows.append(row)
+        del server
+        torch.cuda.empty_cache()
+    print(json.dumps({"form": "serving-sweep", "unit": "req/s", "n_gpus": 1, "data": "synthetic",
+                      "results": rows}), flush=True)
+
+
 def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample=16, form="regtile"):
     """48 resident BF16 weights (12 x {1024x1024 x2, 4096x1024, 1024x4096},
     90 % magnitude-pruned), Poisson requests with log-uniform N in [1, 512]:
@@ -817,6 +883,10 @@ def main():
         return bench_floor(args)
     if args.form == "hybrid":
         return bench_hybrid(args)
+    if args.form == "matrix-cores":
+        return bench_matrix_cores(args)
+    if args.form == "serving-sweep":
+        return bench_serving_sweep(args)
     import numpy as np
     import torch
     srt = importlib.import_module("sparse-register-tiling_amd")
