@@ -151,7 +151,8 @@ def summarize(dirpaths, out, cmd, config=None):
         agg = {}
         for r in pmc:
             agg.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-        lines.append("counters, mean per dispatch (KiB):")
+        if any(r["Counter_Name"] in ("FETCH_SIZE", "WRITE_SIZE") for r in pmc):
+            lines.append("counters, mean per dispatch (KiB):")
         for cn in ("FETCH_SIZE", "WRITE_SIZE"):
             ex = [x for n, d in agg.items() if EXEC_KERNEL in n for x in d.get(cn, [])]
             cands = [(statistics.mean(d[cn]), n) for n, d in agg.items()
@@ -170,6 +171,38 @@ def summarize(dirpaths, out, cmd, config=None):
             if config is not None:
                 lines.append("config: %s" % json.dumps(config, sort_keys=True))
             lines.append("traffic_bytes_per_launch: %.0f" % (result["FETCH_SIZE"] + result["WRITE_SIZE"]))
+        # every other counter of the executor: summed over a dispatch's rows, averaged over dispatches
+        per = {}
+        for d_i, d in enumerate(dirs):
+            for r in _csv_rows(os.path.join(d, "**", "*counter_collection.csv")):
+                if EXEC_KERNEL in r["Kernel_Name"] and r["Counter_Name"] not in ("FETCH_SIZE", "WRITE_SIZE"):
+                    key = (d_i, r["Dispatch_Id"])
+                    per.setdefault(r["Counter_Name"], {}).setdefault(key, 0.0)
+                    per[r["Counter_Name"]][key] += float(r["Counter_Value"])
+        if per:
+            m = {c: statistics.mean(v.values()) for c, v in per.items()}
+            lines.append("executor kernel, mean per dispatch over %d dispatches:" % min(len(v) for v in per.values()))
+            lines += ["  %-30s %14.1f" % (c, m[c]) for c in sorted(m)]
+            w = m.get("SQ_WAVES")
+            derived = []
+            if w and all(c in m for c in ("SQ_INSTS_SALU", "SQ_INSTS_VALU", "SQ_INSTS_BRANCH", "SQ_INSTS_SMEM",
+                                          "SQ_INSTS_VMEM_RD")):
+                derived.append("  per wave: SALU %d, VALU %d, branch %d, SMEM %d, VMEM read %d instructions"
+                               % tuple(round(m[c] / w) for c in ("SQ_INSTS_SALU", "SQ_INSTS_VALU", "SQ_INSTS_BRANCH",
+                                                                  "SQ_INSTS_SMEM", "SQ_INSTS_VMEM_RD")))
+            if "SQ_WAIT_ANY" in m and m.get("SQ_WAVE_CYCLES"):
+                derived.append("  waiting share of wave cycles       %d %% (SQ_WAIT_ANY / SQ_WAVE_CYCLES)"
+                               % round(100 * m["SQ_WAIT_ANY"] / m["SQ_WAVE_CYCLES"]))
+            if "TCP_TCC_READ_REQ_sum" in m and m.get("TCP_TOTAL_CACHE_ACCESSES_sum"):
+                derived.append("  L1 hit rate                        %d %% (1 - TCP_TCC_READ_REQ / TCP_TOTAL_CACHE_ACCESSES)"
+                               % round(100 * (1 - m["TCP_TCC_READ_REQ_sum"] / m["TCP_TOTAL_CACHE_ACCESSES_sum"])))
+            if m.get("TA_BUSY_avr") and "TA_BUSY_max" in m:
+                derived.append("  texture addresser busy             %.1f k cycles average per CU, %.1f k on the busiest "
+                               "(%.2fx)" % (m["TA_BUSY_avr"] / 1e3, m["TA_BUSY_max"] / 1e3, m["TA_BUSY_max"] / m["TA_BUSY_avr"]))
+            if derived:
+                lines += ["", "derived:"] + derived
+            if config is not None and len(result) != 2:
+                lines += ["", "config: %s" % json.dumps(config, sort_keys=True)]
     text = "\n".join(lines) + "\n"
     if out:
         with open(out, "w") as f:
