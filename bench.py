@@ -11,7 +11,7 @@ replayed once between two HIP events on the launch stream, so host launch
 cost is not part of the step; ``ms_per_step`` is therefore the mean launch
 duration plus the dispatcher's gap between consecutive kernels.
 
-Prints one JSON line with the driver's fields plus:
+Prints one JSON line (metric, value, unit, steps, ms_per_step, config, ...) plus:
   roofline      HBM roofline of the executor kernel: algorithmic bytes per
                 launch (nnz*4 + K*N*4 + M*N*4) / mean launch duration over the
                 timed region; ``traffic`` = measured HBM bytes per launch from
