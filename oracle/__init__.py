@@ -2,9 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the shipped package imports this
 directory.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker
-(or, for ``cpu_baseline``, as the timed CPU restatement); the product path
-never routes through it.
+``cpu_baseline`` leg of ``bench.py`` may import it, and only to compare the
+GPU results against (or, for ``cpu_baseline``, as the timed CPU
+restatement); the product path never routes through it.
 
 Contents
 --------
