@@ -65,12 +65,12 @@ def _hybrid_case(i):
     B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
     tau = float(rng.choice([0.05, 0.2, 0.3, 0.5]))
     cfg = dict(vec_width=int(rng.choice([1, 2, 4])), waves_per_block=int(rng.choice([4, 8, 16])),
-               layout=int(rng.choice([1, 2])))
+               layout=int(rng.choice([1, 2])), K_c=int(rng.choice([0, 0, 128, 256])))
     cfg["nsplit"] = int(rng.integers(1, cfg["waves_per_block"] + 1))
     return A, B, MAPPINGS[i % len(MAPPINGS)], tau, cfg
 
 
-@pytest.mark.parametrize("i", range(24))
+@pytest.mark.parametrize("i", range(32))
 def test_random_hybrid(srt, oracle, i):
     """Hybrid plans at random: rows of register-tiled panels bit-identical
     to the oracle's chain over the matrix with the dense panels emptied, the
