@@ -113,6 +113,8 @@ def parse():
     ap.add_argument("--summary-out")
     ap.add_argument("--summary-cmd", default="")
     ap.add_argument("--summary-config", help="JSON line printed by --traffic-probe")
+    ap.add_argument("--summary-timed", metavar="W,K",
+                    help="summarize: the K timed executor launches of a run with W warmup steps (kernel trace)")
     return ap.parse_args()
 
 
@@ -127,15 +129,32 @@ def _csv_rows(pattern):
     return rows
 
 
-def summarize(dirpaths, out, cmd, config=None):
+def summarize(dirpaths, out, cmd, config=None, timed=None):
     """Write a short text summary of rocprofv3 --output-format csv results
     (one or more directories, comma-separated): the kernel-stats table (top
     kernels) and, if counters were collected, the per-dispatch means of every
     counter of the executor and of the copy kernel, with the HBM traffic per
     executor launch calibrated against the copy kernel (COPY_BYTES read and
-    written by one elementwise kernel: the largest elementwise dispatch)."""
+    written by one elementwise kernel: the largest elementwise dispatch).
+    timed = (W, K): from the kernel trace of a plain-form run, the K timed
+    executor launches, which follow the correctness launch and the W warmup
+    launches in dispatch order."""
     lines = ["command: %s" % cmd, ""]
     dirs = dirpaths.split(",")
+    if timed:
+        w, k = timed
+        tr = [r for d in dirs for r in _csv_rows(os.path.join(d, "**", "*kernel_trace.csv"))]
+        ex = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in tr
+                    if r["Kernel_Name"].startswith("void srt::exec_kernel<"))
+        sel = ex[1 + w:1 + w + k]
+        if len(sel) == k and k > 0:
+            dur = [(e - b) / 1e3 for b, e in sel]
+            lines.append("timed executor launches (dispatches %d..%d of %d, after 1 correctness + %d warmup launches):"
+                         % (2 + w, 1 + w + k, len(ex), w))
+            lines.append("  mean %.3f us, median %.3f us, min %.3f us, max %.3f us per launch"
+                         % (statistics.mean(dur), statistics.median(dur), min(dur), max(dur)))
+            lines.append("  first start to last end: %.3f us per launch" % ((sel[-1][1] - sel[0][0]) / 1e3 / k))
+            lines.append("")
     stats = [r for d in dirs for r in _csv_rows(os.path.join(d, "**", "*kernel_stats.csv"))]
     if stats:
         lines.append("kernel stats (rocprofv3 --stats):")
@@ -898,7 +917,8 @@ def main():
         if args.summary_config:
             with open(args.summary_config) as f:
                 config = json.load(f)["config"]
-        summarize(args.summarize, args.summary_out, args.summary_cmd, config)
+        timed = tuple(int(x) for x in args.summary_timed.split(",")) if args.summary_timed else None
+        summarize(args.summarize, args.summary_out, args.summary_cmd, config, timed)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: runs on more than one GPU are not part of this build "
