@@ -101,7 +101,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="plain: leave out the compact served-vs-plain comparison")
-    ap.add_argument("--serving-qps", default="20000,40000,80000,160000",
+    ap.add_argument("--serving-qps", default="20000,40000,50000,60000,80000,160000",
                     help="plain: offered loads of the compact served-vs-plain comparison")
     ap.add_argument("--serving-requests", type=int, default=600,
                     help="plain: requests per load level of the compact served-vs-plain comparison")
