@@ -583,7 +583,8 @@ def bench_matrix_cores(args):
 
 def bench_serving_sweep(args):
     """Launch settings of the register-tiled served form on the 48 weights:
-    the narrow launch's width, and slots x batch size, each measured with
+    the narrow launch's width (with and without the middle launch at two
+    floats per lane), and slots x batch size, each measured with
     1200 requests arriving at once (saturation), then 800 at 60k and 80k
     requests/s."""
     import torch
@@ -592,7 +593,7 @@ def bench_serving_sweep(args):
     Ws = sv.serving_weights()
     shapes = [W.shape for W in Ws]
     w = sv.RegTileWeights(Ws)
-    settings = [dict(narrow_cols=c) for c in (0, 32, 64, 128)] + \
+    settings = [dict(narrow_cols=c) for c in (0, 32, 64, 128)] + [dict(mid_cols=128)] + \
                [dict(slots=s, max_batch=b) for s, b in ((2, 64), (3, 32), (3, 48), (4, 32), (4, 64), (6, 32))]
     rows = []
     for kw in settings:

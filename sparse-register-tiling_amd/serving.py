@@ -213,7 +213,7 @@ class MfmaBackend:
         pp["tile_begin"] = np.cumsum(ptiles) - ptiles
         pp["src_bf16"] = 1
         pp["pad_to"] = 0
-        slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()), 0, 0)
+        slot.hdr[:] = (n, int(ptiles.sum()), n, int(tiles.sum()), 0, 0, 0, 0)
 
     def enqueue(self, slot: "_Slot", s: ct.c_void_p) -> None:
         _lib.call("srt_mfma_grouped_b_prep", self.code, ct.c_void_p(slot.d_prep.data_ptr()),
@@ -290,7 +290,8 @@ class RegTileBackend:
     prob_dtype = EXEC_PROBLEM_DTYPE
     V = 4
 
-    def __init__(self, weights: RegTileWeights, grid: int, merge: bool = True, narrow_cols: int = 64):
+    def __init__(self, weights: RegTileWeights, grid: int, merge: bool = True, narrow_cols: int = 64,
+                 mid_cols: int = 0):
         self.weights = weights
         self.plans = weights.plans
         self.grid = int(grid)
@@ -299,6 +300,12 @@ class RegTileBackend:
         # float per lane (a wave then spans 64 columns, not 256: a narrow
         # request's B gathers and FMAs shrink 4x); 0 = one launch
         self.narrow_cols = int(narrow_cols)
+        # SpMMs wider than narrow_cols and at most mid_cols wide run in a third
+        # launch at two floats per lane: one pass over the weight's packed
+        # columns, half the B bytes a 4-float pass gathers.  0 = no third
+        # launch (the default: on the 48-weight mix a third launch at 128
+        # saturated at 62.8k requests/s against 64.8k with two)
+        self.mid_cols = int(mid_cols)
         self.wM = np.array([p.m for p in self.plans], dtype=np.int64)
         self.wK = np.array([p.k for p in self.plans], dtype=np.int64)
 
@@ -397,12 +404,15 @@ class RegTileBackend:
         slot.hdr[:] = (m, int(wtiles.sum())) + spmm
 
     def _spmm_tables(self, slot: "_Slot", wi, N, ld, bt, cp):
-        """SpMM records of a batch: wider than narrow_cols into slot.prob (4
-        floats per lane), the others into slot.prob2 (1 float per lane);
-        returns the two headers {problems, workgroups}."""
-        narrow = N <= self.narrow_cols if self.narrow_cols > 0 else np.zeros(len(N), dtype=bool)
+        """SpMM records of a batch: at most narrow_cols wide into slot.prob2
+        (1 float per lane), wider and at most mid_cols into slot.prob3 (2
+        floats per lane), the rest into slot.prob (4 floats per lane); returns
+        the three headers {problems, workgroups}: wide, narrow, middle."""
+        none = np.zeros(len(N), dtype=bool)
+        narrow = N <= self.narrow_cols if self.narrow_cols > 0 else none
+        mid = ~narrow & (N <= self.mid_cols) if self.mid_cols > 0 else none
         out = ()
-        for table, sel, V in ((slot.prob, ~narrow, self.V), (slot.prob2, narrow, 1)):
+        for table, sel, V in ((slot.prob, ~(narrow | mid), self.V), (slot.prob2, narrow, 1), (slot.prob3, mid, 2)):
             if table is None:
                 out += (0, 0)
                 continue
@@ -448,6 +458,9 @@ class RegTileBackend:
         if slot.prob2 is not None:
             _lib.call("srt_exec_persistent_execute", self.plans[0]._h, 1, ct.c_void_p(slot.d_prob2.data_ptr()),
                       ct.c_void_p(slot.d_hdr.data_ptr() + 16), self.grid, s)
+        if slot.prob3 is not None:
+            _lib.call("srt_exec_persistent_execute", self.plans[0]._h, 2, ct.c_void_p(slot.d_prob3.data_ptr()),
+                      ct.c_void_p(slot.d_hdr.data_ptr() + 24), self.grid, s)
 
 
 class _Slot:
@@ -457,17 +470,23 @@ class _Slot:
         pdt = backend.prob_dtype
         self.h_prob = torch.zeros(max_batch * pdt.itemsize, dtype=torch.uint8).pin_memory()
         self.h_prep = torch.zeros(max_batch * BPREP_DTYPE.itemsize, dtype=torch.uint8).pin_memory()
-        # headers: prep {n, tiles}, spmm {n, tiles}, narrow spmm {n, tiles}
-        self.h_hdr = torch.zeros(6, dtype=torch.int32).pin_memory()
+        # headers: prep {n, tiles}, spmm {n, tiles}, narrow spmm {n, tiles},
+        # middle spmm {n, tiles}
+        self.h_hdr = torch.zeros(8, dtype=torch.int32).pin_memory()
         self.d_prob = torch.zeros_like(self.h_prob, device=device)
         self.d_prep = torch.zeros_like(self.h_prep, device=device)
-        self.d_hdr = torch.zeros(6, dtype=torch.int32, device=device)
+        self.d_hdr = torch.zeros(8, dtype=torch.int32, device=device)
         self.prob = np.frombuffer(self.h_prob.numpy(), dtype=pdt)
         self.h_prob2 = self.d_prob2 = self.prob2 = None
         if getattr(backend, "narrow_cols", 0) > 0:  # a second SpMM table (narrow problems)
             self.h_prob2 = torch.zeros_like(self.h_prob).pin_memory()
             self.d_prob2 = torch.zeros_like(self.h_prob, device=device)
             self.prob2 = np.frombuffer(self.h_prob2.numpy(), dtype=pdt)
+        self.h_prob3 = self.d_prob3 = self.prob3 = None
+        if getattr(backend, "mid_cols", 0) > 0:  # a third SpMM table (middle widths)
+            self.h_prob3 = torch.zeros_like(self.h_prob).pin_memory()
+            self.d_prob3 = torch.zeros_like(self.h_prob, device=device)
+            self.prob3 = np.frombuffer(self.h_prob3.numpy(), dtype=pdt)
         self.prep = np.frombuffer(self.h_prep.numpy(), dtype=BPREP_DTYPE)
         self.hdr = self.h_hdr.numpy()
         self.graph: Optional[torch.cuda.CUDAGraph] = None
@@ -479,6 +498,8 @@ class _Slot:
         self.d_prob.copy_(self.h_prob, non_blocking=True)
         if self.d_prob2 is not None:
             self.d_prob2.copy_(self.h_prob2, non_blocking=True)
+        if self.d_prob3 is not None:
+            self.d_prob3.copy_(self.h_prob3, non_blocking=True)
         self.d_prep.copy_(self.h_prep, non_blocking=True)
         self.d_hdr.copy_(self.h_hdr, non_blocking=True)
         self.backend.enqueue(self, s)
@@ -510,15 +531,18 @@ class Server:
     list of ``PanelPlan`` (MFMA panels).  ``grid``: workgroups of the
     persistent launches; ``shape``: MFMA launch shape (MFMA weights only);
     ``narrow_cols`` (register-tiled weights): SpMMs at most this wide run in
-    a second persistent launch at one float per lane (0 = one launch)."""
+    a second persistent launch at one float per lane (0 = none);
+    ``mid_cols``: wider SpMMs up to this width run in a third at two floats
+    per lane (0 = none)."""
 
     def __init__(self, weights, budget_bytes: int = 256 << 20, max_batch: int = 64,
                  slots: int = 3, shape: int = 6, grid: int = 1024, use_graphs: bool = True,
-                 merge_same_weight: bool = True, narrow_cols: int = 64):
+                 merge_same_weight: bool = True, narrow_cols: int = 64, mid_cols: int = 0):
         if weights is None or len(weights) == 0:
             raise ValueError("no weights")
         if isinstance(weights, RegTileWeights):
-            self.backend = RegTileBackend(weights, grid, merge=merge_same_weight, narrow_cols=narrow_cols)
+            self.backend = RegTileBackend(weights, grid, merge=merge_same_weight, narrow_cols=narrow_cols,
+                                          mid_cols=mid_cols)
             self.dtype = "bf16"
         else:
             self.backend = MfmaBackend(weights, shape, grid)

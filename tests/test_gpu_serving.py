@@ -184,6 +184,33 @@ def test_regtile_served_results_exact(rt_env, oracle, graphs):
     assert server.pool.used == 0
 
 
+def test_regtile_one_spmm_in_each_launch(rt_env, oracle):
+    """With the middle launch on (mid_cols = 128), a batch with one SpMM per
+    table: 20 columns (narrow, 1 float per lane), 100 (middle, 2 floats per
+    lane) and 200 (wide, 4 floats per lane), on three weights; every result
+    is the oracle's chain bit for bit, and each request alone gives the same
+    bits."""
+    srt, torch, shapes, Ws, weights = rt_env
+    so, cb = oracle
+    _log("regtile narrow / middle / wide launches")
+    server = srt.serving.Server(weights, budget_bytes=64 << 20, max_batch=8, slots=2, grid=128, mid_cols=128)
+    g = torch.Generator().manual_seed(21)
+    reqs = [srt.serving.Request(rid=i, weight=w, B=(torch.rand((shapes[w][1], n), generator=g) * 2 - 1)
+                                .to(torch.bfloat16).cuda())
+            for i, (w, n) in enumerate(((0, 20), (1, 100), (3, 200)))]
+    server.run_batch(reqs, slot_index=0)
+    hdr = [int(x) for x in server.slots[0].hdr]
+    assert (hdr[2], hdr[4], hdr[6]) == (1, 1, 1)
+    for r in reqs:
+        got = r.C.cpu().numpy()
+        ref = _oracle_served(so, cb, Ws[r.weight], r.B, weights[r.weight])
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "request %d" % r.rid
+        alone = srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B)
+        server.run_batch([alone], slot_index=1)
+        assert np.array_equal(alone.C.cpu().numpy().view(np.uint32), got.view(np.uint32)), "request %d" % r.rid
+    assert server.pool.used == 0
+
+
 def test_regtile_alone_equals_mixed_and_budget(rt_env, oracle):
     srt, torch, shapes, Ws, weights = rt_env
     so, cb = oracle
@@ -196,9 +223,9 @@ def test_regtile_alone_equals_mixed_and_budget(rt_env, oracle):
     mixed = [srt.serving.Request(rid=r.rid, weight=r.weight, B=r.B) for r in reqs]
     server.run_batch(list(reversed(mixed)), slot_index=1)
     # 12 requests over 4 weights: requests of one weight ran merged, one SpMM
-    # each (in the wide table or the narrow one)
+    # each (in the wide, narrow or middle table)
     hdr = server.slots[1].hdr
-    assert hdr[2] + hdr[4] == len({r.weight for r in mixed}) < len(mixed)
+    assert hdr[2] + hdr[4] + hdr[6] == len({r.weight for r in mixed}) < len(mixed)
     assert server.pool.used == 0
     for r, a in zip(mixed, alone):
         assert np.array_equal(r.C.cpu().numpy().view(np.uint32), a.view(np.uint32)), "request %d" % r.rid

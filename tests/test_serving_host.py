@@ -1,7 +1,8 @@
 """Host logic of the register-tiled serving form, on the CPU: the slab
 budget, merging the requests of one weight into one SpMM, the split of a
-batch's SpMMs between the wide table (4 floats per lane) and the narrow one
-(1 float per lane), and what happens when the budget is full.  The backend's
+batch's SpMMs between the wide table (4 floats per lane), the narrow one
+(1 float per lane) and the middle one (2 floats per lane), and what happens
+when the budget is full.  The backend's
 per-weight records are stand-ins (no plans are built); the tables and the
 slab arithmetic are the ones the GPU launches read."""
 import numpy as np
@@ -9,11 +10,11 @@ import pytest
 import torch
 
 
-def _backend(sv, shapes, wblocks):
+def _backend(sv, shapes, wblocks, mid_cols=0):
     """A RegTileBackend over stand-in weights: (M, K) per weight and its
     workgroups per N tile."""
     be = object.__new__(sv.RegTileBackend)
-    be.merge, be.narrow_cols, be.grid = True, 64, 64
+    be.merge, be.narrow_cols, be.mid_cols, be.grid = True, 64, mid_cols, 64
     be.wM = np.array([m for m, _ in shapes], dtype=np.int64)
     be.wK = np.array([k for _, k in shapes], dtype=np.int64)
     be.wblocks = np.array(wblocks, dtype=np.int64)
@@ -28,8 +29,9 @@ class _Slot:
     def __init__(self, sv, n):
         self.prob = np.zeros(n, dtype=sv.EXEC_PROBLEM_DTYPE)
         self.prob2 = np.zeros(n, dtype=sv.EXEC_PROBLEM_DTYPE)
+        self.prob3 = np.zeros(n, dtype=sv.EXEC_PROBLEM_DTYPE)
         self.prep = np.zeros(n, dtype=sv.BPREP_DTYPE)
-        self.hdr = np.zeros(6, dtype=np.int32)
+        self.hdr = np.zeros(8, dtype=np.int32)
 
 
 def _server(sv, be, budget):
@@ -58,7 +60,8 @@ def test_merge_and_narrow_split(sv):
         assert srv.admit(r)
     slot = _Slot(sv, 8)
     be.fill_batch(srv, slot, reqs)
-    m, tiles, nw, bw, nn, bn = (int(x) for x in slot.hdr)
+    m, tiles, nw, bw, nn, bn, nm, bm = (int(x) for x in slot.hdr)
+    assert (nm, bm) == (0, 0)   # no middle table here
     # weight 0: 10 + 30 columns, padded to 12 + 32 = 44 <= 64: one narrow SpMM;
     # weight 1: 100 + 5 -> 100 + 8 = 108 > 64: one wide SpMM; weight 2 alone (300, wide)
     assert (nw, nn, m) == (2, 1, 5)
@@ -98,7 +101,8 @@ def test_full_budget_merges_in_the_members_blocks(sv):
     assert srv.pool.used == 32 << 10 and srv.pool.alloc(1) is None
     slot = _Slot(sv, 4)
     be.fill_batch(srv, slot, [a, b])
-    assert int(slot.hdr[2]) + int(slot.hdr[4]) == 1 and a._group is b._group and a._group is not None
+    assert int(slot.hdr[2]) + int(slot.hdr[4]) + int(slot.hdr[6]) == 1 and a._group is b._group
+    assert a._group is not None
     assert a._off == b._off == -1 and srv.pool.live == 1 and srv.pool.used == 32 << 10
     srv.release(a)
     assert srv.pool.live == 1          # the merged block lives until its last member is released
@@ -118,10 +122,38 @@ def test_members_run_alone_when_the_merged_block_cannot_fit(sv):
         assert srv.admit(r)                # three 16 KiB blocks: the slab is full
     slot = _Slot(sv, 4)
     be.fill_batch(srv, slot, [other, a, b])
-    assert int(slot.hdr[2]) + int(slot.hdr[4]) == 3 and a._group is None and b._group is None
+    assert int(slot.hdr[2]) + int(slot.hdr[4]) + int(slot.hdr[6]) == 3 and a._group is None and b._group is None
     assert a._off >= 0 and b._off >= 0 and srv.pool.live == 3
     rows = slot.prob2[:int(slot.hdr[4])]
     assert sorted(int(x) for x in rows["B"]) == sorted(srv.pool.base + r._bt_off for r in (other, a, b))
     for r in (other, a, b):
         srv.release(r)
     assert srv.pool.used == 0
+
+
+def test_middle_widths_get_their_own_table(sv):
+    """With mid_cols = 128: SpMMs of 65..128 columns go to the middle table
+    (2 floats per lane, 128-column tiles), narrower ones to the narrow table,
+    wider ones to the wide table."""
+    shapes = [(48, 32), (16, 64), (40, 24)]
+    be = _backend(sv, shapes, [3, 5, 2], mid_cols=128)
+    srv = _server(sv, be, 4 << 20)
+    reqs = [_req(sv, 0, 0, 32, 10), _req(sv, 1, 1, 64, 100), _req(sv, 2, 0, 32, 30), _req(sv, 3, 1, 64, 5),
+            _req(sv, 4, 2, 24, 300), _req(sv, 5, 2, 24, 130)]
+    for r in reqs:
+        assert srv.admit(r)
+    slot = _Slot(sv, 8)
+    be.fill_batch(srv, slot, reqs)
+    m, tiles, nw, bw, nn, bn, nm, bm = (int(x) for x in slot.hdr)
+    # weight 0: 12 + 32 = 44 columns (narrow); weight 1: 100 + 8 = 108 (middle);
+    # weight 2: 300 + 132 = 432 (wide)
+    assert (nw, nn, nm, m) == (1, 1, 1, 6)
+    assert int(slot.prob[0]["M"]) == 40 and int(slot.prob[0]["N"]) == 432
+    assert int(slot.prob2[0]["M"]) == 48 and int(slot.prob2[0]["N"]) == 44
+    assert int(slot.prob3[0]["M"]) == 16 and int(slot.prob3[0]["N"]) == 108
+    # workgroups: n_wblocks x N tiles of 256, 64 and 128 columns
+    assert (bw, bn, bm) == (2 * 2, 3 * 1, 5 * 1)
+    assert int(slot.prob3[0]["block_begin"]) == 0
+    for r in reqs:
+        srv.release(r)
+    assert srv.pool.used == 0 and srv.pool.live == 0
