@@ -639,7 +639,9 @@ def serving_comparison(qps_levels, requests, slo_ms, mfma_shape, accuracy_sample
         plans32.append(srt.MatMulSpecialized.from_csr(csr, 512, mapping_id=srt.named_mapping(4, "orig"),
                                                       tile_config=srt.TileConfig(xcd_remap=1)))
     log("packed 48 weights in %.1f s" % (time.perf_counter() - t0))
-    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=3, shape=mfma_shape, grid=grid)
+    # 4 slots: the best p99 at 60k requests/s in two launch-setting sweeps
+    # (--form serving-sweep)
+    server = sv.Server(panels, budget_bytes=512 << 20, max_batch=64, slots=4, shape=mfma_shape, grid=grid)
     levels = []
     for qps in qps_levels:
         reqs = sv.poisson_requests(requests, qps, shapes, seed=int(qps) % 1000 + 1)
